@@ -1,0 +1,58 @@
+"""Build libautoplace_gp.so (the HIP kernels and the C ABI) in-tree for gfx950.
+
+    python -m autoplace_amd.build_lib            # or __graft_entry__.build()
+
+hipcc compiles each csrc/*.hip for --offload-arch=gfx950 (packed-FP32 instructions disabled) and links one shared
+library next to this file. No GPU is needed to build.
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libautoplace_gp.so")
+SOURCES = ["gp_kernels.hip", "gp_poisson.hip", "gp_fft.hip", "gp_engine.hip", "gp_batched.hip", "gp_mlp.hip"]
+ARCH = os.environ.get("AUTOPLACE_ARCH", "gfx950")
+FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
+         "-packed-fp32-ops", "-fvisibility=hidden", "-Wall", "-Wno-unused-function"]
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found (ROCm is needed to build the HIP library)")
+
+
+def build(verbose=False):
+    """Compile every HIP source and link the library (always rebuilds). -> path of the library."""
+    hipcc = _hipcc()
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with tempfile.TemporaryDirectory() as tmp:
+        def compile_one(src):
+            obj = os.path.join(tmp, src.replace(".hip", ".o"))
+            cmd = [hipcc] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError("hipcc failed on %s:\n%s" % (src, r.stderr))
+            if verbose and r.stderr.strip():
+                print(r.stderr, file=sys.stderr)
+            return obj
+
+        with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+            objs = list(ex.map(compile_one, srcs))
+        out = LIB + ".tmp"
+        cmd = [hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH] + objs + ["-o", out]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("link failed:\n%s" % r.stderr)
+        os.replace(out, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

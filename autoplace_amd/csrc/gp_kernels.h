@@ -1,0 +1,45 @@
+// Kernel argument blocks shared by the drop-in entry points and the GP engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ap {
+
+constexpr int WA_BLOCK = 256;   // lanes per WA workgroup
+constexpr int WA_SPAN = 256;    // a workgroup owns the nets whose first pin is in its 256-pin window
+constexpr int WA_CAP = 512;     // pins staged in LDS at once
+
+struct WaArgs {
+    const float2* pos;
+    const int* pin_cell;
+    const float2* pin_off;
+    const int* net_start;
+    int num_nets, num_mov;
+    const float* gamma;
+    float2* pin_grad;
+    float2* out;        // per-net (WA, HPWL), may be null
+    double* partial;    // per-workgroup (WA, HPWL) in FP64, may be null
+    const float2* pos_other;   // with flag: the position is pos_other when *flag == 0, else pos
+    const int* flag;           // null: pos as given
+};
+
+struct DgradArgs {
+    const float2* pos;
+    const float2* size;
+    const float* ratio;
+    const float* field_x;
+    const float* field_y;
+    long long ld;       // row stride of the field planes
+    const float2* wl_grad;
+    const float* pins;
+    const float* area;
+    int num_nodes, num_wl;
+    const float* weight;
+    const float* precond;
+    float inv_bx, inv_by;
+    int nbx, nby;
+    float2* grad;
+};
+
+int launch_wa(const WaArgs& a, int num_pins, hipStream_t s);
+
+}  // namespace ap

@@ -1,0 +1,472 @@
+// HIP kernels of the GP iteration for gfx950 and the drop-in C ABI (include/autoplace_gp.h, part 1).
+//
+//   WA        wa_net_kernel: a workgroup owns the nets whose first pin falls in its 256-pin window. Pin loads
+//             (cell, offset, position gather) are one per lane, coalesced; the pins are staged in LDS; one lane per
+//             net reduces max / min and the exponential sums over its pins in pin order (the same order as a
+//             sequential pass, so the per-net values are deterministic); one lane per pin writes its gradient.
+//             Block partial sums of (WA, HPWL) in FP64. wa_cell_kernel: per-cell sum over the cell CSR.
+//   density   integer fixed-point scatter (2^32 per bin area) with 64-bit atomics: order-free, hence deterministic
+//             and bit-identical to any other order; post kernels turn the maps into densities.
+//   gather    one lane per node: field at the node's bins, weight, WA gradient, preconditioner.
+//   control   one lane, double precision, the decision order of the reference's control kernel.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "gp_common.h"
+#include "gp_control.h"
+#include "gp_kernels.h"
+
+namespace ap {
+
+static thread_local char g_err[512];
+
+void set_error(const char* fmt, ...) {
+    va_list ap_;
+    va_start(ap_, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap_);
+    va_end(ap_);
+}
+
+// ------------------------------------------------------------------------------------------- WA wirelength
+
+// smallest i in [lo, hi) with a[i] >= key (hi if none)
+__device__ __forceinline__ int lower_bound(const int* a, int lo, int hi, int key) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float2 pin_xy(const float2* pos, const int* pin_cell, const float2* pin_off, int p,
+                                         int num_mov) {
+    int c = pin_cell[p];
+    float2 o = pin_off[p];
+    if (c < num_mov) {
+        float2 q = pos[c];
+        o.x += q.x;
+        o.y += q.y;
+    }
+    return o;
+}
+
+// One net whose pins do not fit the LDS stage: the whole workgroup loops over its pins (three passes over global
+// memory), with fixed-order block reductions. Rare (degree > WA_CAP).
+__device__ void wa_big_net(const WaArgs& a, int net, double& wa_acc, double& hpwl_acc, float* red /* [4*256] */) {
+    const int p0 = a.net_start[net], p1 = a.net_start[net + 1], t = threadIdx.x, nt = blockDim.x;
+    const float ig = 1.0f / *a.gamma;
+    float hx = -1e30f, hy = -1e30f, lx = 1e30f, ly = 1e30f;
+    for (int p = p0 + t; p < p1; p += nt) {
+        float2 x = pin_xy(a.pos, a.pin_cell, a.pin_off, p, a.num_mov);
+        hx = fmaxf(hx, x.x); hy = fmaxf(hy, x.y); lx = fminf(lx, x.x); ly = fminf(ly, x.y);
+    }
+    red[t] = hx; red[nt + t] = hy; red[2 * nt + t] = lx; red[3 * nt + t] = ly;
+    __syncthreads();
+    for (int s = nt / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            red[t] = fmaxf(red[t], red[t + s]);
+            red[nt + t] = fmaxf(red[nt + t], red[nt + t + s]);
+            red[2 * nt + t] = fminf(red[2 * nt + t], red[2 * nt + t + s]);
+            red[3 * nt + t] = fminf(red[3 * nt + t], red[3 * nt + t + s]);
+        }
+        __syncthreads();
+    }
+    hx = red[0]; hy = red[nt]; lx = red[2 * nt]; ly = red[3 * nt];
+    __syncthreads();
+    float sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // sp.x sp.y sn.x sn.y (x ep).x (x ep).y (x en).x (x en).y
+    for (int p = p0 + t; p < p1; p += nt) {
+        float2 x = pin_xy(a.pos, a.pin_cell, a.pin_off, p, a.num_mov);
+        float epx = expf((x.x - hx) * ig), epy = expf((x.y - hy) * ig);
+        float enx = expf((lx - x.x) * ig), eny = expf((ly - x.y) * ig);
+        sums[0] += epx; sums[1] += epy; sums[2] += enx; sums[3] += eny;
+        sums[4] += x.x * epx; sums[5] += x.y * epy; sums[6] += x.x * enx; sums[7] += x.y * eny;
+    }
+    float tot[8];
+    for (int k = 0; k < 8; k += 4) {
+        for (int j = 0; j < 4; j++) red[j * nt + t] = sums[k + j];
+        __syncthreads();
+        for (int s = nt / 2; s > 0; s >>= 1) {
+            if (t < s)
+                for (int j = 0; j < 4; j++) red[j * nt + t] += red[j * nt + t + s];
+            __syncthreads();
+        }
+        for (int j = 0; j < 4; j++) tot[k + j] = red[j * nt];
+        __syncthreads();
+    }
+    const float xpx = tot[4] / tot[0], xpy = tot[5] / tot[1], xnx = tot[6] / tot[2], xny = tot[7] / tot[3];
+    for (int p = p0 + t; p < p1; p += nt) {
+        float2 x = pin_xy(a.pos, a.pin_cell, a.pin_off, p, a.num_mov);
+        float epx = expf((x.x - hx) * ig), epy = expf((x.y - hy) * ig);
+        float enx = expf((lx - x.x) * ig), eny = expf((ly - x.y) * ig);
+        float2 gr;
+        gr.x = epx * (1 + (x.x - xpx) * ig) / tot[0] - enx * (1 - (x.x - xnx) * ig) / tot[2];
+        gr.y = epy * (1 + (x.y - xpy) * ig) / tot[1] - eny * (1 - (x.y - xny) * ig) / tot[3];
+        a.pin_grad[p] = gr;
+    }
+    if (t == 0) {
+        float wa = (xpx - xnx) + (xpy - xny), hp = (hx - lx) + (hy - ly);
+        if (a.out) a.out[net] = make_float2(wa, hp);
+        wa_acc += wa;
+        hpwl_acc += hp;
+    }
+}
+
+__global__ __launch_bounds__(WA_BLOCK) void wa_net_kernel(WaArgs a) {
+    if (a.flag && *a.flag == 0) a.pos = a.pos_other;
+    __shared__ float2 sx[WA_CAP];       // pin coordinates
+    __shared__ float2 sep[WA_CAP];      // e^{(x - hi) / gamma}
+    __shared__ float2 sen[WA_CAP];      // e^{(lo - x) / gamma}
+    __shared__ int snet[WA_CAP];        // local net of the pin
+    __shared__ float4 sacc[WA_CAP];     // per net: sp.x, sp.y, sn.x, sn.y
+    __shared__ float4 smean[WA_CAP];    // per net: xp.x, xp.y, xn.x, xn.y
+    __shared__ int sbound[2];
+    __shared__ double sred[16 * 2];
+
+    const int t = threadIdx.x;
+    const long long win0 = (long long)blockIdx.x * WA_SPAN;
+    if (t < 2) {
+        long long key = win0 + (t ? WA_SPAN : 0);
+        int k = key > 0x7fffffff ? 0x7fffffff : (int)key;
+        sbound[t] = lower_bound(a.net_start, 0, a.num_nets, k);
+    }
+    __syncthreads();
+    const int net_lo = sbound[0], net_hi = sbound[1];
+    const float ig = 1.0f / *a.gamma;
+    double wa_acc = 0, hpwl_acc = 0;
+
+    int net_cur = net_lo;
+    while (net_cur < net_hi) {
+        const int base = a.net_start[net_cur];
+        // nets [net_cur, net_stop) whose pins fit the stage
+        __syncthreads();
+        if (t == 0) {
+            // largest stop in [net_cur, min(net_hi, net_cur + WA_CAP)] with pins <= WA_CAP (the cap on nets keeps
+            // the per-net LDS arrays in bounds when nets are empty)
+            int lo = net_cur + 1, hi = min(net_hi, net_cur + WA_CAP) + 1;
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (a.net_start[mid] - base <= WA_CAP) lo = mid + 1;
+                else hi = mid;
+            }
+            sbound[0] = lo - 1;
+        }
+        __syncthreads();
+        const int net_stop = sbound[0];
+        if (net_stop == net_cur) {            // a single net larger than the stage
+            wa_big_net(a, net_cur, wa_acc, hpwl_acc, (float*)sx);
+            __syncthreads();
+            net_cur++;
+            continue;
+        }
+        const int npin = a.net_start[net_stop] - base, nnet = net_stop - net_cur;
+        for (int q = t; q < npin; q += WA_BLOCK) sx[q] = pin_xy(a.pos, a.pin_cell, a.pin_off, base + q, a.num_mov);
+        __syncthreads();
+        for (int j = t; j < nnet; j += WA_BLOCK) {
+            const int s = a.net_start[net_cur + j] - base, e = a.net_start[net_cur + j + 1] - base;
+            float hx = -1e30f, hy = -1e30f, lx = 1e30f, ly = 1e30f;
+            for (int q = s; q < e; q++) {
+                float2 x = sx[q];
+                hx = fmaxf(hx, x.x); hy = fmaxf(hy, x.y); lx = fminf(lx, x.x); ly = fminf(ly, x.y);
+            }
+            float spx = 0, spy = 0, snx = 0, sny = 0, xpx = 0, xpy = 0, xnx = 0, xny = 0;
+            for (int q = s; q < e; q++) {
+                float2 x = sx[q];
+                float epx = expf((x.x - hx) * ig), epy = expf((x.y - hy) * ig);
+                float enx = expf((lx - x.x) * ig), eny = expf((ly - x.y) * ig);
+                sep[q] = make_float2(epx, epy);
+                sen[q] = make_float2(enx, eny);
+                snet[q] = j;
+                spx += epx; spy += epy; snx += enx; sny += eny;
+                xpx += x.x * epx; xpy += x.y * epy; xnx += x.x * enx; xny += x.y * eny;
+            }
+            if (e > s) {
+                xpx /= spx; xpy /= spy; xnx /= snx; xny /= sny;
+                sacc[j] = make_float4(spx, spy, snx, sny);
+                smean[j] = make_float4(xpx, xpy, xnx, xny);
+                float wa = (xpx - xnx) + (xpy - xny), hp = (hx - lx) + (hy - ly);
+                if (a.out) a.out[net_cur + j] = make_float2(wa, hp);
+                wa_acc += wa;
+                hpwl_acc += hp;
+            } else if (a.out) {
+                a.out[net_cur + j] = make_float2(0.f, 0.f);
+            }
+        }
+        __syncthreads();
+        for (int q = t; q < npin; q += WA_BLOCK) {
+            const int j = snet[q];
+            const float2 x = sx[q], ep = sep[q], en = sen[q];
+            const float4 acc = sacc[j], m = smean[j];
+            float2 gr;
+            gr.x = ep.x * (1 + (x.x - m.x) * ig) / acc.x - en.x * (1 - (x.x - m.z) * ig) / acc.z;
+            gr.y = ep.y * (1 + (x.y - m.y) * ig) / acc.y - en.y * (1 - (x.y - m.w) * ig) / acc.w;
+            a.pin_grad[base + q] = gr;
+        }
+        net_cur = net_stop;
+    }
+    if (a.partial) {
+        double v[2] = {wa_acc, hpwl_acc};
+        block_sum<2>(v, sred);
+        if (t == 0) {
+            a.partial[2 * blockIdx.x] = v[0];
+            a.partial[2 * blockIdx.x + 1] = v[1];
+        }
+    }
+}
+
+// grad[c] = sum of the pin gradients of cell c, in cell_pin order
+__global__ __launch_bounds__(256) void wa_cell_kernel(const float2* pin_grad, const int* cell_start,
+                                                      const int* cell_pin, int num_mov, float2* grad) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= num_mov) return;
+    float gx = 0, gy = 0;
+    for (int k = cell_start[c], e = cell_start[c + 1]; k < e; k++) {
+        float2 g = pin_grad[cell_pin[k]];
+        gx += g.x;
+        gy += g.y;
+    }
+    grad[c] = make_float2(gx, gy);
+}
+
+int launch_wa(const WaArgs& a, int num_pins, hipStream_t s) {
+    if (a.num_nets <= 0) return AP_OK;
+    unsigned grid = grid_of((long long)num_pins + 1, WA_SPAN);   // +1: a trailing empty net still has an owner
+    hipLaunchKernelGGL(wa_net_kernel, dim3(grid), dim3(WA_BLOCK), 0, s, a);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+// ------------------------------------------------------------------------------------------- density map
+
+__global__ __launch_bounds__(256) void density_map_kernel(const float2* pos, const float2* size, const float* ratio,
+                                                          int num_nodes, int num_mov, float inv_bx, float inv_by,
+                                                          int nbx, int nby, unsigned long long* map) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_nodes) return;
+    const float2 p = pos[i], sz = size[i];
+    const float r = ratio[i];
+    Span sx = make_span(p.x, sz.x, inv_bx, nbx), sy = make_span(p.y, sz.y, inv_by, nby);
+    unsigned long long* m = map + (i < num_mov ? 0 : (long long)nbx * nby);
+    for (int bx = sx.b0; bx <= sx.b1; bx++) {
+        float wx = r * span_overlap(sx, bx);
+        unsigned long long* row = m + (long long)bx * nby;
+        for (int by = sy.b0; by <= sy.b1; by++) {
+            unsigned long long q = (unsigned long long)((double)(wx * span_overlap(sy, by)) * kDensityScale);
+            if (q) atomicAdd(row + by, q);
+        }
+    }
+}
+
+// maps -> mirrored total density and the overflow map (drop-in layout)
+__global__ __launch_bounds__(256) void density_post_mirror_kernel(const unsigned long long* map, const float* fixed,
+                                                                  float target, int nbx, int nby, float* rho,
+                                                                  float* over) {
+    long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x, nb = (long long)nbx * nby;
+    if (b >= nb) return;
+    int bx = (int)(b / nby), by = (int)(b - (long long)bx * nby);
+    long long w = 2LL * nby;
+    int mx = 2 * nbx - 1 - bx, my = 2 * nby - 1 - by;
+    float mov = (float)((double)map[b] / kDensityScale) + fixed[b];
+    float r = mov + (float)((double)map[nb + b] / kDensityScale);
+    over[b] = fmaxf(mov - target, 0.f);
+    rho[bx * w + by] = r;
+    rho[bx * w + my] = r;
+    rho[mx * w + by] = r;
+    rho[mx * w + my] = r;
+}
+
+// ------------------------------------------------------------------------------------------- gather
+
+__global__ __launch_bounds__(256) void density_grad_kernel(DgradArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.num_nodes) return;
+    const float weight = *a.weight, precond = *a.precond;
+    const float2 p = a.pos[i], sz = a.size[i];
+    const float r = a.ratio[i];
+    Span sx = make_span(p.x, sz.x, a.inv_bx, a.nbx), sy = make_span(p.y, sz.y, a.inv_by, a.nby);
+    float gx = 0, gy = 0;
+    for (int bx = sx.b0; bx <= sx.b1; bx++) {
+        float wx = r * span_overlap(sx, bx);
+        const float* fxr = a.field_x + (long long)bx * a.ld;
+        const float* fyr = a.field_y + (long long)bx * a.ld;
+        for (int by = sy.b0; by <= sy.b1; by++) {
+            float w = wx * span_overlap(sy, by);
+            gx -= w * fxr[by];
+            gy -= w * fyr[by];
+        }
+    }
+    gx *= weight;
+    gy *= weight;
+    if (i < a.num_wl) {
+        float2 wl = a.wl_grad[i];
+        gx += wl.x;
+        gy += wl.y;
+    }
+    float scale = precond < 0 ? 1.f : 1.f / fmaxf(a.pins[i] + precond * a.area[i], 1.f);
+    a.grad[i] = make_float2(gx * scale, gy * scale);
+}
+
+// ------------------------------------------------------------------------------------------- Nesterov, select
+
+__global__ __launch_bounds__(256) void nesterov_step_kernel(const float* u, const float* v, const float* g,
+                                                            const float* lo, const float* hi, const float* alpha_p,
+                                                            const float* coef_p, int n, float* u_next, float* v_next) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float alpha = *alpha_p, coef = *coef_p;
+    float un = v[i] - alpha * g[i];
+    u_next[i] = un;
+    v_next[i] = fminf(fmaxf(un + coef * (un - u[i]), lo[i]), hi[i]);
+}
+
+__global__ __launch_bounds__(256) void select_kernel(const double* S, float* u, float* v, float* g, const float* u_next,
+                                                     const float* v_next, const float* g_next, float* best, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || S[ACCEPT] == 0) return;
+    u[i] = u_next[i];
+    v[i] = v_next[i];
+    g[i] = g_next[i];
+    if (S[IS_BEST] != 0) best[i] = v_next[i];
+}
+
+// ------------------------------------------------------------------------------------------- control
+
+__global__ void control_kernel(double* S, float* kp, const double* c, const float* moved, const float* turned,
+                               const float* hpwl, const float* ovf, const float* wa) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) control_step(S, kp, c, *moved, *turned, *hpwl, *ovf, *wa);
+}
+
+// ------------------------------------------------------------------------------------------- exact HPWL
+
+// HPWL over every net (evaluator.py:4-8): one lane per net, FP64 per-workgroup partials, then one workgroup sums the
+// partials in a fixed order.
+__global__ __launch_bounds__(256) void net_hpwl_kernel(const float2* pos, const int* pin_cell, const float2* pin_off,
+                                                       const int* net_start, int num_nets, int num_mov, double* part) {
+    __shared__ double red[16];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double h[1] = {0.0};
+    if (i < num_nets) {
+        const int p0 = net_start[i], p1 = net_start[i + 1];
+        if (p1 > p0) {
+            float hx = -INFINITY, hy = -INFINITY, lx = INFINITY, ly = INFINITY;
+            for (int p = p0; p < p1; p++) {
+                float2 x = pin_xy(pos, pin_cell, pin_off, p, num_mov);
+                hx = fmaxf(hx, x.x); hy = fmaxf(hy, x.y); lx = fminf(lx, x.x); ly = fminf(ly, x.y);
+            }
+            h[0] = ((double)hx - lx) + ((double)hy - ly);
+        }
+    }
+    block_sum<1>(h, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = h[0];
+}
+
+__global__ __launch_bounds__(256) void sum_partials_kernel(const double* part, int n, double* out) {
+    __shared__ double red[16];
+    double v[1] = {0.0};
+    for (int k = threadIdx.x; k < n; k += blockDim.x) v[0] += part[k];
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) *out = v[0];
+}
+
+}  // namespace ap
+
+using namespace ap;
+
+// ------------------------------------------------------------------------------------------- C ABI, part 1
+
+extern "C" {
+
+const char* ap_last_error(void) { return ap::g_err; }
+int ap_version(void) { return 100; }
+
+int ap_wa_wirelength(const float* pos, const int* pin_cell, const float* pin_off, const int* net_start,
+                     const int* cell_start, const int* cell_pin, int num_nets, int num_mov, const float* gamma,
+                     float* pin_grad, float* out, float* grad, int num_pins, void* stream) {
+    AP_REQUIRE(num_nets >= 0 && num_mov >= 0 && num_pins >= 0, "wa: negative size");
+    AP_REQUIRE(num_nets == 0 || (net_start && gamma && pin_grad), "wa: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    WaArgs a{(const float2*)pos, pin_cell, (const float2*)pin_off, net_start, num_nets, num_mov, gamma,
+             (float2*)pin_grad, (float2*)out, nullptr, nullptr, nullptr};
+    int st = launch_wa(a, num_pins, s);
+    if (st) return st;
+    if (num_mov > 0) {
+        hipLaunchKernelGGL(wa_cell_kernel, dim3(grid_of(num_mov, 256)), dim3(256), 0, s, (const float2*)pin_grad,
+                           cell_start, cell_pin, num_mov, (float2*)grad);
+        AP_LAUNCHED();
+    }
+    return AP_OK;
+}
+
+int ap_density_map(const float* pos, const float* size, const float* ratio, int num_nodes, int num_mov, float bx,
+                   float by, int nbx, int nby, unsigned long long* map, const float* fixed, float target, float* rho,
+                   float* over, void* stream) {
+    AP_REQUIRE(nbx > 0 && nby > 0 && num_nodes >= 0, "density_map: bad grid %d x %d", nbx, nby);
+    hipStream_t s = (hipStream_t)stream;
+    if (num_nodes > 0) {
+        hipLaunchKernelGGL(density_map_kernel, dim3(grid_of(num_nodes, 256)), dim3(256), 0, s, (const float2*)pos,
+                           (const float2*)size, ratio, num_nodes, num_mov, 1.f / bx, 1.f / by, nbx, nby, map);
+        AP_LAUNCHED();
+    }
+    hipLaunchKernelGGL(density_post_mirror_kernel, dim3(grid_of((long long)nbx * nby, 256)), dim3(256), 0, s, map,
+                       fixed, target, nbx, nby, rho, over);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+int ap_nesterov_step(const float* u, const float* v, const float* g, const float* lo, const float* hi,
+                     const float* alpha, const float* coef, int n, float* u_next, float* v_next, void* stream) {
+    AP_REQUIRE(n >= 0, "nesterov_step: negative size");
+    if (n == 0) return AP_OK;
+    hipLaunchKernelGGL(nesterov_step_kernel, dim3(grid_of(n, 256)), dim3(256), 0, (hipStream_t)stream, u, v, g, lo, hi,
+                       alpha, coef, n, u_next, v_next);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+int ap_density_grad(const float* pos, const float* size, const float* ratio, const float* field, const float* wl_grad,
+                    const float* pins, const float* area, int num_nodes, int num_wl, const float* weight,
+                    const float* precond, float bx, float by, int nbx, int nby, float* grad, void* stream) {
+    AP_REQUIRE(nbx > 0 && nby > 0 && num_nodes >= 0 && num_wl >= 0 && num_wl <= num_nodes, "density_grad: bad sizes");
+    if (num_nodes == 0) return AP_OK;
+    DgradArgs a{(const float2*)pos, (const float2*)size, ratio, field, field + 4LL * nbx * nby, 2LL * nby,
+                (const float2*)wl_grad, pins, area, num_nodes, num_wl, weight, precond, 1.f / bx, 1.f / by, nbx, nby,
+                (float2*)grad};
+    hipLaunchKernelGGL(density_grad_kernel, dim3(grid_of(num_nodes, 256)), dim3(256), 0, (hipStream_t)stream, a);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+int ap_control(double* S, float* kp, const double* c, const float* moved, const float* turned, const float* hpwl,
+               const float* ovf, const float* wa, void* stream) {
+    AP_REQUIRE(S && kp && c && moved && turned && hpwl && ovf && wa, "control: null pointer");
+    hipLaunchKernelGGL(control_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, S, kp, c, moved, turned, hpwl, ovf, wa);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+int ap_net_hpwl(const float* pos, const int* pin_cell, const float* pin_off, const int* net_start, int num_nets,
+                int num_mov, double* work, double* out, void* stream) {
+    AP_REQUIRE(num_nets >= 0 && out && (num_nets == 0 || (work && net_start)), "net_hpwl: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const int nblk = num_nets > 0 ? (int)grid_of(num_nets, 256) : 0;
+    if (nblk) {
+        hipLaunchKernelGGL(net_hpwl_kernel, dim3(nblk), dim3(256), 0, s, (const float2*)pos, pin_cell,
+                           (const float2*)pin_off, net_start, num_nets, num_mov, work);
+        AP_LAUNCHED();
+    }
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, work, nblk, out);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+int ap_select_step(const double* S, float* u, float* v, float* g, const float* u_next, const float* v_next,
+                   const float* g_next, float* best, int n, void* stream) {
+    AP_REQUIRE(n >= 0, "select: negative size");
+    if (n == 0) return AP_OK;
+    hipLaunchKernelGGL(select_kernel, dim3(grid_of(n, 256)), dim3(256), 0, (hipStream_t)stream, S, u, v, g, u_next,
+                       v_next, g_next, best, n);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+}  // extern "C"

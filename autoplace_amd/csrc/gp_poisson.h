@@ -1,0 +1,33 @@
+// Poisson plan shared by the C ABI and the GP engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace ap {
+
+void poisson_weights(int N, int M, double bx, double by, std::vector<float>& wx, std::vector<float>& wy);
+
+struct Poisson {
+    int N = 0, M = 0, method = 0;
+    double bx = 1, by = 1;
+    float *d_wx = nullptr, *d_wy = nullptr, *d_work = nullptr;
+    // GEMM method: Cx, Cx^T, Sx^T (N x N); Cy, Sy, Cy^T (M x M)
+    float *d_cx = nullptr, *d_cxt = nullptr, *d_sxt = nullptr, *d_cy = nullptr, *d_sy = nullptr, *d_cyt = nullptr;
+    // FFT method: twiddles
+    float2* d_tw = nullptr;
+    static constexpr int fft_min = 64;       // AUTO picks the FFT from this side up
+    static constexpr int fft_max = 4096;     // largest side the LDS FFT holds
+    static bool fft_supported(int n);
+    static bool fft_max_ok();
+
+    int create(int nbx, int nby, double bx, double by, int method);
+    void destroy();
+    int solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s);
+
+    int fft_create();
+    void fft_destroy();
+    int fft_solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s);
+};
+
+}  // namespace ap

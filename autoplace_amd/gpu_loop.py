@@ -1,0 +1,156 @@
+"""The rest of a level with the run's control on the device (src/gpu_loop.py:1-81 of the reference).
+
+What NesterovOptimizer.step, ParamScheduler.step and ParamScheduler.need_to_stop decide on the host is decided by
+the control kernel, in the same order and in double precision; the kernels read gamma, the density weight and the
+step from device memory. Each host check runs `check_every` passes in one native call (ap_level_run, a captured HIP
+graph after the first call); once the run has stopped the control freezes, so the result is that of a run checked
+after every pass.
+"""
+import ctypes
+import math
+
+import torch
+
+from . import _lib
+
+FIELDS = ("lam gam alpha a iter hpwl_prev motion ww mu settled best_hpwl best_ovf is_best improved stop accept "
+          "loop_it evals res_hpwl res_ovf has_best").split()
+K = {name: i for i, name in enumerate(FIELDS)}
+REASON = {1: "converged", 2: "stalled", 3: "diverged", 4: "max_iter", 5: "handover"}
+
+
+def usable(optimizer):
+    return optimizer.kernel
+
+
+def next_coef(a):
+    a_next = (1 + math.sqrt(4 * a * a + 1)) / 2
+    return (a - 1) / a_next
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Engine(object):
+    """A level's problem bound to the native GP iteration engine (ap_level). Keeps every tensor it hands to the
+    library alive."""
+
+    def __init__(self, wirelength, density, u, v, g, best, S, kp, c, poisson="auto"):
+        w, d, j = wirelength.opt, density, density.judge
+        self._keep = [w["cell32"], w["off"], w["net_start"], w["cell_start"], w["cell_pin"], u, v, g, best, S, kp, c]
+        for t in self._keep:
+            if not t.is_contiguous():
+                raise ValueError("engine tensors must be contiguous")
+        desc = _lib.LevelDesc()
+        desc.num_nodes, desc.num_mov = d.num_nodes, d.num_mov
+        desc.num_pins, desc.num_nets = w["cell32"].numel(), w["num_nets"]
+        desc.pin_cell, desc.pin_off, desc.net_start = _p(w["cell32"]), _p(w["off"]), _p(w["net_start"])
+        desc.cell_start, desc.cell_pin = _p(w["cell_start"]), _p(w["cell_pin"])
+        desc.size, desc.ratio = _p(d.smooth["size"]), _p(d.smooth["ratio"])
+        desc.pins, desc.area, desc.lo, desc.hi = _p(d.pins), _p(d.area), _p(d.pos_lo), _p(d.pos_hi)
+        desc.nbx, desc.nby = d.nbx, d.nby
+        desc.bx, desc.by = float(d.bin_np[0]), float(d.bin_np[1])
+        desc.fixed, desc.target = _p(d.fixed_map), float(d.target)
+        desc.bin_area, desc.mov_area = d.bin_area, d.mov_area
+        self._keep += [d.smooth["size"], d.smooth["ratio"], d.pins, d.area, d.pos_lo, d.pos_hi, d.fixed_map]
+        if j is not None:
+            desc.has_judge, desc.njx, desc.njy = 1, j.nbx, j.nby
+            desc.jbx, desc.jby, desc.judge_bin_area = float(j.bin_np[0]), float(j.bin_np[1]), j.bin_area
+            desc.judge_size, desc.judge_ratio, desc.judge_fixed = _p(j.smooth["size"]), _p(j.smooth["ratio"]), \
+                _p(j.fixed_map)
+            self._keep += [j.smooth["size"], j.smooth["ratio"], j.fixed_map]
+        desc.poisson_method = {"auto": 0, "gemm": 1, "fft": 2}[poisson]
+        desc.u, desc.v, desc.g, desc.best = _p(u), _p(v), _p(g), _p(best)
+        desc.S, desc.kp, desc.c = _p(S), _p(kp), _p(c)
+        self.desc = desc
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().ap_level_create(ctypes.byref(desc), ctypes.byref(h)), "level_create")
+        self._h = h
+
+    def run(self, passes, use_graph=True):
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(_lib.lib().ap_level_run(self._h, passes, int(use_graph), stream), "level_run")
+
+    def profile(self, passes):
+        """Average ms per pass of every kernel class (direct launches with HIP events). -> dict."""
+        ms = (ctypes.c_double * len(_lib.PROF_CLASSES))()
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(_lib.lib().ap_level_profile(self._h, passes, ms, stream), "level_profile")
+        return dict(zip(_lib.PROF_CLASSES, list(ms)))
+
+    def last_ms(self):
+        ms = ctypes.c_float()
+        _lib.check(_lib.lib().ap_level_last_ms(self._h, ctypes.byref(ms)), "level_last_ms")
+        return ms.value
+
+    def objective(self, pos, grad, stats):
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(_lib.lib().ap_level_objective(self._h, _p(pos), _p(grad), _p(stats), stream), "level_objective")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().ap_level_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def control_state(ps, optimizer, iteration, args):
+    """(S f64[21], kp f32[5], c f64[13]) from the host state (gpu_loop.py:39-53)."""
+    S = [0.0] * len(FIELDS)
+    S[K["lam"]], S[K["gam"]], S[K["alpha"]], S[K["a"]] = ps.density_weight, ps.wa_coeff, optimizer.alpha, optimizer.a
+    S[K["iter"]], S[K["hpwl_prev"]], S[K["motion"]], S[K["ww"]] = ps.iter, ps.hpwl[-1], ps.motion, ps.weighted_weight
+    S[K["mu"]], S[K["settled"]], S[K["improved"]] = ps.mu, ps.settled, ps.improved_iter
+    S[K["best_hpwl"]], S[K["best_ovf"]] = ps.best_hpwl, ps.best_overflow
+    S[K["loop_it"]] = iteration
+    kp = [ps.wa_coeff, ps.density_weight, ps.density_weight, optimizer.alpha, next_coef(optimizer.a)]
+    a = ps.args
+    c = [ps.base_gamma, ps.motion_unit, ps.total_pins, ps.total_area, ps.pitch, a.min_iter, a.stop_overflow,
+         a.settle_iter, a.stall_iter, args.inner_iter, ps.refine, float(a.wa_rule == "accuracy"),
+         ps.handover_gamma or 0.0]
+    return S, kp, c
+
+
+def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every=16, use_graph=True,
+                     poisson="auto"):
+    """Continue the level from `iteration` (density weight on) until it stops; updates optimizer, ps and
+    objective.evals as the host loop would. -> (reason, iterations, hpwl, overflow) of the kept placement."""
+    device, pos = optimizer.v.device, optimizer.v
+    S, kp, c = control_state(ps, optimizer, iteration, args)
+    S = torch.tensor(S, dtype=torch.float64, device=device)
+    kp = torch.tensor(kp, dtype=torch.float32, device=device)
+    c = torch.tensor(c, dtype=torch.float64, device=device)
+    u = optimizer.u.contiguous()
+    g = optimizer.g.contiguous()
+    best = pos.clone()
+    eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson)
+    evals_before, logged = objective.evals, iteration // args.log_freq
+    watch = torch.tensor([K["stop"], K["loop_it"], K["lam"], K["gam"], K["res_hpwl"], K["res_ovf"]], device=device)
+    try:
+        while True:
+            eng.run(check_every, use_graph)
+            stop, it, lam, gam, hpwl, ovf = S[watch].tolist()
+            if log and int(it) // args.log_freq > logged:
+                logged = int(it) // args.log_freq
+                log("iter: %d | hpwl: %.4E overflow: %.4f density_weight: %.4E wa_coeff: %.4E" % (it, hpwl, ovf, lam, gam))
+            if stop:
+                break
+    finally:
+        torch.cuda.synchronize(device)
+        eng.close()
+    S = S.tolist()
+    get = lambda name: S[K[name]]   # noqa: E731
+    ps.density_weight, ps.wa_coeff, ps.iter, ps.motion = get("lam"), get("gam"), int(get("iter")), get("motion")
+    ps.weighted_weight, ps.mu, ps.settled = get("ww"), get("mu"), int(get("settled"))
+    ps.best_hpwl, ps.best_overflow, ps.improved_iter = get("best_hpwl"), get("best_ovf"), int(get("improved"))
+    ps.hpwl.append(get("hpwl_prev"))
+    optimizer.u, optimizer.g, optimizer.alpha, optimizer.a = u, g, get("alpha"), get("a")
+    objective.evals = evals_before + int(get("evals"))
+    if get("has_best"):
+        pos.copy_(best)
+    return REASON[int(get("stop"))], int(get("loop_it")), get("res_hpwl"), get("res_ovf")

@@ -1,0 +1,158 @@
+"""The placement flow on the device: problem -> two-level global placement -> evaluation
+(src/run_placement.py:34-200 of the reference; LEF / DEF reading and writing in lefdef.py).
+"""
+import math
+import time
+from types import SimpleNamespace
+
+import numpy as np
+import torch
+
+from . import gpu_loop
+from .calculator import Objective
+from .cluster import coarse_problem, expand
+from .density import ElectronicDensityLayer
+from .nesterov_optimizer import NesterovOptimizer, estimate_initial_learning_rate
+from .param_scheduler import ParamScheduler
+from .problem import build_problem
+from .resolution import refine
+from .wirelength import Wirelength
+
+
+def _sync(device):
+    torch.cuda.synchronize(device)
+
+
+def _scheduler(density, num_mov, mov_area, total_pins, total_area, num_nodes, args):
+    """pitch: the spacing of the cells once spread at the target density (run_placement.py:34-38)."""
+    return ParamScheduler(SimpleNamespace(bin=density.bin_np, total_pins=total_pins, total_area=total_area,
+                                          num_nodes=num_nodes, pitch=math.sqrt(mov_area / density.target / num_mov),
+                                          refine=refine(density)), args)
+
+
+def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None):
+    """One level (run_placement.py:41-76): host loop through the wirelength-only start, then the device loop.
+    -> namespace(reason, iters, start_iters, hpwl, overflow)."""
+    opts = opts or {}
+    bounds = (objective.density.pos_lo, objective.density.pos_hi)
+    lr = estimate_initial_learning_rate(pos, objective, args.lr, bounds)
+    optimizer = NesterovOptimizer(pos, objective, lr, bounds, stats=lambda: (objective.hpwl, objective.overflow,
+                                                                               objective.wa))
+    reason, iteration, start_iters = "max_iter", first_iter, 0
+    hpwl = overflow = float("nan")
+    for iteration in range(first_iter, args.inner_iter):
+        if ps.density_weight > 0 and gpu_loop.usable(optimizer):
+            reason, iters, hpwl, overflow = gpu_loop.run_level_on_gpu(
+                optimizer, objective, ps, args, log, iteration, use_graph=opts.get("graph", True),
+                poisson=opts.get("poisson", "auto"))
+            return SimpleNamespace(reason=reason, iters=iters, start_iters=start_iters, hpwl=hpwl, overflow=overflow)
+        optimizer.step()
+        if optimizer.diverged:
+            reason = "diverged"
+            break
+        hpwl, overflow, wa = optimizer.fetched
+        if log and iteration % args.log_freq == 0:
+            log("iter: %d | hpwl: %.4E overflow: %.4f density_weight: %.4E wa_coeff: %.4E"
+                % (iteration, hpwl, overflow, ps.density_weight, ps.wa_coeff))
+        if ps.density_weight == 0:
+            if ps.start_step(hpwl, wa):
+                ps.density_weight, start_iters = args.density_weight * objective.tension_unit(pos), iteration + 1
+                if log:
+                    log("wirelength-only start over after %d iterations (hpwl %.4E): density weight %.4E"
+                        % (start_iters, hpwl, ps.density_weight))
+            continue
+        raise RuntimeError("unreachable: the scheduled part runs on the device")
+    return SimpleNamespace(reason=reason, iters=iteration + 1, start_iters=start_iters, hpwl=hpwl, overflow=overflow)
+
+
+def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=None):
+    """Optimize pos in place (run_placement.py:79-117). -> dict of GP statistics (gp_time_s from the synced start)."""
+    device, n = pos.device, prob.num_mov
+    data, node_size, fixed_fraction = prob.data, prob.node_size, prob.fixed_fraction
+    opts = opts or {}
+    _sync(device)
+    start = time.time()
+    ps = _scheduler(density, n, data.mov_area, float(data.cell_pins.sum()), float(node_size.prod(1).sum()),
+                    len(node_size), args)
+    objective = Objective(wirelength, density, ps)
+    stats, evals, first_iter, start_iters = {}, 0, 0, 0
+    if args.cluster_ratio > 1:
+        coarse = coarse_problem(data, node_size, wirelength, density, fixed_fraction, args,
+                                poisson=opts.get("poisson", "auto"))
+        _sync(device)
+        stats = {"clusters": coarse.num, "cluster_time_s": time.time() - start, "coarse_num_bin": coarse.density.nbx}
+        if log:
+            log("two levels: %d clusters of %d cells (%.2fs), %d x %d bins"
+                % (coarse.num, n, stats["cluster_time_s"], coarse.density.nbx, coarse.density.nby))
+        cargs = SimpleNamespace(**vars(args))
+        cargs.settle_iter, cargs.min_iter = 0, 0
+        cps = _scheduler(coarse.density, coarse.num, data.mov_area, coarse.total_pins, coarse.total_area,
+                         coarse.num_nodes, cargs)
+        cps.handover_gamma = ps.wa_coeff
+        cobjective = Objective(coarse.wirelength, coarse.density, cps)
+        cpos = torch.cat([pos[:coarse.num], pos[n:]])
+        level = _place_level(cpos, cobjective, cps, cargs, log, 0, opts)
+        pos.copy_(expand(cpos, coarse, node_size, args.seed)).clamp_(min=density.pos_lo, max=density.pos_hi)
+        weight = cps.density_weight * objective.tension_unit(pos) / cobjective.tension_unit(cpos)
+        overflow = density.forward(pos)[1].item() if args.wa_rule == "overflow" else None
+        ps.resume(cps, weight, wirelength.forward(pos, ps.wa_coeff)[2].item(), overflow)
+        if log:
+            log("clusters placed after %d iterations (stop: %s, hpwl %.4E, overflow %.4f): density weight %.4E"
+                % (level.iters, level.reason, level.hpwl, level.overflow, weight))
+        evals, first_iter, start_iters = cobjective.evals, level.iters, level.start_iters
+        stats["coarse_iters"] = level.iters
+        stats["cluster_ids"] = coarse.cid
+        del coarse, cobjective, cps, cpos
+    level = _place_level(pos, objective, ps, args, log, first_iter, opts)
+    _sync(device)
+    gp_time, iters = time.time() - start, level.iters
+    if log:
+        log("GP stop (%s): #iters %d hpwl: %.6E overflow: %.4f GP time: %.4fs (%d gradient calls)"
+            % (level.reason, iters, level.hpwl, level.overflow, gp_time, evals + objective.evals))
+    stats.update({"stop_reason": level.reason, "iters": iters, "start_iters": start_iters or level.start_iters,
+                  "gp_time_s": gp_time, "per_iter_s": gp_time / max(iters, 1), "grad_calls": evals + objective.evals,
+                  "gp_hpwl": level.hpwl, "gp_overflow": level.overflow})
+    return stats
+
+
+def build_operators(prob, args, device, poisson="auto"):
+    """Wirelength, density (force grid) with its overflow grid, as run_placement.py:163-170."""
+    data = prob.data
+    wirelength = Wirelength(data.pin_cell, data.pin_off, data.pin_net, data.num_nets, data.num_mov,
+                            args.ignore_net_degree, device)
+    density = ElectronicDensityLayer(data.width, data.height, prob.force_bin, prob.force_bin, prob.fixed_fraction,
+                                     args.target_density, prob.node_size, data.num_mov, device, prob.node_pins,
+                                     poisson=poisson)
+    if prob.force_bin != prob.num_bin:
+        density.judge = ElectronicDensityLayer(data.width, data.height, prob.num_bin, prob.num_bin,
+                                               prob.judge_fraction, args.target_density, data.size, data.num_mov,
+                                               device, field=False)
+    return wirelength, density
+
+
+def evaluate_placement(pos, wirelength, density, data):
+    """Exact HPWL (every net) and exact overflow (real sizes, overflow grid) (evaluator.py:4-8)."""
+    hpwl = wirelength.hpwl(pos).item()
+    return {"hpwl_sites": hpwl, "hpwl_um": hpwl * data.unit_dbu / float(data.d.dbu),
+            "overflow": density.exact_overflow(pos[:data.num_mov]).item()}
+
+
+def place(data, args, device="cuda", log=None, opts=None):
+    """The whole run on a PlaceData: problem, two-level GP, evaluation. -> (positions [nodes, 2] on the device,
+    metrics dict)."""
+    device = torch.device(device)
+    if device.type == "cuda" and device.index is not None:
+        torch.cuda.set_device(device)
+    opts = opts or {}
+    prob = build_problem(data, args, log)
+    wirelength, density = build_operators(prob, args, device, opts.get("poisson", "auto"))
+    metrics = {"design": data.name, "num_movable": data.num_mov, "num_nets": data.num_nets,
+               "num_pins": data.num_pins, "num_fillers": prob.num_fillers, "num_bin": prob.num_bin,
+               "force_num_bin": prob.force_bin, "target_density": args.target_density, "seed": args.seed,
+               "device": str(device), "poisson": density.plan.method}
+    pos = torch.tensor(prob.pos0, dtype=torch.float32, device=device)
+    metrics.update(global_placement_main(prob, wirelength, density, pos, args, log, opts))
+    if not bool(torch.isfinite(pos[:data.num_mov]).all()):
+        raise RuntimeError("the placement is not finite (diverged before any usable point)")
+    metrics.update(evaluate_placement(pos, wirelength, density, data))
+    return pos, metrics

@@ -1,0 +1,170 @@
+/* C ABI of the MI355X (gfx950) global-placement library, libautoplace_gp.so.
+ *
+ * Plain pointers (device memory) and sizes; no framework types. Every call is asynchronous on the given HIP stream
+ * (`stream` is a hipStream_t; NULL = the null stream of the current device) and returns AP_OK or an error code;
+ * ap_last_error() describes the last error of the calling thread.
+ *
+ * Part 1 replaces the six launchers the reference's binding declares (cpp_to_py/gp_cuda.cpp:4-13, kernels in
+ * cpp_to_py/gp_cuda_kernel.cu) with the same arguments, in the same order, plus the stream (and, for the WA, the pin
+ * count the reference reads from its tensor). Part 2 is the Poisson solve the reference leaves to torch.fft
+ * (src/core/electronic_density_layer.py:133). Part 3 runs whole passes of the GP iteration on the device, the work of
+ * src/gpu_loop.py:58-64 (nesterov_step, Objective.on_gpu, control, select) as one native engine.
+ */
+#ifndef AUTOPLACE_GP_H
+#define AUTOPLACE_GP_H
+
+#if defined(__GNUC__)
+#define AP_API __attribute__((visibility("default")))
+#else
+#define AP_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AP_OK 0
+#define AP_ERR_ARG 1          /* invalid argument (shape, size, null pointer) */
+#define AP_ERR_HIP 2          /* HIP runtime error */
+#define AP_ERR_UNSUPPORTED 3  /* valid request this build does not support */
+
+/* Text of the last error of the calling thread (empty if none). */
+AP_API const char* ap_last_error(void);
+/* Library ABI version (major * 100 + minor). */
+AP_API int ap_version(void);
+
+/* ---------------------------------------------------------------- part 1: drop-in kernels */
+
+/* WA wirelength (gp_cuda.cpp:4-5 `wa_wirelength`; kernels gp_cuda_kernel.cu:12-65).
+ * pos f32[>=num_mov][2]; pin_cell i32[num_pins] (num_mov = fixed pin at off); pin_off f32[num_pins][2];
+ * net_start i32[num_nets+1] (pins grouped by net); cell_start i32[num_mov+1], cell_pin i32[>=cell_start[num_mov]];
+ * gamma f32[1] on the device. Writes pin_grad f32[num_pins][2], out f32[num_nets][2] = per net (WA, HPWL) and
+ * grad f32[num_mov][2]. */
+AP_API int ap_wa_wirelength(const float* pos, const int* pin_cell, const float* pin_off, const int* net_start,
+                     const int* cell_start, const int* cell_pin, int num_nets, int num_mov, const float* gamma,
+                     float* pin_grad, float* out, float* grad, int num_pins, void* stream);
+
+/* Density map (gp_cuda.cpp:6-7 `density_map`; kernels gp_cuda_kernel.cu:84-95,130-146).
+ * map u64[2][nbx][nby] must be zero on entry (as the reference's binding allocates it); fixed f32[nbx][nby].
+ * Writes rho f32[2 nbx][2 nby] (total density mirrored in both axes) and over f32[nbx][nby]. */
+AP_API int ap_density_map(const float* pos, const float* size, const float* ratio, int num_nodes, int num_mov, float bx,
+                   float by, int nbx, int nby, unsigned long long* map, const float* fixed, float target, float* rho,
+                   float* over, void* stream);
+
+/* Nesterov step (gp_cuda.cpp:8-9 `nesterov_step`; kernel gp_cuda_kernel.cu:157-170); n = coordinates. */
+AP_API int ap_nesterov_step(const float* u, const float* v, const float* g, const float* lo, const float* hi,
+                     const float* alpha, const float* coef, int n, float* u_next, float* v_next, void* stream);
+
+/* Preconditioned gradient from the field (gp_cuda.cpp:10-11 `density_grad`; kernel gp_cuda_kernel.cu:102-125,148-153).
+ * field f32[2][2 nbx][2 nby] (only [:, :nbx, :nby] is read); wl_grad f32[num_wl][2]; weight, precond f32[1]. */
+AP_API int ap_density_grad(const float* pos, const float* size, const float* ratio, const float* field, const float* wl_grad,
+                    const float* pins, const float* area, int num_nodes, int num_wl, const float* weight,
+                    const float* precond, float bx, float by, int nbx, int nby, float* grad, void* stream);
+
+/* One pass of the run's control (gp_cuda.cpp:12 `control`; kernel gp_cuda_kernel.cu:172-240).
+ * S f64[21], kp f32[5], c f64[13]; the five scalars are f32[1] on the device. */
+AP_API int ap_control(double* S, float* kp, const double* c, const float* moved, const float* turned, const float* hpwl,
+               const float* ovf, const float* wa, void* stream);
+
+/* Take an accepted step, keep the best placement (gp_cuda.cpp:13 `select_step`; kernel gp_cuda_kernel.cu:242-259). */
+AP_API int ap_select_step(const double* S, float* u, float* v, float* g, const float* u_next, const float* v_next,
+                   const float* g_next, float* best, int n, void* stream);
+
+/* Exact HPWL over every net (src/evaluator.py:4-8; src/core/wa_wirelength_hpwl.py:58-70), reduced in FP64 on the
+ * device: work f64[ceil(num_nets / 256)], out f64[1]. */
+AP_API int ap_net_hpwl(const float* pos, const int* pin_cell, const float* pin_off, const int* net_start, int num_nets,
+                       int num_mov, double* work, double* out, void* stream);
+
+/* ---------------------------------------------------------------- part 2: Poisson solve */
+
+#define AP_POISSON_AUTO 0
+#define AP_POISSON_GEMM 1   /* cosine / sine series as FP32 MFMA GEMMs */
+#define AP_POISSON_FFT 2    /* LDS FFTs (powers of two only) */
+
+typedef struct ap_poisson ap_poisson;
+
+/* Plan for an nbx x nby grid of bins bx x by; method as above. */
+AP_API int ap_poisson_create(int nbx, int nby, double bx, double by, int method, ap_poisson** plan);
+AP_API int ap_poisson_destroy(ap_poisson* plan);
+/* Method the plan uses (AP_POISSON_GEMM or AP_POISSON_FFT). */
+AP_API int ap_poisson_method(const ap_poisson* plan);
+/* field of the map rho (row stride rho_ld floats; rho_ld = nby for an unmirrored map, 2 nby for the top-left
+ * quadrant of a mirrored one) into field_x / field_y (row stride field_ld). Equal to the reference's
+ * irfft2(rfft2(mirror(rho)) * to_field)[:, :nbx, :nby]. */
+AP_API int ap_poisson_solve(ap_poisson* plan, const float* rho, int rho_ld, float* field_x, float* field_y, int field_ld,
+                     void* stream);
+
+/* ---------------------------------------------------------------- part 3: the GP iteration engine */
+
+typedef struct ap_level ap_level;
+
+/* One level's problem and state, all device pointers (the caller owns them and keeps them alive). */
+typedef struct ap_level_desc {
+    int num_nodes, num_mov, num_pins, num_nets;
+    /* netlist: optimized nets, pins grouped by net; cell CSR of the movable pins */
+    const int* pin_cell;
+    const float* pin_off;
+    const int* net_start;
+    const int* cell_start;
+    const int* cell_pin;
+    /* nodes (cells then fillers): smoothed size f32[N][2], density ratio f32[N], pins f32[N], area f32[N],
+     * position bounds f32[N][2] */
+    const float* size;
+    const float* ratio;
+    const float* pins;
+    const float* area;
+    const float* lo;
+    const float* hi;
+    /* force grid */
+    int nbx, nby;
+    double bx, by;
+    const float* fixed;      /* f32[nbx][nby] blocked density */
+    float target;
+    double bin_area, mov_area;
+    /* overflow grid (has_judge = 0: the overflow is measured on the force grid) */
+    int has_judge, njx, njy;
+    double jbx, jby, judge_bin_area;
+    const float* judge_size;   /* f32[num_mov][2] */
+    const float* judge_ratio;  /* f32[num_mov] */
+    const float* judge_fixed;  /* f32[njx][njy] */
+    int poisson_method;        /* AP_POISSON_* */
+    /* optimizer buffers f32[N][2]: u, v, g of the current point; best placement */
+    float* u;
+    float* v;
+    float* g;
+    float* best;
+    /* control: S f64[21], kp f32[5], c f64[13] (layouts of src/gpu_loop.py:20-21, gp_cuda_kernel.cu:178-181) */
+    double* S;
+    float* kp;
+    const double* c;
+} ap_level_desc;
+
+AP_API int ap_level_create(const ap_level_desc* desc, ap_level** level);
+AP_API int ap_level_destroy(ap_level* level);
+/* Run `passes` passes (each: Nesterov step, objective, control, select). With use_graph the passes are replayed from
+ * a HIP graph captured on the first call. On return (stream-ordered) u / v / g / best / S / kp hold the state after
+ * the passes, as the reference's loop leaves them. */
+AP_API int ap_level_run(ap_level* level, int passes, int use_graph, void* stream);
+/* Kernel classes of a pass, in launch order, for ap_level_profile. */
+#define AP_PROF_NESTEROV 0
+#define AP_PROF_WA 1
+#define AP_PROF_MAP 2
+#define AP_PROF_POST 3
+#define AP_PROF_POISSON 4
+#define AP_PROF_GATHER 5
+#define AP_PROF_CONTROL 6
+#define AP_PROF_CLASSES 7
+/* Run `passes` passes with direct launches and HIP events around every kernel class; ms[AP_PROF_CLASSES] receives
+ * the average milliseconds per pass of each class. Blocks the calling thread until the passes are done. */
+AP_API int ap_level_profile(ap_level* level, int passes, double* ms, void* stream);
+/* Timing of the last ap_level_run (ms between its first and last kernel), for the caller's instrumentation. */
+AP_API int ap_level_last_ms(ap_level* level, float* ms);
+
+/* One gradient evaluation of the objective at pos (Objective.on_gpu, src/calculator.py:37-43) with the parameters
+ * in kp (gamma, weight, precond): grad f32[N][2]; stats f64[4] = (WA, HPWL, overflow, 0). */
+AP_API int ap_level_objective(ap_level* level, const float* pos, float* grad, double* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

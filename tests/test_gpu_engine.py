@@ -1,0 +1,121 @@
+"""The native GP iteration engine on the GPU: its objective against the oracle, graph replay against direct launches
+(bit for bit), determinism, and a short trajectory against the oracle's control loop."""
+import math
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import norm_err, rel_err
+from oracle import control as octl
+from oracle import place as opl
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _setup(data, weight=2e-3, gamma=30.0, seed=1):
+    from autoplace_amd import gpu_loop
+    from autoplace_amd.problem import build_problem, default_args
+    from autoplace_amd.run_placement import _scheduler, build_operators
+    args = default_args()
+    prob = build_problem(data, args)
+    wl, den = build_operators(prob, args, DEV, "gemm")
+    rng = np.random.default_rng(seed)
+    lo, hi = den.pos_lo.cpu().numpy(), den.pos_hi.cpu().numpy()
+    pos = (lo + rng.random(lo.shape) * (hi - lo)).astype(np.float32)
+    ps = _scheduler(den, data.num_mov, data.mov_area, float(data.cell_pins.sum()), float(prob.node_size.prod(1).sum()),
+                    len(prob.node_size), args)
+    ps.density_weight, ps.wa_coeff, ps.iter = weight, gamma, 40
+    ps.hpwl.append(1e30)
+    opt = SimpleNamespace(alpha=0.05, a=1.0)
+    S, kp, c = gpu_loop.control_state(ps, opt, 40, args)
+    return args, prob, wl, den, pos, S, kp, c
+
+
+def _oracle_level(prob, args, data):
+    n = data.num_mov
+    judge = None if prob.force_bin == prob.num_bin else {"nb": prob.num_bin, "fraction": prob.judge_fraction,
+                                                          "size": prob.node_size[:n]}
+    return opl.Level(prob.width, prob.height, prob.force_bin, prob.fixed_fraction, prob.target, prob.node_size, n,
+                     prob.node_pins, prob.pin_cell, prob.pin_off, prob.pin_net, prob.num_nets, args.ignore_net_degree,
+                     judge)
+
+
+def _engine(wl, den, pos, S, kp, c, g0):
+    from autoplace_amd.gpu_loop import Engine
+    t = lambda a, dt=torch.float32: torch.tensor(np.asarray(a), dtype=dt, device=DEV)   # noqa: E731
+    u, v, g, best = t(pos), t(pos), t(g0), t(pos)
+    dS, dkp, dc = t(S, torch.float64), t(kp), t(c, torch.float64)
+    return Engine(wl, den, u, v, g, best, dS, dkp, dc, "gemm"), (u, v, g, best, dS, dkp, dc)
+
+
+def test_engine_objective_vs_oracle(small_design):
+    _, _, data = small_design
+    args, prob, wl, den, pos, S, kp, c = _setup(data)
+    eng, (u, v, g, best, dS, dkp, dc) = _engine(wl, den, pos, S, kp, c, np.zeros_like(pos))
+    grad = torch.empty_like(v)
+    stats = torch.zeros(8, dtype=torch.float64, device=DEV)
+    eng.objective(v, grad, stats)
+    torch.cuda.synchronize()
+    lvl = _oracle_level(prob, args, data)
+    kp32 = np.array(kp, np.float32)
+    g_ref, wa, hpwl, ovf = opl._evaluate_kp(lvl, pos, kp32)
+    st = stats.cpu().numpy()
+    print("engine objective: grad err %.2e, wa %.2e, hpwl %.2e, ovf %.2e" % (
+        norm_err(grad.cpu().numpy(), g_ref), rel_err(st[0], wa), rel_err(st[1], hpwl), rel_err(st[2], ovf)))
+    assert norm_err(grad.cpu().numpy(), g_ref) < 1e-5
+    assert rel_err(st[0], wa) < 1e-6 and rel_err(st[1], hpwl) < 1e-6 and rel_err(st[2], ovf) < 1e-6
+    eng.close()
+
+
+def _run(data, passes, use_graph, calls=1):
+    args, prob, wl, den, pos, S, kp, c = _setup(data)
+    g0 = np.zeros_like(pos)
+    eng, bufs = _engine(wl, den, pos, S, kp, c, g0)
+    grad = torch.empty_like(bufs[1])
+    stats = torch.zeros(8, dtype=torch.float64, device=DEV)
+    eng.objective(bufs[1], grad, stats)            # g at the start point
+    bufs[2].copy_(grad)
+    for _ in range(calls):
+        eng.run(passes, use_graph)
+    torch.cuda.synchronize()
+    out = [b.cpu().numpy().copy() for b in bufs]
+    eng.close()
+    return out, (args, prob, pos, S, kp, c, grad.cpu().numpy())
+
+
+def test_engine_graph_equals_direct_and_is_deterministic(small_design):
+    _, _, data = small_design
+    a, _ = _run(data, 16, False, calls=2)
+    b, _ = _run(data, 16, True, calls=2)
+    c, _ = _run(data, 16, True, calls=2)
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, y) and np.array_equal(y, z)
+    K = octl.K
+    assert a[4][K["loop_it"]] == 40 + 32 and a[4][K["evals"]] == 32
+
+
+def test_engine_trajectory_vs_oracle(small_design):
+    """16 passes of the engine against 16 oracle passes from the same state: the control decisions agree and the
+    points stay within FP32 drift of each other."""
+    _, _, data = small_design
+    (u, v, g, best, S_dev, kp_dev, _), (args, prob, pos, S, kp, c, g0) = _run(data, 16, True)
+    lvl = _oracle_level(prob, args, data)
+    S = list(S)
+    kp = np.array(kp, np.float32)
+    uo, vo, go = pos.copy(), pos.copy(), g0.copy()
+    K = octl.K
+    for _ in range(16):
+        un, vn = octl.nesterov_step(uo, vo, go, lvl.pos_lo, lvl.pos_hi, kp[3], kp[4])
+        gn, wa, hpwl, ovf = opl._evaluate_kp(lvl, vn, kp)
+        octl.control_pass(S, kp, c, octl.dist(vn, vo), octl.dist(gn, go), hpwl, ovf, wa)
+        if S[K["accept"]]:
+            uo, vo, go = un, vn, gn
+    for name in ("iter", "loop_it", "evals", "settled", "stop"):
+        assert S_dev[K[name]] == S[K[name]], name
+    for name in ("lam", "gam", "alpha", "hpwl_prev"):
+        assert rel_err(S_dev[K[name]], S[K[name]]) < 1e-4, (name, S_dev[K[name]], S[K[name]])
+    print("engine vs oracle after 16 passes: v err %.2e" % norm_err(v, vo))
+    assert norm_err(v, vo) < 1e-4
