@@ -79,22 +79,102 @@ struct MapJob {
     int blocks;          // workgroups of this job
 };
 
+// A workgroup takes MAP_NODES consecutive nodes (spatial neighbours after the engine's Morton renumbering). When
+// their bins fit MAP_LDS and they share a plane, the contributions are summed with integer LDS atomics and each
+// touched bin is flushed with one global atomic; otherwise every contribution goes to global memory directly.
+// Integer sums either way: the map is the same whatever the path and the order.
+constexpr int MAP_PER_LANE = 4;
+constexpr int MAP_NODES = EB * MAP_PER_LANE;
+constexpr int MAP_LDS = 4096;
+
+__device__ __forceinline__ unsigned long long contribution(float wx, const Span& sy, int by) {
+    return (unsigned long long)((double)(wx * span_overlap(sy, by)) * kDensityScale);
+}
+
 __global__ __launch_bounds__(EB) void eng_density_map(Pick pos_p, MapJob f, MapJob j) {
+    __shared__ unsigned long long tile[MAP_LDS];
+    __shared__ int red[4][EB / 64];
     const bool judge = (int)blockIdx.x >= f.blocks;
     const MapJob& J = judge ? j : f;
-    const int i = ((int)blockIdx.x - (judge ? f.blocks : 0)) * EB + threadIdx.x;
-    if (i >= J.num_nodes) return;
+    const int base = ((int)blockIdx.x - (judge ? f.blocks : 0)) * MAP_NODES;
+    if (base >= J.num_nodes) return;                               // uniform over the workgroup
     const float2* pos = pos_p.get(true);
-    const float2 p = pos[i], sz = J.size[i];
-    const float r = J.ratio[i];
-    Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
-    unsigned long long* m = J.map + (i < J.num_mov ? 0 : (long long)J.nbx * J.nby);
-    for (int bx = sx.b0; bx <= sx.b1; bx++) {
-        const float wx = r * span_overlap(sx, bx);
-        unsigned long long* row = m + (long long)bx * J.nby;
-        for (int by = sy.b0; by <= sy.b1; by++) {
-            unsigned long long q = (unsigned long long)((double)(wx * span_overlap(sy, by)) * kDensityScale);
-            if (q) atomicAdd(row + by, q);
+    Span sx[MAP_PER_LANE], sy[MAP_PER_LANE];
+    float r[MAP_PER_LANE];
+    int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1;
+#pragma unroll
+    for (int q = 0; q < MAP_PER_LANE; q++) {
+        const int i = base + q * EB + threadIdx.x;
+        r[q] = 0.f;
+        sx[q].b0 = 1;
+        sx[q].b1 = 0;                                             // empty unless a node is here
+        sy[q].b0 = 1;
+        sy[q].b1 = 0;
+        if (i < J.num_nodes) {
+            const float2 p = pos[i], sz = J.size[i];
+            r[q] = J.ratio[i];
+            sx[q] = make_span(p.x, sz.x, J.inv_bx, J.nbx);
+            sy[q] = make_span(p.y, sz.y, J.inv_by, J.nby);
+            if (sx[q].b0 <= sx[q].b1 && sy[q].b0 <= sy[q].b1) {
+                x0 = min(x0, sx[q].b0); x1 = max(x1, sx[q].b1);
+                y0 = min(y0, sy[q].b0); y1 = max(y1, sy[q].b1);
+            }
+        }
+    }
+    // bounding box of the workgroup's bins
+    for (int o = 32; o > 0; o >>= 1) {
+        x0 = min(x0, __shfl_xor(x0, o, 64)); y0 = min(y0, __shfl_xor(y0, o, 64));
+        x1 = max(x1, __shfl_xor(x1, o, 64)); y1 = max(y1, __shfl_xor(y1, o, 64));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wave] = x0; red[1][wave] = y0; red[2][wave] = x1; red[3][wave] = y1;
+    }
+    __syncthreads();
+    x0 = red[0][0]; y0 = red[1][0]; x1 = red[2][0]; y1 = red[3][0];
+    for (int w = 1; w < EB / 64; w++) {
+        x0 = min(x0, red[0][w]); y0 = min(y0, red[1][w]); x1 = max(x1, red[2][w]); y1 = max(y1, red[3][w]);
+    }
+    const int last = min(base + MAP_NODES, J.num_nodes) - 1;
+    const bool one_plane = (base < J.num_mov) == (last < J.num_mov);
+    const long long plane = base < J.num_mov ? 0 : (long long)J.nbx * J.nby;
+    const int w = x1 - x0 + 1, h = y1 - y0 + 1;
+    if (x1 < x0) return;                                           // no bins at all (uniform)
+    if (one_plane && (long long)w * h <= MAP_LDS) {
+        for (int e = threadIdx.x; e < w * h; e += EB) tile[e] = 0ull;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MAP_PER_LANE; q++)
+            for (int bx = sx[q].b0; bx <= sx[q].b1; bx++) {
+                const float wx = r[q] * span_overlap(sx[q], bx);
+                unsigned long long* row = tile + (bx - x0) * h - y0;
+                for (int by = sy[q].b0; by <= sy[q].b1; by++) {
+                    const unsigned long long c = contribution(wx, sy[q], by);
+                    if (c) atomicAdd(row + by, c);
+                }
+            }
+        __syncthreads();
+        unsigned long long* m = J.map + plane;
+        for (int e = threadIdx.x; e < w * h; e += EB) {
+            const unsigned long long v = tile[e];
+            if (v) {
+                const int bx = x0 + e / h, by = y0 + e % h;
+                atomicAdd(m + (long long)bx * J.nby + by, v);
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < MAP_PER_LANE; q++) {
+        const int i = base + q * EB + threadIdx.x;
+        unsigned long long* m = J.map + (i < J.num_mov ? 0 : (long long)J.nbx * J.nby);
+        for (int bx = sx[q].b0; bx <= sx[q].b1; bx++) {
+            const float wx = r[q] * span_overlap(sx[q], bx);
+            unsigned long long* row = m + (long long)bx * J.nby;
+            for (int by = sy[q].b0; by <= sy[q].b1; by++) {
+                const unsigned long long c = contribution(wx, sy[q], by);
+                if (c) atomicAdd(row + by, c);
+            }
         }
     }
 }
@@ -226,26 +306,44 @@ struct Segs {
     double over_scale;       // bin_area / mov_area of the grid the overflow is measured on
 };
 
-__device__ void reduce_segs(const Segs& s, double* out /* moved2 turned2 wa hpwl over */) {
+// First stage of the reduction of the partials: workgroup b sums the b-th fixed slice of every segment
+// (moved^2, turned^2, WA, HPWL, overflow) -> out[b][5]. The slices and the order inside them do not depend on the
+// launch, so the sums are deterministic.
+constexpr int RED_BLOCKS = 64;
+
+__device__ __forceinline__ void slice(int n, int& lo, int& hi) {
+    const int per = (n + RED_BLOCKS - 1) / RED_BLOCKS;
+    lo = min(n, (int)blockIdx.x * per);
+    hi = min(n, lo + per);
+}
+
+__global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) {
     __shared__ double red[16 * 5];
     double v[5] = {0, 0, 0, 0, 0};
-    for (int k = threadIdx.x; k < s.n_moved; k += blockDim.x) v[0] += s.moved[k];
-    for (int k = threadIdx.x; k < s.n_turned; k += blockDim.x) v[1] += s.turned[k];
-    for (int k = threadIdx.x; k < s.n_wa; k += blockDim.x) {
+    int lo, hi;
+    slice(s.n_moved, lo, hi);
+    for (int k = lo + threadIdx.x; k < hi; k += EB) v[0] += s.moved[k];
+    slice(s.n_turned, lo, hi);
+    for (int k = lo + threadIdx.x; k < hi; k += EB) v[1] += s.turned[k];
+    slice(s.n_wa, lo, hi);
+    for (int k = lo + threadIdx.x; k < hi; k += EB) {
         v[2] += s.wa[2 * k];
         v[3] += s.wa[2 * k + 1];
     }
-    for (int k = threadIdx.x; k < s.n_over; k += blockDim.x) v[4] += s.over[k];
+    slice(s.n_over, lo, hi);
+    for (int k = lo + threadIdx.x; k < hi; k += EB) v[4] += s.over[k];
     block_sum<5>(v, red);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 5; k++) out[k] = v[k];
+        for (int k = 0; k < 5; k++) out[blockIdx.x * 5 + k] = v[k];
 }
 
-__global__ __launch_bounds__(EB) void eng_control(Segs s, double* S, float* kp, const double* c, int* flags,
-                                                  double* stats) {
-    __shared__ double r[5];
-    reduce_segs(s, r);
+__global__ __launch_bounds__(64) void eng_control(const double* red5, double over_scale, double* S, float* kp,
+                                                  const double* c, int* flags, double* stats) {
     if (threadIdx.x == 0) {
+        double r[5] = {0, 0, 0, 0, 0};
+        for (int b = 0; b < RED_BLOCKS; b++)
+            for (int k = 0; k < 5; k++) r[k] += red5[b * 5 + k];
+        const Segs s{nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, over_scale};
         const double moved = sqrt(r[0]), turned = sqrt(r[1]), wa = r[2], hpwl = r[3], ovf = r[4] * s.over_scale;
         if (stats) {
             stats[0] = wa; stats[1] = hpwl; stats[2] = ovf; stats[3] = moved; stats[4] = turned;
@@ -288,6 +386,7 @@ struct Level {
     float *rho = nullptr, *fx = nullptr, *fy = nullptr;
     double* part = nullptr;
     double* stats = nullptr;
+    double* red5 = nullptr;    // first-stage sums, RED_BLOCKS x 5
     int* flags = nullptr;
     int nblk_n = 0, nblk_wa = 0, nblk_fpost = 0, nblk_jpost = 0, nblk_fmap = 0, nblk_jmap = 0;
     double *p_moved = nullptr, *p_wa = nullptr, *p_over = nullptr, *p_turned = nullptr;
@@ -351,8 +450,8 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipMalloc(&fy, nb * sizeof(float)));
     nblk_n = grid_of(N, EB);
     nblk_wa = grid_of((long long)d.num_pins + 1, WA_SPAN);
-    nblk_fmap = grid_of(N, EB);
-    nblk_jmap = nj ? grid_of(d.num_mov, EB) : 0;
+    nblk_fmap = grid_of(N, MAP_NODES);
+    nblk_jmap = nj ? grid_of(d.num_mov, MAP_NODES) : 0;
     nblk_fpost = grid_of(nb, EB);
     nblk_jpost = nj ? grid_of(nj, EB) : 0;
     const long long np = nblk_n + 2LL * nblk_wa + nblk_fpost + nblk_jpost + nblk_n;
@@ -363,6 +462,7 @@ int Level::create(const ap_level_desc& desc) {
     p_over = p_wa + 2LL * nblk_wa;
     p_turned = p_over + nblk_fpost + nblk_jpost;
     AP_HIP(hipMalloc(&stats, 8 * sizeof(double)));
+    AP_HIP(hipMalloc(&red5, RED_BLOCKS * 5 * sizeof(double)));
     AP_HIP(hipMalloc(&flags, 4 * sizeof(int)));
     AP_HIP(hipMemset(flags, 0, 4 * sizeof(int)));
     AP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -379,7 +479,7 @@ void Level::destroy() {
     if (graph) (void)hipGraphDestroy(graph);
     for (hipEvent_t& ev : pev)
         if (ev) (void)hipEventDestroy(ev);
-    void* ptrs[] = {u1, v1, g1, pin_grad, map, jmap, rho, fx, fy, part, stats, flags};
+    void* ptrs[] = {u1, v1, g1, pin_grad, map, jmap, rho, fx, fy, part, stats, flags, red5};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     poisson.destroy();
@@ -461,7 +561,9 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     mark(AP_PROF_CONTROL, s);
     Segs sg{p_moved, with_state ? nblk_n : 0, p_wa, nblk_wa, p_over, n_over, p_turned, g_prev ? nblk_n : 0,
             over_scale};
-    hipLaunchKernelGGL(eng_control, dim3(1), dim3(EB), 0, s, sg, S, kp_out, d.c, flags, stats);
+    hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(EB), 0, s, sg, red5);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red5, over_scale, S, kp_out, d.c, flags, stats);
     AP_LAUNCHED();
     mark(AP_PROF_CLASSES, s);
     return AP_OK;

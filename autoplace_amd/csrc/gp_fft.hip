@@ -1,19 +1,255 @@
-// FFT form of the Poisson solve (power-of-two grids). Not built yet: AUTO falls back to the GEMM form.
+// FFT form of the Poisson solve for power-of-two grids (2 .. 4096 per side).
+//
+// The cosine / sine series of gp_poisson.hip evaluated with LDS-resident complex FFTs, one workgroup per row:
+//   pass A  rows of rho (length M):   T = rho Cy^T                 DCT-II of two real rows in one complex FFT
+//   transpose T -> Tt [M][N]
+//   pass B  rows of Tt (length N):   C = DCT-II along x; Ax = wx C, Ay = wy C (weights computed in place);
+//                                     U^T = Sx^T Ax, V^T = Cx^T Ay  (sine and cosine series of one row from one
+//                                     complex inverse FFT: both outputs are real, so they share it)
+//   transpose U^T, V^T -> U, V [N][M]
+//   pass C  rows of U, V (length M):  field_x = U Cy, field_y = V Sy  (one inverse FFT per row)
+// DCT-II via Makhoul's reordering (v[n] = x[2n], v[L-1-n] = x[2n+1]; X[k] = Re(e^{-i pi k/2L} DFT(v)[k])); the
+// cosine series y[n] = sum_k a[k] cos(pi k (2n+1)/2L) via its inverse (V[k] = e^{i pi k/2L}(X'[k] - i X'[L-k]),
+// X'[0] = 2a[0]; y[2n] = v[n]/2, y[2n+1] = v[L-1-n]/2); the sine series as (-1)^n times the cosine series of the
+// reversed coefficients. Each pass reads and writes every element once; the map stays unmirrored.
+#include <math.h>
+
+#include <vector>
+
 #include "gp_common.h"
 #include "gp_poisson.h"
 
 namespace ap {
 
-bool Poisson::fft_supported(int) { return false; }
-bool Poisson::fft_max_ok() { return false; }
-int Poisson::fft_create() {
-    set_error("poisson: FFT method not available in this build");
-    return AP_ERR_UNSUPPORTED;
+constexpr int FB = 256;   // lanes per FFT workgroup
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-void Poisson::fft_destroy() {}
-int Poisson::fft_solve(const float*, int, float*, float*, int, hipStream_t) {
-    set_error("poisson: FFT method not available in this build");
-    return AP_ERR_UNSUPPORTED;
+__device__ __forceinline__ int bitrev(int n, int lg) { return (int)(__brev((unsigned)n) >> (32 - lg)); }
+// position of input element m in Makhoul's order: even m -> m / 2, odd m -> L - 1 - (m - 1) / 2
+__device__ __forceinline__ int makhoul(int m, int L) { return (m & 1) ? L - 1 - (m >> 1) : (m >> 1); }
+
+// in-place radix-2 decimation in time on bit-reversed input; tw[j] = e^{-2 pi i j / L} (j < L / 2) in LDS;
+// inverse: conjugate twiddles (no 1 / L scale)
+__device__ void fft_lds(float2* s, const float2* tw, int L, int lg, bool inverse) {
+    for (int st = 1; st <= lg; st++) {
+        const int half = 1 << (st - 1), stride = L >> st;
+        for (int b = threadIdx.x; b < (L >> 1); b += blockDim.x) {
+            const int j = b & (half - 1);
+            const int i0 = ((b >> (st - 1)) << st) + j, i1 = i0 + half;
+            float2 w = tw[j * stride];
+            if (inverse) w.y = -w.y;
+            const float2 a = s[i0], c = cmul(s[i1], w);
+            s[i0] = make_float2(a.x + c.x, a.y + c.y);
+            s[i1] = make_float2(a.x - c.x, a.y - c.y);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void load_twiddles(float2* tw_s, const float2* tw, int L) {
+    for (int j = threadIdx.x; j < (L >> 1); j += blockDim.x) tw_s[j] = tw[j];
+}
+
+// DCT-II of the rows r0 = 2 blockIdx.x, r0 + 1 (real, length L) -> out rows; spectrum of the pair from one FFT
+__device__ void dct2_pair_to_lds(const float* in0, const float* in1, float2* s, const float2* tw_s, const float2* ph,
+                                 int L, int lg, float* X0, float* X1) {
+    for (int m = threadIdx.x; m < L; m += blockDim.x) s[bitrev(makhoul(m, L), lg)] = make_float2(in0[m], in1[m]);
+    __syncthreads();
+    fft_lds(s, tw_s, L, lg, false);
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        const float2 Z = s[k], Zr = s[(L - k) & (L - 1)];
+        // V0 = (Z + conj Zr) / 2,  V1 = (Z - conj Zr) / 2i
+        const float v0r = 0.5f * (Z.x + Zr.x), v0i = 0.5f * (Z.y - Zr.y);
+        const float v1r = 0.5f * (Z.y + Zr.y), v1i = -0.5f * (Z.x - Zr.x);
+        const float2 p = ph[k];   // (cos, sin)(pi k / 2L)
+        X0[k] = p.x * v0r + p.y * v0i;
+        X1[k] = p.x * v1r + p.y * v1i;
+    }
+}
+
+__global__ __launch_bounds__(FB) void fft_pass_a(const float* rho, int ld_in, float* T, int ld_out, const float2* tw,
+                                                 const float2* ph, int L, int lg) {
+    extern __shared__ float2 lds[];
+    float2* s = lds;
+    float2* tw_s = lds + L;
+    load_twiddles(tw_s, tw, L);
+    const int r0 = 2 * blockIdx.x;
+    __syncthreads();
+    dct2_pair_to_lds(rho + (long long)r0 * ld_in, rho + (long long)(r0 + 1) * ld_in, s, tw_s, ph, L, lg,
+                     T + (long long)r0 * ld_out, T + (long long)(r0 + 1) * ld_out);
+}
+
+// cosine series of a_cos and sine series of a_sin (both real, length L, in LDS) from one inverse FFT:
+// h[k] = Vc[k] + i Vs[k], Vc / Vs Makhoul's inverse spectra of a_cos and of the reversed a_sin
+__device__ void cos_sin_series(const float* a_cos, const float* a_sin, float2* s, const float2* tw_s, const float2* ph,
+                               int L, int lg, float* y_cos, float* y_sin) {
+    for (int k = threadIdx.x; k < L; k += blockDim.x) {
+        const int kr = (L - k) & (L - 1);
+        // X'[k] - i X'[L-k] for the cosine input (X'[0] = 2 a[0], X'[L] = 0)
+        const float xc = k ? a_cos[k] : 2.f * a_cos[0], xcr = k ? a_cos[kr] : 0.f;
+        // reversed sine input: at[k] = a_sin[L - k] (k >= 1), at[0] = 0; X'[k] = at[k] (X'[0] = 0), X'[L-k] = at[L-k]
+        const float xs = k ? a_sin[kr] : 0.f, xsr = k ? a_sin[k] : 0.f;
+        const float2 p = ph[k];   // e^{i pi k / 2L}
+        const float2 vc = cmul(p, make_float2(xc, -xcr)), vs = cmul(p, make_float2(xs, -xsr));
+        s[bitrev(k, lg)] = make_float2(vc.x - vs.y, vc.y + vs.x);   // vc + i vs
+    }
+    __syncthreads();
+    fft_lds(s, tw_s, L, lg, true);
+    for (int j = threadIdx.x; j < L; j += blockDim.x) {
+        const float2 y = s[makhoul(j, L)];
+        y_cos[j] = 0.5f * y.x;
+        y_sin[j] = (j & 1) ? -0.5f * y.y : 0.5f * y.y;
+    }
+}
+
+struct PassB {
+    const float* Tt;       // [M][N]
+    float* Ut;             // [M][N]  sine series along x of Ax
+    float* Vt;             // [M][N]  cosine series along x of Ay
+    const float* omega;    // [N]  pi k / (N bx)
+    const float* nu;       // [M]  pi l / (M by)
+    float scale;           // 4 / (N M)
+    float bx;
+    int N, lgN;
+    const float2* tw;
+    const float2* ph;
+};
+
+__global__ __launch_bounds__(FB) void fft_pass_b(PassB a) {
+    extern __shared__ float2 lds[];
+    const int L = a.N;
+    float2* s = lds;
+    float2* tw_s = lds + L;
+    float* c0 = (float*)(tw_s + L / 2);
+    float* c1 = c0 + L;
+    float* ac = c1 + L;
+    float* as = ac + L;
+    load_twiddles(tw_s, a.tw, L);
+    const int l0 = 2 * blockIdx.x;
+    __syncthreads();
+    dct2_pair_to_lds(a.Tt + (long long)l0 * L, a.Tt + (long long)(l0 + 1) * L, s, tw_s, a.ph, L, a.lgN, c0, c1);
+    __syncthreads();
+    for (int q = 0; q < 2; q++) {
+        const int l = l0 + q;
+        const float* C = q ? c1 : c0;
+        const float nu = a.nu[l];
+        for (int k = threadIdx.x; k < L; k += blockDim.x) {
+            const float w = a.omega[k], den = (w * w + nu * nu) * a.bx;
+            const float inv = den > 0.f ? a.scale / den : 0.f;
+            ac[k] = nu * inv * (k == 0 ? 0.5f : 1.f) * C[k];   // Ay
+            as[k] = w * inv * (l == 0 ? 0.5f : 1.f) * C[k];    // Ax
+        }
+        __syncthreads();
+        cos_sin_series(ac, as, s, tw_s, a.ph, L, a.lgN, a.Vt + (long long)l * L, a.Ut + (long long)l * L);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V, float* fx, float* fy, int ld_out,
+                                                 const float2* tw, const float2* ph, int L, int lg) {
+    extern __shared__ float2 lds[];
+    float2* s = lds;
+    float2* tw_s = lds + L;
+    float* uc = (float*)(tw_s + L / 2);
+    float* vs = uc + L;
+    load_twiddles(tw_s, tw, L);
+    const int n = blockIdx.x;
+    for (int j = threadIdx.x; j < L; j += blockDim.x) {
+        uc[j] = U[(long long)n * L + j];
+        vs[j] = V[(long long)n * L + j];
+    }
+    __syncthreads();
+    cos_sin_series(uc, vs, s, tw_s, ph, L, lg, fx + (long long)n * ld_out, fy + (long long)n * ld_out);
+}
+
+// out[c][r] = in[r][c] for 2 planes (blockIdx.z), 32 x 32 tiles through LDS
+constexpr int TT = 32;
+__global__ __launch_bounds__(256) void transpose_kernel(const float* in0, const float* in1, float* out0, float* out1,
+                                                        int R, int C) {
+    __shared__ float tile[TT][TT + 1];
+    const float* in = blockIdx.z ? in1 : in0;
+    float* out = blockIdx.z ? out1 : out0;
+    const int c0 = blockIdx.x * TT, r0 = blockIdx.y * TT;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int k = ty; k < TT; k += 8) {
+        const int r = r0 + k, c = c0 + tx;
+        if (r < R && c < C) tile[k][tx] = in[(long long)r * C + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < TT; k += 8) {
+        const int c = c0 + k, r = r0 + tx;
+        if (r < R && c < C) out[(long long)c * R + r] = tile[tx][k];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------ the plan
+
+bool Poisson::fft_supported(int n) { return n >= 2 && n <= fft_max && (n & (n - 1)) == 0; }
+bool Poisson::fft_max_ok() { return true; }
+
+static int ilog2(int n) {
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return l;
+}
+
+int Poisson::fft_create() {
+    // per side: twiddles e^{-2 pi i j / L} (L / 2) then phases e^{i pi k / 2L} (L); omega / nu tables
+    std::vector<float2> h;
+    auto add = [&](int L) {
+        for (int j = 0; j < L / 2; j++) h.push_back(make_float2((float)cos(-2 * M_PI * j / L), (float)sin(-2 * M_PI * j / L)));
+        for (int k = 0; k < L; k++) h.push_back(make_float2((float)cos(M_PI * k / (2.0 * L)), (float)sin(M_PI * k / (2.0 * L))));
+    };
+    add(N);
+    add(M);
+    std::vector<float> om(N), nu(M);
+    for (int k = 0; k < N; k++) om[k] = (float)(M_PI * k / (N * bx));
+    for (int l = 0; l < M; l++) nu[l] = (float)(M_PI * l / (M * by));
+    AP_HIP(hipMalloc(&d_tw, h.size() * sizeof(float2)));
+    AP_HIP(hipMemcpy(d_tw, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
+    AP_HIP(hipMalloc(&d_omega, (N + M) * sizeof(float)));
+    AP_HIP(hipMemcpy(d_omega, om.data(), N * sizeof(float), hipMemcpyHostToDevice));
+    AP_HIP(hipMemcpy(d_omega + N, nu.data(), M * sizeof(float), hipMemcpyHostToDevice));
+    return AP_OK;
+}
+
+void Poisson::fft_destroy() {
+    if (d_tw) (void)hipFree(d_tw);
+    if (d_omega) (void)hipFree(d_omega);
+    d_tw = nullptr;
+    d_omega = nullptr;
+}
+
+int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s) {
+    const size_t nm = (size_t)N * M;
+    const int lgN = ilog2(N), lgM = ilog2(M);
+    const float2* twN = d_tw;
+    const float2* phN = d_tw + N / 2;
+    const float2* twM = d_tw + N / 2 + N;
+    const float2* phM = twM + M / 2;
+    float *w0 = d_work, *w1 = d_work + nm, *w2 = d_work + 2 * nm, *w3 = d_work + 3 * nm;
+    // pass A: T = rho Cy^T (rows of length M), into w0
+    hipLaunchKernelGGL(fft_pass_a, dim3(N / 2), dim3(FB), (size_t)(M + M / 2) * sizeof(float2), s, rho, rho_ld, w0,
+                       M, twM, phM, M, lgM);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(transpose_kernel, dim3(grid_of(M, TT), grid_of(N, TT), 1), dim3(256), 0, s, w0, w0, w1, w1, N,
+                       M);
+    AP_LAUNCHED();
+    // pass B: rows of Tt (length N) -> Ut (w2), Vt (w3)
+    PassB b{w1, w2, w3, d_omega, d_omega + N, (float)(4.0 / ((double)N * M)), (float)bx, N, lgN, twN, phN};
+    hipLaunchKernelGGL(fft_pass_b, dim3(M / 2), dim3(FB), (size_t)(N + N / 2) * sizeof(float2) + 4 * N * sizeof(float),
+                       s, b);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(transpose_kernel, dim3(grid_of(N, TT), grid_of(M, TT), 2), dim3(256), 0, s, w2, w3, w0, w1, M,
+                       N);
+    AP_LAUNCHED();
+    // pass C: rows of U (w0), V (w1) -> field
+    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), (size_t)(M + M / 2) * sizeof(float2) + 2 * M * sizeof(float), s,
+                       w0, w1, fx, fy, fld, twM, phM, M, lgM);
+    AP_LAUNCHED();
+    return AP_OK;
 }
 
 }  // namespace ap

@@ -11,11 +11,13 @@ void poisson_weights(int N, int M, double bx, double by, std::vector<float>& wx,
 struct Poisson {
     int N = 0, M = 0, method = 0;
     double bx = 1, by = 1;
-    float *d_wx = nullptr, *d_wy = nullptr, *d_work = nullptr;
+    float *d_wx = nullptr, *d_wy = nullptr, *d_work = nullptr, *d_mean = nullptr;
+    double* d_part = nullptr;
     // GEMM method: Cx, Cx^T, Sx^T (N x N); Cy, Sy, Cy^T (M x M)
     float *d_cx = nullptr, *d_cxt = nullptr, *d_sxt = nullptr, *d_cy = nullptr, *d_sy = nullptr, *d_cyt = nullptr;
     // FFT method: twiddles
     float2* d_tw = nullptr;
+    float* d_omega = nullptr;   // FFT method: pi k / (N bx) then pi l / (M by)
     static constexpr int fft_min = 64;       // AUTO picks the FFT from this side up
     static constexpr int fft_max = 4096;     // largest side the LDS FFT holds
     static bool fft_supported(int n);

@@ -33,6 +33,7 @@ struct GemmProblem {
     const float* W1;
     const float* W2;
     int lda, ldb, ldc;
+    const float* a_shift;  // optional device scalar subtracted from every element of A as it is loaded
 };
 struct GemmBatch {
     GemmProblem p[2];
@@ -48,6 +49,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmBatch batch) {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const int m0 = blockIdx.y * GB_M, n0 = blockIdx.x * GB_N;
+    const float shift = pr.a_shift ? *pr.a_shift : 0.f;
     floatx16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmBatch batch) {
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int gk = k0 + kc + q;
-                As[kc + q][r] = (gm < M && gk < K) ? pr.A[(long long)gm * pr.lda + gk] : 0.f;
+                As[kc + q][r] = (gm < M && gk < K) ? pr.A[(long long)gm * pr.lda + gk] - shift : 0.f;
             }
         }
         // B tile: 16 k x 128 cols; thread loads 8 values (row t / 16, cols (t & 15) * 8)
@@ -110,6 +112,28 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(GemmBatch batch) {
                     }
                 }
             }
+}
+
+// mean of the map (two fixed-order stages, FP64): the cosine series of the zero-mean map has the same field (the
+// spectral weight of the constant mode is 0) and far less cancellation in FP32 at low frequencies
+constexpr int MEAN_BLOCKS = 256;
+__global__ __launch_bounds__(256) void map_sum_kernel(const float* rho, int ld, int N, int M, double* part) {
+    __shared__ double red[16];
+    double v[1] = {0.0};
+    const long long total = (long long)N * M;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        const int r = (int)(e / M), c = (int)(e - (long long)r * M);
+        v[0] += rho[(long long)r * ld + c];
+    }
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+__global__ __launch_bounds__(256) void map_mean_kernel(const double* part, int n, double inv_count, float* mean) {
+    __shared__ double red[16];
+    double v[1] = {0.0};
+    for (int k = threadIdx.x; k < n; k += 256) v[0] += part[k];
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) *mean = (float)(v[0] * inv_count);
 }
 
 static int launch_gemm(const GemmBatch& b, int nbatch, hipStream_t s) {
@@ -179,6 +203,8 @@ int Poisson::create(int nbx, int nby, double bx_, double by_, int method_) {
     if ((st = upload(wx, &d_wx)) || (st = upload(wy, &d_wy))) return st;
     const size_t nm = (size_t)N * M;
     AP_HIP(hipMalloc(&d_work, 4 * nm * sizeof(float)));
+    AP_HIP(hipMalloc(&d_part, MEAN_BLOCKS * sizeof(double)));
+    AP_HIP(hipMalloc(&d_mean, sizeof(float)));
     if (method == AP_POISSON_GEMM) {
         std::vector<float> cx, sx, cxt, sxt, cy, sy, cyt, syt;
         trig(N, cx, sx, cxt, sxt);
@@ -193,8 +219,8 @@ int Poisson::create(int nbx, int nby, double bx_, double by_, int method_) {
 }
 
 void Poisson::destroy() {
-    float* ptrs[] = {d_wx, d_wy, d_work, d_cx, d_cxt, d_sxt, d_cy, d_sy, d_cyt};
-    for (float* p : ptrs)
+    void* ptrs[] = {d_wx, d_wy, d_work, d_cx, d_cxt, d_sxt, d_cy, d_sy, d_cyt, d_part, d_mean};
+    for (void* p : ptrs)
         if (p) (void)hipFree(p);
     fft_destroy();
 }
@@ -205,10 +231,14 @@ int Poisson::solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, 
     float *T1 = d_work, *Ax = d_work + nm, *Ay = d_work + 2 * nm, *U = d_work + 3 * nm;
     float* V = T1;   // T1 is dead once Ax / Ay exist
     int st;
+    hipLaunchKernelGGL(map_sum_kernel, dim3(MEAN_BLOCKS), dim3(256), 0, s, rho, rho_ld, N, M, d_part);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(map_mean_kernel, dim3(1), dim3(256), 0, s, d_part, MEAN_BLOCKS, 1.0 / ((double)N * M), d_mean);
+    AP_LAUNCHED();
     GemmBatch b{};
-    // T1 = rho Cy^T  [N x M] = [N x M][M x M]
+    // T1 = (rho - mean) Cy^T  [N x M] = [N x M][M x M]
     b.M = N; b.N = M; b.K = M; b.epilogue = 0;
-    b.p[0] = GemmProblem{rho, d_cyt, T1, nullptr, nullptr, nullptr, rho_ld, M, M};
+    b.p[0] = GemmProblem{rho, d_cyt, T1, nullptr, nullptr, nullptr, rho_ld, M, M, d_mean};
     if ((st = launch_gemm(b, 1, s))) return st;
     // C = Cx T1 [N x M] = [N x N][N x M]; Ax = wx C, Ay = wy C
     b.M = N; b.N = M; b.K = N; b.epilogue = 1;

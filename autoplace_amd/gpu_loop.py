@@ -12,6 +12,7 @@ import math
 import torch
 
 from . import _lib
+from .locality import morton_order, renumber_netlist
 
 FIELDS = ("lam gam alpha a iter hpwl_prev motion ww mu settled best_hpwl best_ovf is_best improved stop accept "
           "loop_it evals res_hpwl res_ovf has_best").split()
@@ -33,35 +34,53 @@ def _p(t):
 
 
 class Engine(object):
-    """A level's problem bound to the native GP iteration engine (ap_level). Keeps every tensor it hands to the
-    library alive."""
+    """A level's problem bound to the native GP iteration engine (ap_level).
 
-    def __init__(self, wirelength, density, u, v, g, best, S, kp, c, poisson="auto"):
+    With reorder=True (default) the engine works on a spatially renumbered copy of the level (locality.py): the
+    node buffers u / v / g / best are permuted in, and `finish()` writes them back in the caller's order. Keeps
+    every tensor it hands to the library alive."""
+
+    def __init__(self, wirelength, density, u, v, g, best, S, kp, c, poisson="auto", reorder=True):
         w, d, j = wirelength.opt, density, density.judge
-        self._keep = [w["cell32"], w["off"], w["net_start"], w["cell_start"], w["cell_pin"], u, v, g, best, S, kp, c]
+        self.user = (u, v, g, best)
+        n, N = d.num_mov, d.num_nodes
+        if reorder:
+            perm = morton_order(v, d.nbx, d.nby, float(d.bin_np[0]), float(d.bin_np[1]), n)
+            net = renumber_netlist(w["cell32"], w["off"], w["net_start"], n, perm)
+        else:
+            perm = None
+            net = w
+        pm = (lambda t: t[perm].contiguous()) if reorder else (lambda t: t)           # noqa: E731
+        pc = (lambda t: t[perm[:n]].contiguous()) if reorder else (lambda t: t)       # noqa: E731
+        self.perm = perm
+        self.u, self.v, self.g, self.best = pm(u), pm(v), pm(g), pm(best)
+        arrays = dict(size=pm(d.smooth["size"]), ratio=pm(d.smooth["ratio"]), pins=pm(d.pins), area=pm(d.area),
+                      lo=pm(d.pos_lo), hi=pm(d.pos_hi))
+        self._keep = [net["cell32"], net["off"], net["net_start"], net["cell_start"], net["cell_pin"], self.u, self.v,
+                      self.g, self.best, S, kp, c, d.fixed_map] + list(arrays.values())
         for t in self._keep:
             if not t.is_contiguous():
                 raise ValueError("engine tensors must be contiguous")
         desc = _lib.LevelDesc()
-        desc.num_nodes, desc.num_mov = d.num_nodes, d.num_mov
-        desc.num_pins, desc.num_nets = w["cell32"].numel(), w["num_nets"]
-        desc.pin_cell, desc.pin_off, desc.net_start = _p(w["cell32"]), _p(w["off"]), _p(w["net_start"])
-        desc.cell_start, desc.cell_pin = _p(w["cell_start"]), _p(w["cell_pin"])
-        desc.size, desc.ratio = _p(d.smooth["size"]), _p(d.smooth["ratio"])
-        desc.pins, desc.area, desc.lo, desc.hi = _p(d.pins), _p(d.area), _p(d.pos_lo), _p(d.pos_hi)
+        desc.num_nodes, desc.num_mov = N, n
+        desc.num_pins, desc.num_nets = net["cell32"].numel(), net["num_nets"]
+        desc.pin_cell, desc.pin_off, desc.net_start = _p(net["cell32"]), _p(net["off"]), _p(net["net_start"])
+        desc.cell_start, desc.cell_pin = _p(net["cell_start"]), _p(net["cell_pin"])
+        desc.size, desc.ratio = _p(arrays["size"]), _p(arrays["ratio"])
+        desc.pins, desc.area, desc.lo, desc.hi = _p(arrays["pins"]), _p(arrays["area"]), _p(arrays["lo"]), \
+            _p(arrays["hi"])
         desc.nbx, desc.nby = d.nbx, d.nby
         desc.bx, desc.by = float(d.bin_np[0]), float(d.bin_np[1])
         desc.fixed, desc.target = _p(d.fixed_map), float(d.target)
         desc.bin_area, desc.mov_area = d.bin_area, d.mov_area
-        self._keep += [d.smooth["size"], d.smooth["ratio"], d.pins, d.area, d.pos_lo, d.pos_hi, d.fixed_map]
         if j is not None:
+            js, jr = pc(j.smooth["size"]), pc(j.smooth["ratio"])
+            self._keep += [js, jr, j.fixed_map]
             desc.has_judge, desc.njx, desc.njy = 1, j.nbx, j.nby
             desc.jbx, desc.jby, desc.judge_bin_area = float(j.bin_np[0]), float(j.bin_np[1]), j.bin_area
-            desc.judge_size, desc.judge_ratio, desc.judge_fixed = _p(j.smooth["size"]), _p(j.smooth["ratio"]), \
-                _p(j.fixed_map)
-            self._keep += [j.smooth["size"], j.smooth["ratio"], j.fixed_map]
+            desc.judge_size, desc.judge_ratio, desc.judge_fixed = _p(js), _p(jr), _p(j.fixed_map)
         desc.poisson_method = {"auto": 0, "gemm": 1, "fft": 2}[poisson]
-        desc.u, desc.v, desc.g, desc.best = _p(u), _p(v), _p(g), _p(best)
+        desc.u, desc.v, desc.g, desc.best = _p(self.u), _p(self.v), _p(self.g), _p(self.best)
         desc.S, desc.kp, desc.c = _p(S), _p(kp), _p(c)
         self.desc = desc
         h = ctypes.c_void_p()
@@ -71,6 +90,13 @@ class Engine(object):
     def run(self, passes, use_graph=True):
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         _lib.check(_lib.lib().ap_level_run(self._h, passes, int(use_graph), stream), "level_run")
+
+    def finish(self):
+        """Write u / v / g / best back into the caller's tensors (caller's node order)."""
+        if self.perm is None:
+            return
+        for mine, theirs in zip((self.u, self.v, self.g, self.best), self.user):
+            theirs[self.perm] = mine
 
     def profile(self, passes):
         """Average ms per pass of every kernel class (direct launches with HIP events). -> dict."""
@@ -85,6 +111,7 @@ class Engine(object):
         return ms.value
 
     def objective(self, pos, grad, stats):
+        """One objective evaluation at pos (in the engine's node order) with the parameters in kp."""
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         _lib.check(_lib.lib().ap_level_objective(self._h, _p(pos), _p(grad), _p(stats), stream), "level_objective")
 
@@ -117,7 +144,7 @@ def control_state(ps, optimizer, iteration, args):
 
 
 def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every=16, use_graph=True,
-                     poisson="auto"):
+                     poisson="auto", reorder=True):
     """Continue the level from `iteration` (density weight on) until it stops; updates optimizer, ps and
     objective.evals as the host loop would. -> (reason, iterations, hpwl, overflow) of the kept placement."""
     device, pos = optimizer.v.device, optimizer.v
@@ -128,7 +155,7 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
     u = optimizer.u.contiguous()
     g = optimizer.g.contiguous()
     best = pos.clone()
-    eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson)
+    eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
     evals_before, logged = objective.evals, iteration // args.log_freq
     watch = torch.tensor([K["stop"], K["loop_it"], K["lam"], K["gam"], K["res_hpwl"], K["res_ovf"]], device=device)
     try:
@@ -140,6 +167,7 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
                 log("iter: %d | hpwl: %.4E overflow: %.4f density_weight: %.4E wa_coeff: %.4E" % (it, hpwl, ovf, lam, gam))
             if stop:
                 break
+        eng.finish()
     finally:
         torch.cuda.synchronize(device)
         eng.close()

@@ -1,0 +1,62 @@
+"""Spatial renumbering of a level's nodes and nets for the device engine.
+
+The engine's scatter, WA and gather kernels read node data by index. In the problem's own order, the nodes of a
+workgroup (and the cells of a net) are spread over the whole die, so every access is a separate cache line and every
+density contribution a separate global atomic. Renumbering cells (and, separately, fillers) along a Morton curve of
+their bins at the start of a level makes index neighbours spatial neighbours; nets are renumbered by their first
+cell in the new order so that the pins of neighbouring nets, and the pins a cell gathers, are close too.
+
+The renumbering is a permutation: the integer density sums do not change, per-net and per-cell sums keep their pin
+order within the net / cell, so results change only by the order of FP64 norm sums. It is deterministic (stable
+sorts of integer keys).
+"""
+import torch
+
+
+def _spread_bits(v):
+    v = v.long() & 0xFFFF
+    v = (v | (v << 8)) & 0x00FF00FF
+    v = (v | (v << 4)) & 0x0F0F0F0F
+    v = (v | (v << 2)) & 0x33333333
+    v = (v | (v << 1)) & 0x55555555
+    return v
+
+
+def morton_order(pos, nbx, nby, bx, by, num_mov):
+    """Permutation new -> old of the nodes: cells, then fillers, each sorted by the Morton code of their bin."""
+    ix = (pos[:, 0] / bx).floor().clamp_(0, nbx - 1)
+    iy = (pos[:, 1] / by).floor().clamp_(0, nby - 1)
+    key = _spread_bits(ix) | (_spread_bits(iy) << 1)
+    cells = torch.argsort(key[:num_mov], stable=True)
+    fill = torch.argsort(key[num_mov:], stable=True) + num_mov
+    return torch.cat([cells, fill])
+
+
+def renumber_netlist(cell32, off, net_start, num_mov, perm):
+    """Pins grouped by net (optimized-net CSR) -> the same netlist with cells renumbered by `perm` (new -> old) and
+    nets ordered by their smallest new cell index. -> dict(cell32, off, net_start, cell_start, cell_pin)."""
+    dev = cell32.device
+    inv = torch.empty(num_mov + 1, dtype=torch.long, device=dev)
+    inv[perm[:num_mov]] = torch.arange(num_mov, device=dev)
+    inv[num_mov] = num_mov                                     # a fixed pin stays a fixed pin
+    new_cell = inv[cell32.long()]
+    ns = net_start.long()
+    deg = ns[1:] - ns[:-1]
+    nets = len(deg)
+    pins = len(cell32)
+    net_of = torch.repeat_interleave(torch.arange(nets, device=dev), deg)
+    key = torch.full((nets,), num_mov + 1, dtype=torch.long, device=dev)
+    key.scatter_reduce_(0, net_of, new_cell, "amin", include_self=True)
+    order = torch.argsort(key, stable=True)
+    deg_new = deg[order]
+    start_new = torch.zeros(nets + 1, dtype=torch.long, device=dev)
+    start_new[1:] = torch.cumsum(deg_new, 0)
+    src = torch.repeat_interleave(ns[:-1][order], deg_new) + \
+        (torch.arange(pins, device=dev) - torch.repeat_interleave(start_new[:-1], deg_new))
+    c = new_cell[src]
+    o = off[src].contiguous()
+    by_cell = torch.argsort(c, stable=True)
+    cell_start = torch.searchsorted(c[by_cell], torch.arange(num_mov + 1, device=dev))
+    return {"cell32": c.int().contiguous(), "off": o, "net_start": start_new.int().contiguous(),
+            "cell_start": cell_start.int().contiguous(),
+            "cell_pin": by_cell[:int(cell_start[-1])].int().contiguous(), "num_nets": nets}

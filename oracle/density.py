@@ -140,7 +140,7 @@ def density_grad(pos, size, ratio, field_x, field_y, wl_grad, pins, area, weight
 def brute_overlap_map(pos, size, ratio, bx, by, nbx, nby):
     """Per-bin charge by direct rectangle intersection in float64 (tiny grids only)."""
     out = np.zeros((nbx, nby))
-    for (cx, cy), (w, h), r in zip(np.asarray(pos, np.float64), np.asarray(size, np.float64), ratio):
+    for (cx, cy), (w, h), r in zip(np.asarray(pos, np.float64), np.asarray(size, np.float64), np.asarray(ratio, np.float64)):
         x0, x1, y0, y1 = cx - w / 2, cx + w / 2, cy - h / 2, cy + h / 2
         for i in range(nbx):
             ox = max(0.0, min(x1, (i + 1) * bx) - max(x0, i * bx))

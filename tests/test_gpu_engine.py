@@ -43,18 +43,18 @@ def _oracle_level(prob, args, data):
                      judge)
 
 
-def _engine(wl, den, pos, S, kp, c, g0):
+def _engine(wl, den, pos, S, kp, c, g0, reorder=True, poisson="gemm"):
     from autoplace_amd.gpu_loop import Engine
     t = lambda a, dt=torch.float32: torch.tensor(np.asarray(a), dtype=dt, device=DEV)   # noqa: E731
     u, v, g, best = t(pos), t(pos), t(g0), t(pos)
     dS, dkp, dc = t(S, torch.float64), t(kp), t(c, torch.float64)
-    return Engine(wl, den, u, v, g, best, dS, dkp, dc, "gemm"), (u, v, g, best, dS, dkp, dc)
+    return Engine(wl, den, u, v, g, best, dS, dkp, dc, poisson, reorder), (u, v, g, best, dS, dkp, dc)
 
 
 def test_engine_objective_vs_oracle(small_design):
     _, _, data = small_design
     args, prob, wl, den, pos, S, kp, c = _setup(data)
-    eng, (u, v, g, best, dS, dkp, dc) = _engine(wl, den, pos, S, kp, c, np.zeros_like(pos))
+    eng, (u, v, g, best, dS, dkp, dc) = _engine(wl, den, pos, S, kp, c, np.zeros_like(pos), reorder=False)
     grad = torch.empty_like(v)
     stats = torch.zeros(8, dtype=torch.float64, device=DEV)
     eng.objective(v, grad, stats)
@@ -70,16 +70,20 @@ def test_engine_objective_vs_oracle(small_design):
     eng.close()
 
 
-def _run(data, passes, use_graph, calls=1):
+def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm"):
     args, prob, wl, den, pos, S, kp, c = _setup(data)
     g0 = np.zeros_like(pos)
-    eng, bufs = _engine(wl, den, pos, S, kp, c, g0)
+    eng, bufs = _engine(wl, den, pos, S, kp, c, g0, reorder=False, poisson=poisson)
     grad = torch.empty_like(bufs[1])
     stats = torch.zeros(8, dtype=torch.float64, device=DEV)
-    eng.objective(bufs[1], grad, stats)            # g at the start point
-    bufs[2].copy_(grad)
+    eng.objective(bufs[1], grad, stats)            # g at the start point (caller's order)
+    torch.cuda.synchronize()
+    eng.close()
+    g0 = grad.cpu().numpy()
+    eng, bufs = _engine(wl, den, pos, S, kp, c, g0, reorder=reorder, poisson=poisson)
     for _ in range(calls):
         eng.run(passes, use_graph)
+    eng.finish()
     torch.cuda.synchronize()
     out = [b.cpu().numpy().copy() for b in bufs]
     eng.close()
@@ -93,15 +97,22 @@ def test_engine_graph_equals_direct_and_is_deterministic(small_design):
     c, _ = _run(data, 16, True, calls=2)
     for x, y, z in zip(a, b, c):
         assert np.array_equal(x, y) and np.array_equal(y, z)
+    # without the spatial renumbering the same passes give the same control decisions (FP64 sums in another order)
+    d, _ = _run(data, 16, True, calls=2, reorder=False)
+    K = octl.K
+    for name in ("iter", "loop_it", "evals", "stop"):
+        assert d[4][K[name]] == a[4][K[name]]
+    assert norm_err(d[1], a[1]) < 1e-4
     K = octl.K
     assert a[4][K["loop_it"]] == 40 + 32 and a[4][K["evals"]] == 32
 
 
-def test_engine_trajectory_vs_oracle(small_design):
+@pytest.mark.parametrize("poisson", ["gemm", "fft"])
+def test_engine_trajectory_vs_oracle(small_design, poisson):
     """16 passes of the engine against 16 oracle passes from the same state: the control decisions agree and the
     points stay within FP32 drift of each other."""
     _, _, data = small_design
-    (u, v, g, best, S_dev, kp_dev, _), (args, prob, pos, S, kp, c, g0) = _run(data, 16, True)
+    (u, v, g, best, S_dev, kp_dev, _), (args, prob, pos, S, kp, c, g0) = _run(data, 16, True, poisson=poisson)
     lvl = _oracle_level(prob, args, data)
     S = list(S)
     kp = np.array(kp, np.float32)

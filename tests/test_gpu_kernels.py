@@ -102,6 +102,45 @@ def test_poisson_gemm(shape):
     assert e64 < 1e-5
 
 
+@pytest.mark.parametrize("shape", [(2, 2, 1.0, 1.0), (16, 16, 3.0, 3.0), (64, 32, 12.5, 4.0), (128, 128, 6.3, 6.3),
+                                   (512, 512, 1.7, 1.7), (2048, 2048, 6.0, 6.0)])
+def test_poisson_fft(shape):
+    from autoplace_amd import gp_cuda
+    N, M, bx, by = shape
+    rng = np.random.default_rng(N * 3 + M)
+    rho = (rng.random((N, M)) * 2).astype(np.float32)
+    ref = opo.field_mirrored_fft(oden.mirror(rho.astype(np.float64)), opo.to_field(N, M, bx, by))[:, :N, :M]
+    plan = gp_cuda.PoissonPlan(N, M, bx, by, "fft")
+    assert plan.method == "fft"
+    f = plan.field(_t(rho)).cpu().numpy()
+    e64 = norm_err(f, ref)
+    print("poisson fft %dx%d: err vs fp64 %.2e" % (N, M, e64))
+    assert e64 < 1e-5
+
+
+def test_poisson_methods_agree_on_a_cosine_mode():
+    """rho = 1 + cos(pi p (n + 1/2) / N) along x has the field field_x = sin(pi p (n + 1/2) / N) / (w_p bx) with
+    w_p = pi p / (N bx), and no field along y (both methods)."""
+    from autoplace_amd import gp_cuda
+    N = M = 64
+    bx = by = 2.0
+    x = np.arange(N)
+    rho = np.repeat(np.cos(np.pi * 3 * (x + 0.5) / N)[:, None], M, 1).astype(np.float32) + 1.0
+    fg = gp_cuda.PoissonPlan(N, M, bx, by, "gemm").field(_t(rho)).cpu().numpy()
+    ff = gp_cuda.PoissonPlan(N, M, bx, by, "fft").field(_t(rho)).cpu().numpy()
+    w = np.pi * 3 / (N * bx)
+    expect = np.repeat((np.sin(np.pi * 3 * (x + 0.5) / N) * w / (w * w * bx))[:, None], M, 1)
+    assert norm_err(ff[0], expect) < 1e-5 and norm_err(fg[0], expect) < 1e-5
+    assert np.abs(ff[1]).max() < 1e-5 * np.abs(expect).max() and np.abs(fg[1]).max() < 1e-5 * np.abs(expect).max()
+
+
+def test_poisson_uniform_map_has_no_field():
+    from autoplace_amd import gp_cuda
+    for method in ("gemm", "fft"):
+        f = gp_cuda.PoissonPlan(32, 32, 1.5, 1.5, method).field(_t(np.full((32, 32), 0.7, np.float32))).cpu().numpy()
+        assert np.abs(f).max() < 1e-6
+
+
 def test_poisson_mirrored_layout(small):
     """poisson_field on the reference's mirrored map writes the [:, :nbx, :nby] quadrant the gather reads."""
     from autoplace_amd import gp_cuda

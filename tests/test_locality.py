@@ -1,0 +1,47 @@
+"""The engine's spatial renumbering (CPU tensors): a permutation that keeps every net and every cell's pins."""
+import numpy as np
+import torch
+
+
+def test_morton_order_is_a_permutation_keeping_cells_first():
+    from autoplace_amd.locality import morton_order
+    rng = np.random.default_rng(0)
+    pos = torch.tensor(rng.uniform(0, 100, (1000, 2)), dtype=torch.float32)
+    perm = morton_order(pos, 64, 64, 100 / 64, 100 / 64, 700)
+    assert sorted(perm.tolist()) == list(range(1000))
+    assert perm[:700].max() < 700 and perm[700:].min() >= 700
+    # index neighbours are spatial neighbours: consecutive cells are close on average
+    p = pos[perm[:700]]
+    assert (p[1:] - p[:-1]).norm(dim=1).mean() < 10
+
+
+def test_renumbered_netlist_has_the_same_nets():
+    from autoplace_amd import synth
+    from autoplace_amd.database import PlaceData
+    from autoplace_amd.locality import morton_order, renumber_netlist
+    from autoplace_amd.wirelength import net_csr
+    lef, d = synth.generate(800, seed=4, io_pins=5)
+    data = PlaceData(lef, d)
+    n = data.num_mov
+    csr = net_csr(data.pin_cell, data.pin_off, data.pin_net, data.num_nets, n, 100)
+    t = lambda a: torch.as_tensor(a)   # noqa: E731
+    pos = torch.tensor(np.random.default_rng(1).uniform(0, data.width, (n + 50, 2)), dtype=torch.float32)
+    perm = morton_order(pos, 32, 32, data.width / 32, data.height / 32, n)
+    new = renumber_netlist(t(csr["cell"]), t(csr["off"]), t(csr["net_start"]), n, perm)
+    old_of = torch.cat([perm[:n], torch.tensor([n])])
+
+    def nets(cell, off, start):
+        out = set()
+        for i in range(len(start) - 1):
+            s, e = int(start[i]), int(start[i + 1])
+            out.add(tuple((int(cell[k]), tuple(np.asarray(off[k]).tolist())) for k in range(s, e)))
+        return out
+
+    before = nets(csr["cell"], csr["off"], csr["net_start"])
+    after = nets(old_of[new["cell32"].long()].numpy(), new["off"].numpy(), new["net_start"].numpy())
+    assert before == after
+    # cell CSR of the new numbering: every movable pin once, grouped by cell
+    cs, cp = new["cell_start"].numpy(), new["cell_pin"].numpy()
+    assert sorted(cp.tolist()) == sorted(np.nonzero(new["cell32"].numpy() < n)[0].tolist())
+    for c in range(0, n, 37):
+        assert np.all(new["cell32"].numpy()[cp[cs[c]:cs[c + 1]]] == c)
