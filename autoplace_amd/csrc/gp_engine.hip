@@ -17,6 +17,7 @@
 #include "gp_control.h"
 #include "gp_kernels.h"
 #include "gp_poisson.h"
+#include "gp_wa.h"
 
 namespace ap {
 
@@ -75,8 +76,12 @@ struct MapJob {
     int num_nodes, num_mov;
     float inv_bx, inv_by;
     int nbx, nby;
-    unsigned long long* map;
-    int blocks;          // workgroups of this job
+    unsigned long long* map;   // fallback accumulation (global integer atomics), read and cleared by the merge
+    int blocks;                // workgroups of this job
+    int first_block;           // index of the job's first workgroup among all map workgroups
+    unsigned long long* slots; // per workgroup: its LDS tile (MAP_LDS values), plain stores
+    int4* boxes;               // per workgroup: x0, y0, w, h | plane << 20 (w = 0: nothing in a slot)
+    int* fallback;             // set to 1 when a workgroup used the global atomics
 };
 
 // A workgroup takes MAP_NODES consecutive nodes (spatial neighbours after the engine's Morton renumbering). When
@@ -85,26 +90,33 @@ struct MapJob {
 // Integer sums either way: the map is the same whatever the path and the order.
 constexpr int MAP_PER_LANE = 4;
 constexpr int MAP_NODES = EB * MAP_PER_LANE;
-constexpr int MAP_LDS = 4096;
+constexpr int MAP_LDS = 16384;   // 128 KB: a workgroup's nodes where the density is low (fillers) still fit
 
 __device__ __forceinline__ unsigned long long contribution(float wx, const Span& sy, int by) {
     return (unsigned long long)((double)(wx * span_overlap(sy, by)) * kDensityScale);
 }
 
 __global__ __launch_bounds__(EB) void eng_density_map(Pick pos_p, MapJob f, MapJob j) {
-    __shared__ unsigned long long tile[MAP_LDS];
+    extern __shared__ unsigned long long tile[];   // MAP_LDS entries (dynamic: above the static LDS limit)
     __shared__ int red[4][EB / 64];
     const bool judge = (int)blockIdx.x >= f.blocks;
     const MapJob& J = judge ? j : f;
     const int base = ((int)blockIdx.x - (judge ? f.blocks : 0)) * MAP_NODES;
-    if (base >= J.num_nodes) return;                               // uniform over the workgroup
+    if (base >= J.num_nodes) {                                     // uniform over the workgroup
+        if (threadIdx.x == 0) J.boxes[blockIdx.x] = make_int4(0, 0, 0, 0);
+        return;
+    }
     const float2* pos = pos_p.get(true);
     Span sx[MAP_PER_LANE], sy[MAP_PER_LANE];
     float r[MAP_PER_LANE];
     int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1;
+    // lane l of wave w takes nodes base + 16 l + 4 w + q: the lanes of one instruction are 16 nodes apart along the
+    // curve, so their LDS atomics rarely hit the same bin (same-address lanes serialise)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int first = base + lane * (MAP_NODES / 64) + wv * MAP_PER_LANE;
 #pragma unroll
     for (int q = 0; q < MAP_PER_LANE; q++) {
-        const int i = base + q * EB + threadIdx.x;
+        const int i = first + q;
         r[q] = 0.f;
         sx[q].b0 = 1;
         sx[q].b1 = 0;                                             // empty unless a node is here
@@ -139,7 +151,11 @@ __global__ __launch_bounds__(EB) void eng_density_map(Pick pos_p, MapJob f, MapJ
     const bool one_plane = (base < J.num_mov) == (last < J.num_mov);
     const long long plane = base < J.num_mov ? 0 : (long long)J.nbx * J.nby;
     const int w = x1 - x0 + 1, h = y1 - y0 + 1;
-    if (x1 < x0) return;                                           // no bins at all (uniform)
+    const int gb = (int)blockIdx.x;                                // slot / box of this workgroup
+    if (x1 < x0) {                                                 // no bins at all (uniform)
+        if (threadIdx.x == 0) J.boxes[gb] = make_int4(0, 0, 0, 0);
+        return;
+    }
     if (one_plane && (long long)w * h <= MAP_LDS) {
         for (int e = threadIdx.x; e < w * h; e += EB) tile[e] = 0ull;
         __syncthreads();
@@ -154,19 +170,19 @@ __global__ __launch_bounds__(EB) void eng_density_map(Pick pos_p, MapJob f, MapJ
                 }
             }
         __syncthreads();
-        unsigned long long* m = J.map + plane;
-        for (int e = threadIdx.x; e < w * h; e += EB) {
-            const unsigned long long v = tile[e];
-            if (v) {
-                const int bx = x0 + e / h, by = y0 + e % h;
-                atomicAdd(m + (long long)bx * J.nby + by, v);
-            }
-        }
+        // the tile goes to this workgroup's own slot with plain stores; the merge kernel sums the slots per bin
+        unsigned long long* slot = J.slots + (long long)gb * MAP_LDS;
+        for (int e = threadIdx.x; e < w * h; e += EB) slot[e] = tile[e];
+        if (threadIdx.x == 0) J.boxes[gb] = make_int4(x0, y0, w, h | (plane ? (1 << 20) : 0));
         return;
+    }
+    if (threadIdx.x == 0) {
+        J.boxes[gb] = make_int4(0, 0, 0, 0);
+        *J.fallback = 1;
     }
 #pragma unroll
     for (int q = 0; q < MAP_PER_LANE; q++) {
-        const int i = base + q * EB + threadIdx.x;
+        const int i = first + q;
         unsigned long long* m = J.map + (i < J.num_mov ? 0 : (long long)J.nbx * J.nby);
         for (int bx = sx[q].b0; bx <= sx[q].b1; bx++) {
             const float wx = r[q] * span_overlap(sx[q], bx);
@@ -180,33 +196,101 @@ __global__ __launch_bounds__(EB) void eng_density_map(Pick pos_p, MapJob f, MapJ
 }
 
 // maps -> unmirrored total density (force grid) and the overflow partial sums; the maps are cleared for the next pass
+// Merge + post, one workgroup per 32 x 32 tile of bins: the tile's value is the sum of the slots of the map
+// workgroups whose box covers it (found by scanning the boxes in workgroup order: a fixed order, and integer sums
+// anyway) plus the fallback map when any workgroup used it; then density, overflow, and the map cleared.
+constexpr int MT = 32;
 struct PostJob {
     unsigned long long* map;
-    int planes;          // 2: cells + fillers (force grid), 1: cells only
+    int planes;                // 2: cells + fillers (force grid), 1: cells only
     const float* fixed;
-    float* rho;          // null: overflow only
-    long long nb;
-    int blocks;
-    int count_over;      // 1: this grid's overflow feeds the control
+    float* rho;                // null: overflow only
+    int nbx, nby;
+    int tiles_y;               // tiles per column of tiles
+    int blocks;                // tiles of this grid
+    int count_over;            // 1: this grid's overflow feeds the control
+    const unsigned long long* slots;
+    const int4* boxes;
+    int first_box, num_boxes;  // the map workgroups of this grid
+    const int* fallback;
 };
 
-__global__ __launch_bounds__(EB) void eng_density_post(PostJob f, PostJob j, float target, double* part) {
+__global__ __launch_bounds__(EB) void eng_density_merge(PostJob f, PostJob j, float target, double* part) {
     __shared__ double red[16];
+    __shared__ int list[EB + 1];
+    __shared__ int nlist;
+    __shared__ int wcount[EB / 64];
     const bool judge = (int)blockIdx.x >= f.blocks;
     const PostJob& J = judge ? j : f;
-    const long long b = (long long)((int)blockIdx.x - (judge ? f.blocks : 0)) * EB + threadIdx.x;
-    double o[1] = {0.0};
-    if (b < J.nb) {
-        const unsigned long long m0 = J.map[b];
-        const float mov = (float)((double)m0 / kDensityScale) + J.fixed[b];
-        J.map[b] = 0;
-        float r = mov;
-        if (J.planes == 2) {
-            r = mov + (float)((double)J.map[J.nb + b] / kDensityScale);
-            J.map[J.nb + b] = 0;
+    const int tile = (int)blockIdx.x - (judge ? f.blocks : 0);
+    const int X0 = (tile / J.tiles_y) * MT, Y0 = (tile % J.tiles_y) * MT;
+    const long long nb = (long long)J.nbx * J.nby;
+    const bool fb = *J.fallback != 0;
+    // each lane owns 4 bins of the tile: local e = lane + 256 k -> (e / 32, e % 32)
+    unsigned long long acc0[4], acc1[4];
+    long long bin[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int e = threadIdx.x + EB * k, bx = X0 + e / MT, by = Y0 + e % MT;
+        bin[k] = (bx < J.nbx && by < J.nby) ? (long long)bx * J.nby + by : -1;
+        acc0[k] = acc1[k] = 0ull;
+        if (fb && bin[k] >= 0) {
+            acc0[k] = J.map[bin[k]];
+            J.map[bin[k]] = 0ull;
+            if (J.planes == 2) {
+                acc1[k] = J.map[nb + bin[k]];
+                J.map[nb + bin[k]] = 0ull;
+            }
         }
-        if (J.rho) J.rho[b] = r;
-        if (J.count_over) o[0] = (double)fmaxf(mov - target, 0.f);
+    }
+    // boxes overlapping the tile, in workgroup order, EB at a time
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < J.num_boxes; c0 += EB) {
+        const int wgi = c0 + threadIdx.x;
+        bool hit = false;
+        int4 bx4 = make_int4(0, 0, 0, 0);
+        if (wgi < J.num_boxes) {
+            bx4 = J.boxes[J.first_box + wgi];
+            const int h = bx4.w & 0xFFFFF;
+            hit = bx4.z > 0 && bx4.x < X0 + MT && bx4.x + bx4.z > X0 && bx4.y < Y0 + MT && bx4.y + h > Y0;
+        }
+        const unsigned long long m = __ballot(hit);
+        if (lane == 0) wcount[wave] = __popcll(m);
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int w = 0; w < EB / 64; w++) {
+            if (w < wave) before += wcount[w];
+            total += wcount[w];
+        }
+        if (hit) list[before + __popcll(m & ((1ull << lane) - 1))] = J.first_box + wgi;
+        if (threadIdx.x == 0) nlist = total;
+        __syncthreads();
+        for (int q = 0; q < nlist; q++) {
+            const int g = list[q];
+            const int4 b4 = J.boxes[g];
+            const int h = b4.w & 0xFFFFF, plane = b4.w >> 20;
+            const unsigned long long* slot = J.slots + (long long)g * MAP_LDS;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (bin[k] < 0) continue;
+                const int e = threadIdx.x + EB * k, bx = X0 + e / MT, by = Y0 + e % MT;
+                if (bx >= b4.x && bx < b4.x + b4.z && by >= b4.y && by < b4.y + h) {
+                    const unsigned long long v = slot[(bx - b4.x) * h + (by - b4.y)];
+                    if (plane) acc1[k] += v;
+                    else acc0[k] += v;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double o[1] = {0.0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (bin[k] < 0) continue;
+        const float mov = (float)((double)acc0[k] / kDensityScale) + J.fixed[bin[k]];
+        const float r = J.planes == 2 ? mov + (float)((double)acc1[k] / kDensityScale) : mov;
+        if (J.rho) J.rho[bin[k]] = r;
+        if (J.count_over) o[0] += (double)fmaxf(mov - target, 0.f);
     }
     block_sum<1>(o, red);
     if (threadIdx.x == 0) part[blockIdx.x] = o[0];
@@ -265,8 +349,9 @@ __global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part) {
         }
         if (i < a.num_mov) {
             float wx = 0, wy = 0;
+            // the WA kernel wrote the pin gradients in cell order: a cell's pins are contiguous
             for (int k = a.cell_start[i], e = a.cell_start[i + 1]; k < e; k++) {
-                const float2 q = a.pin_grad[a.cell_pin[k]];
+                const float2 q = a.pin_grad[k];
                 wx += q.x;
                 wy += q.y;
             }
@@ -338,8 +423,9 @@ __global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) {
 }
 
 __global__ __launch_bounds__(64) void eng_control(const double* red5, double over_scale, double* S, float* kp,
-                                                  const double* c, int* flags, double* stats) {
+                                                  const double* c, int* flags, double* stats, int* fallback) {
     if (threadIdx.x == 0) {
+        *fallback = 0;             // the merge of this pass has read it
         double r[5] = {0, 0, 0, 0, 0};
         for (int b = 0; b < RED_BLOCKS; b++)
             for (int k = 0; k < 5; k++) r[k] += red5[b * 5 + k];
@@ -353,6 +439,20 @@ __global__ __launch_bounds__(64) void eng_control(const double* red5, double ove
             if (S[ACCEPT] != 0) flags[0] = 1 - flags[0];
             flags[1] = S[IS_BEST] != 0 ? 1 : 0;
         }
+    }
+}
+
+// level set-up: pin -> its position in the cell CSR, so the WA kernel writes pin gradients in cell order
+__global__ __launch_bounds__(EB) void eng_pin_slot_clear(int* slot, int num_pins) {
+    const int p = blockIdx.x * EB + threadIdx.x;
+    if (p < num_pins) slot[p] = -1;
+}
+__global__ __launch_bounds__(EB) void eng_pin_slot_fill(int* slot, const int* cell_start, const int* cell_pin,
+                                                        int num_mov, int num_pins) {
+    const int k = blockIdx.x * EB + threadIdx.x;
+    if (k < cell_start[num_mov]) {
+        const int p = cell_pin[k];
+        if (p >= 0 && p < num_pins) slot[p] = k;
     }
 }
 
@@ -381,8 +481,14 @@ __global__ void eng_finalize_flags(int* flags) {
 struct Level {
     ap_level_desc d{};
     Poisson poisson;
+    WaPlan wa_plan;
     float2 *u1 = nullptr, *v1 = nullptr, *g1 = nullptr, *pin_grad = nullptr;
+    int* block_net = nullptr;  // WA workgroup -> first owned net
+    int* pin_slot = nullptr;   // pin -> position in the cell CSR (-1: fixed pin)
     unsigned long long *map = nullptr, *jmap = nullptr;
+    unsigned long long* slots = nullptr;   // per map workgroup: its LDS tile
+    int4* boxes = nullptr;                 // per map workgroup: the tile's box
+    int* fallback = nullptr;               // a map workgroup used the global atomics in this pass
     float *rho = nullptr, *fx = nullptr, *fy = nullptr;
     double* part = nullptr;
     double* stats = nullptr;
@@ -439,6 +545,15 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipMalloc(&v1, N * sizeof(float2)));
     AP_HIP(hipMalloc(&g1, N * sizeof(float2)));
     AP_HIP(hipMalloc(&pin_grad, (d.num_pins + 1) * sizeof(float2)));
+    AP_HIP(hipMalloc(&pin_slot, (d.num_pins + 1) * sizeof(int)));
+    if (d.num_pins > 0) {
+        hipLaunchKernelGGL(eng_pin_slot_clear, dim3(grid_of(d.num_pins, EB)), dim3(EB), 0, nullptr, pin_slot,
+                           d.num_pins);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_pin_slot_fill, dim3(grid_of(d.num_pins, EB)), dim3(EB), 0, nullptr, pin_slot,
+                           d.cell_start, d.cell_pin, d.num_mov, d.num_pins);
+        AP_LAUNCHED();
+    }
     AP_HIP(hipMalloc(&map, 2 * nb * sizeof(unsigned long long)));
     AP_HIP(hipMemset(map, 0, 2 * nb * sizeof(unsigned long long)));
     if (nj) {
@@ -449,11 +564,18 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipMalloc(&fx, nb * sizeof(float)));
     AP_HIP(hipMalloc(&fy, nb * sizeof(float)));
     nblk_n = grid_of(N, EB);
-    nblk_wa = grid_of((long long)d.num_pins + 1, WA_SPAN);
+    if ((st = wa_plan.create(d.net_start, d.num_nets))) return st;
+    AP_HIP(hipFuncSetAttribute((const void*)eng_density_map, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               MAP_LDS * sizeof(unsigned long long)));
+    nblk_wa = wa_plan.partial_count();
     nblk_fmap = grid_of(N, MAP_NODES);
     nblk_jmap = nj ? grid_of(d.num_mov, MAP_NODES) : 0;
-    nblk_fpost = grid_of(nb, EB);
-    nblk_jpost = nj ? grid_of(nj, EB) : 0;
+    nblk_fpost = grid_of(d.nbx, MT) * grid_of(d.nby, MT);
+    nblk_jpost = nj ? grid_of(d.njx, MT) * grid_of(d.njy, MT) : 0;
+    AP_HIP(hipMalloc(&slots, (long long)(nblk_fmap + nblk_jmap) * MAP_LDS * sizeof(unsigned long long)));
+    AP_HIP(hipMalloc(&boxes, (nblk_fmap + nblk_jmap + 1) * sizeof(int4)));
+    AP_HIP(hipMalloc(&fallback, sizeof(int)));
+    AP_HIP(hipMemset(fallback, 0, sizeof(int)));
     const long long np = nblk_n + 2LL * nblk_wa + nblk_fpost + nblk_jpost + nblk_n;
     AP_HIP(hipMalloc(&part, np * sizeof(double)));
     AP_HIP(hipMemset(part, 0, np * sizeof(double)));
@@ -479,10 +601,12 @@ void Level::destroy() {
     if (graph) (void)hipGraphDestroy(graph);
     for (hipEvent_t& ev : pev)
         if (ev) (void)hipEventDestroy(ev);
-    void* ptrs[] = {u1, v1, g1, pin_grad, map, jmap, rho, fx, fy, part, stats, flags, red5};
+    void* ptrs[] = {u1,    v1,    g1,       pin_grad, map,      jmap,  rho,   fx, fy, part, stats, flags, red5,
+                    block_net, pin_slot, slots, boxes, fallback};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     poisson.destroy();
+    wa_plan.destroy();
     if (stream) (void)hipStreamDestroy(stream);
     hipEvent_t evs[] = {ev_in, ev_out, t0, t1};
     for (hipEvent_t e : evs)
@@ -498,31 +622,32 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     {
         // the evaluation point: set 1 when the current point is set 0, and the reverse (resolved on the device)
         WaArgs w{pos.a, d.pin_cell, (const float2*)d.pin_off, d.net_start, d.num_nets, d.num_mov, kp_ptr,
-                 pin_grad, nullptr, p_wa, pos.b, pos.flag};
+                 pin_grad, nullptr, p_wa, pos.b, pos.flag, block_net, pin_slot};
         mark(AP_PROF_WA, s);
-        if ((st = launch_wa(w, d.num_pins, s))) return st;
+        if ((st = wa_plan.launch(w, p_wa, s))) return st;
     }
     int n_over = 0;
     double over_scale = 0;
     if (use_density) {
-        MapJob f{(const float2*)d.size, d.ratio, d.num_nodes, d.num_mov, (float)(1.0 / d.bx), (float)(1.0 / d.by),
-                 d.nbx, d.nby, map, nblk_fmap};
         // the same float32 reciprocal as the reference's host code: 1.f / (float)bx
-        f.inv_bx = 1.f / (float)d.bx;
-        f.inv_by = 1.f / (float)d.by;
+        MapJob f{(const float2*)d.size, d.ratio, d.num_nodes, d.num_mov, 1.f / (float)d.bx, 1.f / (float)d.by,
+                 d.nbx, d.nby, map, nblk_fmap, 0, slots, boxes, fallback};
         MapJob j{(const float2*)d.judge_size, d.judge_ratio, d.num_mov, d.num_mov, 0.f, 0.f, d.njx, d.njy, jmap,
-                 nblk_jmap};
+                 nblk_jmap, nblk_fmap, slots, boxes, fallback};
         if (d.has_judge) {
             j.inv_bx = 1.f / (float)d.jbx;
             j.inv_by = 1.f / (float)d.jby;
         }
         mark(AP_PROF_MAP, s);
-        hipLaunchKernelGGL(eng_density_map, dim3(nblk_fmap + nblk_jmap), dim3(EB), 0, s, pos, f, j);
+        hipLaunchKernelGGL(eng_density_map, dim3(nblk_fmap + nblk_jmap), dim3(EB),
+                           MAP_LDS * sizeof(unsigned long long), s, pos, f, j);
         AP_LAUNCHED();
-        PostJob pf{map, 2, d.fixed, rho, (long long)d.nbx * d.nby, nblk_fpost, d.has_judge ? 0 : 1};
-        PostJob pj{jmap, 1, d.judge_fixed, nullptr, d.has_judge ? (long long)d.njx * d.njy : 0, nblk_jpost, 1};
+        PostJob pf{map, 2, d.fixed, rho, d.nbx, d.nby, (int)grid_of(d.nby, MT), nblk_fpost, d.has_judge ? 0 : 1,
+                   slots, boxes, 0, nblk_fmap, fallback};
+        PostJob pj{jmap, 1, d.judge_fixed, nullptr, d.njx, d.njy, d.has_judge ? (int)grid_of(d.njy, MT) : 1,
+                   nblk_jpost, 1, slots, boxes, nblk_fmap, nblk_jmap, fallback};
         mark(AP_PROF_POST, s);
-        hipLaunchKernelGGL(eng_density_post, dim3(nblk_fpost + nblk_jpost), dim3(EB), 0, s, pf, pj, d.target,
+        hipLaunchKernelGGL(eng_density_merge, dim3(nblk_fpost + nblk_jpost), dim3(EB), 0, s, pf, pj, d.target,
                            p_over);
         AP_LAUNCHED();
         n_over = nblk_fpost + nblk_jpost;
@@ -563,7 +688,8 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
             over_scale};
     hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(EB), 0, s, sg, red5);
     AP_LAUNCHED();
-    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red5, over_scale, S, kp_out, d.c, flags, stats);
+    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red5, over_scale, S, kp_out, d.c, flags, stats,
+                       fallback);
     AP_LAUNCHED();
     mark(AP_PROF_CLASSES, s);
     return AP_OK;

@@ -20,7 +20,12 @@ struct WaArgs {
     double* partial;    // per-workgroup (WA, HPWL) in FP64, may be null
     const float2* pos_other;   // with flag: the position is pos_other when *flag == 0, else pos
     const int* flag;           // null: pos as given
+    const int* block_net;      // optional [workgroups + 1]: first net owned by each workgroup (else searched)
+    const int* pin_slot;       // optional [pins]: write pin p's gradient to pin_grad[pin_slot[p]] (< 0: skip)
 };
+
+// block_net[b] = first net whose first pin is at or after b * WA_SPAN, b in [0, blocks]
+int launch_wa_block_table(const int* net_start, int num_nets, int blocks, int* block_net, hipStream_t s);
 
 struct DgradArgs {
     const float2* pos;

@@ -15,6 +15,7 @@
 #include "gp_common.h"
 #include "gp_control.h"
 #include "gp_kernels.h"
+#include "gp_wa_common.h"
 
 namespace ap {
 
@@ -28,89 +29,6 @@ void set_error(const char* fmt, ...) {
 }
 
 // ------------------------------------------------------------------------------------------- WA wirelength
-
-// smallest i in [lo, hi) with a[i] >= key (hi if none)
-__device__ __forceinline__ int lower_bound(const int* a, int lo, int hi, int key) {
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (a[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ float2 pin_xy(const float2* pos, const int* pin_cell, const float2* pin_off, int p,
-                                         int num_mov) {
-    int c = pin_cell[p];
-    float2 o = pin_off[p];
-    if (c < num_mov) {
-        float2 q = pos[c];
-        o.x += q.x;
-        o.y += q.y;
-    }
-    return o;
-}
-
-// One net whose pins do not fit the LDS stage: the whole workgroup loops over its pins (three passes over global
-// memory), with fixed-order block reductions. Rare (degree > WA_CAP).
-__device__ void wa_big_net(const WaArgs& a, int net, double& wa_acc, double& hpwl_acc, float* red /* [4*256] */) {
-    const int p0 = a.net_start[net], p1 = a.net_start[net + 1], t = threadIdx.x, nt = blockDim.x;
-    const float ig = 1.0f / *a.gamma;
-    float hx = -1e30f, hy = -1e30f, lx = 1e30f, ly = 1e30f;
-    for (int p = p0 + t; p < p1; p += nt) {
-        float2 x = pin_xy(a.pos, a.pin_cell, a.pin_off, p, a.num_mov);
-        hx = fmaxf(hx, x.x); hy = fmaxf(hy, x.y); lx = fminf(lx, x.x); ly = fminf(ly, x.y);
-    }
-    red[t] = hx; red[nt + t] = hy; red[2 * nt + t] = lx; red[3 * nt + t] = ly;
-    __syncthreads();
-    for (int s = nt / 2; s > 0; s >>= 1) {
-        if (t < s) {
-            red[t] = fmaxf(red[t], red[t + s]);
-            red[nt + t] = fmaxf(red[nt + t], red[nt + t + s]);
-            red[2 * nt + t] = fminf(red[2 * nt + t], red[2 * nt + t + s]);
-            red[3 * nt + t] = fminf(red[3 * nt + t], red[3 * nt + t + s]);
-        }
-        __syncthreads();
-    }
-    hx = red[0]; hy = red[nt]; lx = red[2 * nt]; ly = red[3 * nt];
-    __syncthreads();
-    float sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // sp.x sp.y sn.x sn.y (x ep).x (x ep).y (x en).x (x en).y
-    for (int p = p0 + t; p < p1; p += nt) {
-        float2 x = pin_xy(a.pos, a.pin_cell, a.pin_off, p, a.num_mov);
-        float epx = expf((x.x - hx) * ig), epy = expf((x.y - hy) * ig);
-        float enx = expf((lx - x.x) * ig), eny = expf((ly - x.y) * ig);
-        sums[0] += epx; sums[1] += epy; sums[2] += enx; sums[3] += eny;
-        sums[4] += x.x * epx; sums[5] += x.y * epy; sums[6] += x.x * enx; sums[7] += x.y * eny;
-    }
-    float tot[8];
-    for (int k = 0; k < 8; k += 4) {
-        for (int j = 0; j < 4; j++) red[j * nt + t] = sums[k + j];
-        __syncthreads();
-        for (int s = nt / 2; s > 0; s >>= 1) {
-            if (t < s)
-                for (int j = 0; j < 4; j++) red[j * nt + t] += red[j * nt + t + s];
-            __syncthreads();
-        }
-        for (int j = 0; j < 4; j++) tot[k + j] = red[j * nt];
-        __syncthreads();
-    }
-    const float xpx = tot[4] / tot[0], xpy = tot[5] / tot[1], xnx = tot[6] / tot[2], xny = tot[7] / tot[3];
-    for (int p = p0 + t; p < p1; p += nt) {
-        float2 x = pin_xy(a.pos, a.pin_cell, a.pin_off, p, a.num_mov);
-        float epx = expf((x.x - hx) * ig), epy = expf((x.y - hy) * ig);
-        float enx = expf((lx - x.x) * ig), eny = expf((ly - x.y) * ig);
-        float2 gr;
-        gr.x = epx * (1 + (x.x - xpx) * ig) / tot[0] - enx * (1 - (x.x - xnx) * ig) / tot[2];
-        gr.y = epy * (1 + (x.y - xpy) * ig) / tot[1] - eny * (1 - (x.y - xny) * ig) / tot[3];
-        a.pin_grad[p] = gr;
-    }
-    if (t == 0) {
-        float wa = (xpx - xnx) + (xpy - xny), hp = (hx - lx) + (hy - ly);
-        if (a.out) a.out[net] = make_float2(wa, hp);
-        wa_acc += wa;
-        hpwl_acc += hp;
-    }
-}
 
 __global__ __launch_bounds__(WA_BLOCK) void wa_net_kernel(WaArgs a) {
     if (a.flag && *a.flag == 0) a.pos = a.pos_other;
@@ -128,7 +46,7 @@ __global__ __launch_bounds__(WA_BLOCK) void wa_net_kernel(WaArgs a) {
     if (t < 2) {
         long long key = win0 + (t ? WA_SPAN : 0);
         int k = key > 0x7fffffff ? 0x7fffffff : (int)key;
-        sbound[t] = lower_bound(a.net_start, 0, a.num_nets, k);
+        sbound[t] = a.block_net ? a.block_net[blockIdx.x + t] : lower_bound(a.net_start, 0, a.num_nets, k);
     }
     __syncthreads();
     const int net_lo = sbound[0], net_hi = sbound[1];
@@ -154,7 +72,7 @@ __global__ __launch_bounds__(WA_BLOCK) void wa_net_kernel(WaArgs a) {
         __syncthreads();
         const int net_stop = sbound[0];
         if (net_stop == net_cur) {            // a single net larger than the stage
-            wa_big_net(a, net_cur, wa_acc, hpwl_acc, (float*)sx);
+            wa_big_net_path(a, net_cur, wa_acc, hpwl_acc, (float*)sx);
             __syncthreads();
             net_cur++;
             continue;
@@ -162,7 +80,11 @@ __global__ __launch_bounds__(WA_BLOCK) void wa_net_kernel(WaArgs a) {
         const int npin = a.net_start[net_stop] - base, nnet = net_stop - net_cur;
         for (int q = t; q < npin; q += WA_BLOCK) sx[q] = pin_xy(a.pos, a.pin_cell, a.pin_off, base + q, a.num_mov);
         __syncthreads();
-        for (int j = t; j < nnet; j += WA_BLOCK) {
+        // nets go round-robin over the four wavefronts (net 4k + w to wavefront w), so a long net holds up only the
+        // quarter of the nets that share its wavefront
+        for (int j0 = 0; j0 < nnet; j0 += WA_BLOCK) {
+            const int j = j0 + (t & 63) * (WA_BLOCK / 64) + (t >> 6);
+            if (j >= nnet) continue;
             const int s = a.net_start[net_cur + j] - base, e = a.net_start[net_cur + j + 1] - base;
             float hx = -1e30f, hy = -1e30f, lx = 1e30f, ly = 1e30f;
             for (int q = s; q < e; q++) {
@@ -200,7 +122,8 @@ __global__ __launch_bounds__(WA_BLOCK) void wa_net_kernel(WaArgs a) {
             float2 gr;
             gr.x = ep.x * (1 + (x.x - m.x) * ig) / acc.x - en.x * (1 - (x.x - m.z) * ig) / acc.z;
             gr.y = ep.y * (1 + (x.y - m.y) * ig) / acc.y - en.y * (1 - (x.y - m.w) * ig) / acc.w;
-            a.pin_grad[base + q] = gr;
+            const int slot = a.pin_slot ? a.pin_slot[base + q] : base + q;
+            if (slot >= 0) a.pin_grad[slot] = gr;
         }
         net_cur = net_stop;
     }
@@ -226,6 +149,21 @@ __global__ __launch_bounds__(256) void wa_cell_kernel(const float2* pin_grad, co
         gy += g.y;
     }
     grad[c] = make_float2(gx, gy);
+}
+
+__global__ __launch_bounds__(256) void wa_block_table_kernel(const int* net_start, int num_nets, int blocks,
+                                                             int* block_net) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > blocks) return;
+    const long long key = (long long)b * WA_SPAN;
+    block_net[b] = lower_bound(net_start, 0, num_nets, key > 0x7fffffff ? 0x7fffffff : (int)key);
+}
+
+int launch_wa_block_table(const int* net_start, int num_nets, int blocks, int* block_net, hipStream_t s) {
+    hipLaunchKernelGGL(wa_block_table_kernel, dim3(grid_of((long long)blocks + 1, 256)), dim3(256), 0, s, net_start,
+                       num_nets, blocks, block_net);
+    AP_LAUNCHED();
+    return AP_OK;
 }
 
 int launch_wa(const WaArgs& a, int num_pins, hipStream_t s) {
@@ -386,7 +324,7 @@ int ap_wa_wirelength(const float* pos, const int* pin_cell, const float* pin_off
     AP_REQUIRE(num_nets == 0 || (net_start && gamma && pin_grad), "wa: null pointer");
     hipStream_t s = (hipStream_t)stream;
     WaArgs a{(const float2*)pos, pin_cell, (const float2*)pin_off, net_start, num_nets, num_mov, gamma,
-             (float2*)pin_grad, (float2*)out, nullptr, nullptr, nullptr};
+             (float2*)pin_grad, (float2*)out, nullptr, nullptr, nullptr, nullptr, nullptr};
     int st = launch_wa(a, num_pins, s);
     if (st) return st;
     if (num_mov > 0) {

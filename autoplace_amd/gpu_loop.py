@@ -12,7 +12,7 @@ import math
 import torch
 
 from . import _lib
-from .locality import morton_order, renumber_netlist
+from .locality import hilbert_order, renumber_netlist
 
 FIELDS = ("lam gam alpha a iter hpwl_prev motion ww mu settled best_hpwl best_ovf is_best improved stop accept "
           "loop_it evals res_hpwl res_ovf has_best").split()
@@ -45,7 +45,7 @@ class Engine(object):
         self.user = (u, v, g, best)
         n, N = d.num_mov, d.num_nodes
         if reorder:
-            perm = morton_order(v, d.nbx, d.nby, float(d.bin_np[0]), float(d.bin_np[1]), n)
+            perm = hilbert_order(v, d.nbx, d.nby, float(d.bin_np[0]), float(d.bin_np[1]), n)
             net = renumber_netlist(w["cell32"], w["off"], w["net_start"], n, perm)
         else:
             perm = None
@@ -144,9 +144,10 @@ def control_state(ps, optimizer, iteration, args):
 
 
 def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every=16, use_graph=True,
-                     poisson="auto", reorder=True):
+                     poisson="auto", reorder=True, hook=None):
     """Continue the level from `iteration` (density weight on) until it stops; updates optimizer, ps and
-    objective.evals as the host loop would. -> (reason, iterations, hpwl, overflow) of the kept placement."""
+    objective.evals as the host loop would. -> (reason, iterations, hpwl, overflow) of the kept placement.
+    hook(objective, u, v, g, S, kp, c), if given, sees the state the device loop starts from (instrumentation)."""
     device, pos = optimizer.v.device, optimizer.v
     S, kp, c = control_state(ps, optimizer, iteration, args)
     S = torch.tensor(S, dtype=torch.float64, device=device)
@@ -154,6 +155,8 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
     c = torch.tensor(c, dtype=torch.float64, device=device)
     u = optimizer.u.contiguous()
     g = optimizer.g.contiguous()
+    if hook is not None:
+        hook(objective, u, pos, g, S, kp, c)
     best = pos.clone()
     eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
     evals_before, logged = objective.evals, iteration // args.log_freq

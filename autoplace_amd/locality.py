@@ -32,6 +32,37 @@ def morton_order(pos, nbx, nby, bx, by, num_mov):
     return torch.cat([cells, fill])
 
 
+def hilbert_key(ix, iy, order):
+    """Index of the cells (ix, iy) along the Hilbert curve of a 2^order x 2^order grid (the classic xy -> d walk,
+    vectorised). Consecutive indices are edge neighbours, so any run of indices covers a compact region."""
+    x, y = ix.long().clone(), iy.long().clone()
+    d = torch.zeros_like(x)
+    s = 1 << (order - 1)
+    while s > 0:
+        rx = ((x & s) > 0).long()
+        ry = ((y & s) > 0).long()
+        d += s * s * ((3 * rx) ^ ry)
+        # rotate the quadrant
+        flip = ry == 0
+        swap_x = torch.where(flip & (rx == 1), s - 1 - x, x)
+        swap_y = torch.where(flip & (rx == 1), s - 1 - y, y)
+        x = torch.where(flip, swap_y, swap_x)
+        y = torch.where(flip, swap_x, swap_y)
+        s >>= 1
+    return d
+
+
+def hilbert_order(pos, nbx, nby, bx, by, num_mov):
+    """Permutation new -> old of the nodes: cells, then fillers, each sorted along the Hilbert curve of their bin."""
+    ix = (pos[:, 0] / bx).floor().clamp_(0, nbx - 1)
+    iy = (pos[:, 1] / by).floor().clamp_(0, nby - 1)
+    order = max(1, int(max(nbx, nby) - 1).bit_length())
+    key = hilbert_key(ix, iy, order)
+    cells = torch.argsort(key[:num_mov], stable=True)
+    fill = torch.argsort(key[num_mov:], stable=True) + num_mov
+    return torch.cat([cells, fill])
+
+
 def renumber_netlist(cell32, off, net_start, num_mov, perm):
     """Pins grouped by net (optimized-net CSR) -> the same netlist with cells renumbered by `perm` (new -> old) and
     nets ordered by their smallest new cell index. -> dict(cell32, off, net_start, cell_start, cell_pin)."""

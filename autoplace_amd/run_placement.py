@@ -44,7 +44,7 @@ def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None):
         if ps.density_weight > 0 and gpu_loop.usable(optimizer):
             reason, iters, hpwl, overflow = gpu_loop.run_level_on_gpu(
                 optimizer, objective, ps, args, log, iteration, use_graph=opts.get("graph", True),
-                poisson=opts.get("poisson", "auto"))
+                poisson=opts.get("poisson", "auto"), reorder=opts.get("reorder", True), hook=opts.get("hook"))
             return SimpleNamespace(reason=reason, iters=iters, start_iters=start_iters, hpwl=hpwl, overflow=overflow)
         optimizer.step()
         if optimizer.diverged:
