@@ -32,6 +32,7 @@ _SIGNATURES = {
     "ap_poisson_destroy": ([P], I),
     "ap_poisson_method": ([P], I),
     "ap_poisson_solve": ([P, P, I, P, P, I, P], I),
+    "ap_poisson_energy": ([P, P, I, P, P], I),
     "ap_level_create": ([P, ctypes.POINTER(P)], I),
     "ap_level_destroy": ([P], I),
     "ap_level_run": ([P, I, I, P], I),

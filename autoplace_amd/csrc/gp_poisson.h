@@ -26,6 +26,7 @@ struct Poisson {
     int create(int nbx, int nby, double bx, double by, int method);
     void destroy();
     int solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s);
+    int energy(const float* rho, int rho_ld, double* out, hipStream_t s);
 
     int fft_create();
     void fft_destroy();

@@ -225,6 +225,58 @@ void Poisson::destroy() {
     fft_destroy();
 }
 
+// E = 1/2 sum_kl w_phi[k][l] C[k][l]^2 with w_phi = 4/(NM) h_k h_l / ((w_k^2 + v_l^2) bx), h_0 = 1/2: the
+// electrostatic energy 1/2 sum rho phi of the map, in the units in which its position gradient is the field gather.
+__global__ __launch_bounds__(256) void energy_partial_kernel(const float* C, int N, int M, double bx, double by,
+                                                             double* part) {
+    __shared__ double red[16];
+    double v[1] = {0.0};
+    const long long total = (long long)N * M;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        const int k = (int)(e / M), l = (int)(e - (long long)k * M);
+        if (k == 0 && l == 0) continue;
+        const double w = M_PI * k / (N * bx), nu = M_PI * l / (M * by);
+        const double wphi = 4.0 / ((double)N * M) * (k ? 1.0 : 0.5) * (l ? 1.0 : 0.5) / ((w * w + nu * nu) * bx);
+        const double c = C[e];
+        v[0] += 0.5 * wphi * c * c;
+    }
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+__global__ __launch_bounds__(256) void energy_final_kernel(const double* part, int n, double* out) {
+    __shared__ double red[16];
+    double v[1] = {0.0};
+    for (int k = threadIdx.x; k < n; k += 256) v[0] += part[k];
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) *out = v[0];
+}
+
+int Poisson::energy(const float* rho, int rho_ld, double* out, hipStream_t s) {
+    if (method != AP_POISSON_GEMM) {
+        set_error("poisson_energy: needs a plan of the GEMM method");
+        return AP_ERR_UNSUPPORTED;
+    }
+    const size_t nm = (size_t)N * M;
+    float *T1 = d_work, *C = d_work + nm;
+    hipLaunchKernelGGL(map_sum_kernel, dim3(MEAN_BLOCKS), dim3(256), 0, s, rho, rho_ld, N, M, d_part);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(map_mean_kernel, dim3(1), dim3(256), 0, s, d_part, MEAN_BLOCKS, 1.0 / ((double)N * M), d_mean);
+    AP_LAUNCHED();
+    GemmBatch b{};
+    b.M = N; b.N = M; b.K = M; b.epilogue = 0;
+    b.p[0] = GemmProblem{rho, d_cyt, T1, nullptr, nullptr, nullptr, rho_ld, M, M, d_mean};
+    int st;
+    if ((st = launch_gemm(b, 1, s))) return st;
+    b.M = N; b.N = M; b.K = N;
+    b.p[0] = GemmProblem{d_cx, T1, C, nullptr, nullptr, nullptr, N, M, M};
+    if ((st = launch_gemm(b, 1, s))) return st;
+    hipLaunchKernelGGL(energy_partial_kernel, dim3(MEAN_BLOCKS), dim3(256), 0, s, C, N, M, bx, by, d_part);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(energy_final_kernel, dim3(1), dim3(256), 0, s, d_part, MEAN_BLOCKS, out);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
 int Poisson::solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s) {
     if (method == AP_POISSON_FFT) return fft_solve(rho, rho_ld, fx, fy, fld, s);
     const size_t nm = (size_t)N * M;
@@ -289,6 +341,12 @@ int ap_poisson_destroy(ap_poisson* plan) {
 }
 
 int ap_poisson_method(const ap_poisson* plan) { return plan ? plan->p.method : -1; }
+
+int ap_poisson_energy(ap_poisson* plan, const float* rho, int rho_ld, double* energy, void* stream) {
+    AP_REQUIRE(plan && rho && energy, "poisson_energy: null argument");
+    AP_REQUIRE(rho_ld >= plan->p.M, "poisson_energy: leading dimension below %d", plan->p.M);
+    return plan->p.energy(rho, rho_ld, energy, (hipStream_t)stream);
+}
 
 int ap_poisson_solve(ap_poisson* plan, const float* rho, int rho_ld, float* field_x, float* field_y, int field_ld,
                      void* stream) {

@@ -117,6 +117,13 @@ class PoissonPlan(object):
                                                ctypes.c_void_p(field_x.data_ptr()), ctypes.c_void_p(field_y.data_ptr()),
                                                field_ld, _stream()), "poisson_solve")
 
+    def energy(self, rho, rho_ld=None):
+        """Electrostatic energy 1/2 sum rho phi of the map (GEMM plans) -> 0-d float64 device tensor."""
+        out = torch.empty((), dtype=torch.float64, device=rho.device)
+        _lib.check(_lib.lib().ap_poisson_energy(self._h, ctypes.c_void_p(rho.data_ptr()), rho_ld or self.nby,
+                                                ctypes.c_void_p(out.data_ptr()), _stream()), "poisson_energy")
+        return out
+
     def field(self, rho):
         """rho [nbx, nby] (unmirrored) -> field [2, nbx, nby]."""
         if not (rho.is_cuda and rho.is_contiguous() and rho.dtype == torch.float32):

@@ -94,6 +94,11 @@ AP_API int ap_poisson_method(const ap_poisson* plan);
 AP_API int ap_poisson_solve(ap_poisson* plan, const float* rho, int rho_ld, float* field_x, float* field_y, int field_ld,
                      void* stream);
 
+/* Electrostatic energy of the map, E = 1/2 sum_b rho_b phi_b (phi the potential whose field ap_poisson_solve
+ * returns), reduced in FP64 into energy f64[1] on the device. The reference never evaluates it (src/calculator.py:4-5);
+ * it is the value whose position gradient the field gather approximates. Needs a plan of the GEMM method. */
+AP_API int ap_poisson_energy(ap_poisson* plan, const float* rho, int rho_ld, double* energy, void* stream);
+
 /* ---------------------------------------------------------------- part 3: the GP iteration engine */
 
 typedef struct ap_level ap_level;

@@ -72,3 +72,26 @@ def field_dct(rho, bx, by, dtype=np.float64):
     Ax = (wx * C).astype(dtype)
     Ay = (wy * C).astype(dtype)
     return Sx.T @ Ax @ Cy, Cx.T @ Ay @ Sy
+
+
+def energy_weights(nbx, nby, bx, by):
+    """w_phi[k, l] = 4/(NM) h_k h_l / ((w_k^2 + v_l^2) bx), h_0 = 1/2, h_k = 1, w_phi[0, 0] = 0: the potential of the
+    map is phi = Cx^T (w_phi o C) Cy (in the units of the field)."""
+    k = np.arange(nbx)[:, None]
+    l = np.arange(nby)[None, :]
+    w = np.pi * k / (nbx * bx)
+    v = np.pi * l / (nby * by)
+    den = (w ** 2 + v ** 2) * bx
+    with np.errstate(divide="ignore"):
+        inv = np.where(den > 0, 1.0 / np.where(den > 0, den, 1.0), 0.0)
+    return 4.0 / (nbx * nby) * np.where(k == 0, 0.5, 1.0) * np.where(l == 0, 0.5, 1.0) * inv
+
+
+def density_energy(rho, bx, by):
+    """E = 1/2 sum_b rho_b phi_b = 1/2 sum_kl w_phi C_kl^2 (C = Cx rho Cy^T), float64. Its gradient with respect to a
+    node's position is the field gather of the density kernels up to the bin discretisation."""
+    nbx, nby = rho.shape
+    Cx, _ = dct_mats(nbx)
+    Cy, _ = dct_mats(nby)
+    C = Cx @ np.asarray(rho, np.float64) @ Cy.T
+    return 0.5 * float((energy_weights(nbx, nby, bx, by) * C * C).sum())
