@@ -16,7 +16,7 @@ D = ctypes.c_double
 
 AP_OK, AP_ERR_ARG, AP_ERR_HIP, AP_ERR_UNSUPPORTED = 0, 1, 2, 3
 AP_POISSON_AUTO, AP_POISSON_GEMM, AP_POISSON_FFT = 0, 1, 2
-PROF_CLASSES = ("nesterov", "wa", "density_map", "density_post", "poisson", "gather", "control")
+PROF_CLASSES = ("nesterov", "wa", "density_bin", "density_tiles", "poisson", "gather", "control")
 
 _SIGNATURES = {
     "ap_last_error": ([], ctypes.c_char_p),
@@ -36,9 +36,13 @@ _SIGNATURES = {
     "ap_level_create": ([P, ctypes.POINTER(P)], I),
     "ap_level_destroy": ([P], I),
     "ap_level_run": ([P, I, I, P], I),
+    "ap_level_prepare": ([P, I], I),
     "ap_level_last_ms": ([P, ctypes.POINTER(F)], I),
     "ap_level_profile": ([P, I, ctypes.POINTER(D), P], I),
+    "ap_level_time_class": ([P, I], I),
+    "ap_level_class_ms": ([P, ctypes.POINTER(D), ctypes.POINTER(I)], I),
     "ap_level_objective": ([P, P, P, P, P], I),
+    "ap_level_density": ([P, P, P], I),
 }
 
 

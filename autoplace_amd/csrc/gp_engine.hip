@@ -1,17 +1,25 @@
 // The GP iteration engine (include/autoplace_gp.h, part 3): whole passes of src/gpu_loop.py:58-64 on the device.
 //
-// One pass = Nesterov step -> WA -> density scatter (force grid + overflow grid) -> post -> Poisson -> gather ->
-// control. Differences from the reference's pass, all in the data movement:
+// One pass = Nesterov step -> WA -> density (force grid + overflow grid) -> Poisson -> gather -> control.
+// Differences from the reference's pass, all in the data movement:
 //   * u / v / g are double-buffered; the control kernel flips a device flag instead of select_kernel's 3 copies,
 //     and the best placement is copied by the next pass's Nesterov kernel (which reads v anyway);
-//   * the map is kept unmirrored (the reference writes it 4x for the FFT), and cleared by the kernel that reads it;
+//   * the density map is built per tile of bins: a counting sort puts every node into the buckets of the tiles its
+//     bins fall in, and one workgroup per tile sums its bucket with LDS integer atomics and writes the tile's density
+//     and overflow once. No global atomics, no mirrored copies, no clearing pass (the reference adds every
+//     contribution to global memory with an atomic and writes the map 4x for the FFT), and the cost does not depend
+//     on the node order;
 //   * |v' - v| and |g' - g| are accumulated in FP64 by the kernels that write v' and g', the WA / HPWL totals by the
-//     WA kernel and the overflow by the post kernel, as per-workgroup partials the control kernel sums in a fixed
+//     WA kernel and the overflow by the tile kernel, as per-workgroup partials the control kernel sums in a fixed
 //     order (deterministic);
 //   * the per-cell WA gradient sum is fused into the gather.
 // Every kernel reads its parameters (gamma, weight, step) from device memory, so a run of passes has no host sync
 // and replays from a HIP graph.
 #include <stdio.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
 
 #include "gp_common.h"
 #include "gp_control.h"
@@ -30,8 +38,12 @@ struct Pick {
     const int* flag;
     __device__ __forceinline__ const float2* get(bool other) const {
         if (!flag) return a;
-        int f = *flag;
-        return (other ? !f : f) ? b : a;
+        const int f = *flag;
+        // select by arithmetic: a select between the two fields becomes an indexed load from a stack copy of the
+        // struct (scratch memory)
+        const uintptr_t ua = (uintptr_t)a, ub = (uintptr_t)b;
+        const uintptr_t m = (other ? f == 0 : f != 0) ? ~(uintptr_t)0 : 0;
+        return (const float2*)(ua ^ ((ua ^ ub) & m));
     }
 };
 
@@ -69,227 +81,259 @@ __global__ __launch_bounds__(EB) void eng_nesterov(Bufs B, const float2* lo, con
     if (threadIdx.x == 0) part[blockIdx.x] = m2[0];
 }
 
-// ------------------------------------------------------------------------------ density scatter, both grids
-struct MapJob {
+// ------------------------------------------------------------------------------ density: tile binning
+// A grid of nbx x nby bins is cut into tiles of T x T bins. Every pass:
+//   eng_bin_count    a node whose bins fall in at most 2 x 2 tiles takes a place in the bucket of each of those tiles
+//                    (LDS histogram per workgroup, then one global atomic per workgroup and bucket); any other node
+//                    goes to the grid's list of big nodes
+//   eng_bin_scan     bucket starts (exclusive scan of the counts); the counts are cleared for the next pass
+//   eng_bin_scatter  each node's centre, size and ratio into its buckets
+//   eng_bin_tiles    one workgroup per tile: the contributions of the bucket's nodes and of the big nodes, clipped to
+//                    the tile and summed with LDS integer atomics; then the tile's density and overflow partial sum
+// A bucket holds the nodes of one tile and one plane (cells, fillers). The contributions are the reference's 2^32
+// fixed-point integers (contribution() below), so the sums, and the map, do not depend on the order in which the
+// buckets were filled.
+constexpr int BIN_PER_LANE = 8;
+constexpr int BIN_NODES = EB * BIN_PER_LANE;
+constexpr int BIN_MAX_BUCKETS = 16384;   // planes x tiles of a grid: the count kernel's LDS histogram, 64 KB
+constexpr int SCAN_T = 1024;
+
+struct BinJob {
     const float2* size;
     const float* ratio;
-    int num_nodes, num_mov;
+    int num_nodes, num_mov;    // nodes of the grid; nodes from num_mov on are plane 1
     float inv_bx, inv_by;
     int nbx, nby;
-    unsigned long long* map;   // fallback accumulation (global integer atomics), read and cleared by the merge
-    int blocks;                // workgroups of this job
-    int first_block;           // index of the job's first workgroup among all map workgroups
-    unsigned long long* slots; // per workgroup: its LDS tile (MAP_LDS values), plain stores
-    int4* boxes;               // per workgroup: x0, y0, w, h | plane << 20 (w = 0: nothing in a slot)
-    int* fallback;             // set to 1 when a workgroup used the global atomics
+    int T, tiles_y, tiles;     // tile side in bins, tiles per column, tiles
+    int planes;                // 2: cells + fillers, 1: cells only
+    int blocks;                // workgroups of the count and scatter kernels for this grid
+    int* count;                // [planes tiles] bucket sizes of the pass; zero between passes
+    int* start;                // [planes tiles + 1] bucket starts
+    int4* off;                 // [num_nodes] place of the node's entry in the bucket of tile (+0,+0) (+0,+1) (+1,+0)
+                               // (+1,+1) from its first tile; -1: no entry
+    float4* geo;               // [4 num_nodes] entries: centre x, y, size x, y
+    float* gr;                 // [4 num_nodes] entries: ratio
+    int* big_count;            // [1] big nodes listed in the pass (cleared by the scan)
+    int* big_n;                // [1] big nodes of the pass (written by the scan)
+    int* big;                  // [num_nodes] big node ids
+    const float* fixed;        // [nbx][nby]
+    float* rho;                // density out (null: overflow only)
+    int count_over;            // 1: this grid's overflow feeds the control
 };
-
-// A workgroup takes MAP_NODES consecutive nodes (spatial neighbours after the engine's Morton renumbering). When
-// their bins fit MAP_LDS and they share a plane, the contributions are summed with integer LDS atomics and each
-// touched bin is flushed with one global atomic; otherwise every contribution goes to global memory directly.
-// Integer sums either way: the map is the same whatever the path and the order.
-constexpr int MAP_PER_LANE = 4;
-constexpr int MAP_NODES = EB * MAP_PER_LANE;
-constexpr int MAP_LDS = 16384;   // 128 KB: a workgroup's nodes where the density is low (fillers) still fit
 
 __device__ __forceinline__ unsigned long long contribution(float wx, const Span& sy, int by) {
     return (unsigned long long)((double)(wx * span_overlap(sy, by)) * kDensityScale);
 }
 
-__global__ __launch_bounds__(EB) void eng_density_map(Pick pos_p, MapJob f, MapJob j) {
-    extern __shared__ unsigned long long tile[];   // MAP_LDS entries (dynamic: above the static LDS limit)
-    __shared__ int red[4][EB / 64];
+// first tile of a node and whether its bins stay within 2 x 2 tiles
+__device__ __forceinline__ bool small_node(const Span& sx, const Span& sy, int T, int tiles_y, int& t0, int& wx,
+                                           int& wy) {
+    const int tx0 = sx.b0 / T, ty0 = sy.b0 / T;
+    wx = sx.b1 / T - tx0;
+    wy = sy.b1 / T - ty0;
+    t0 = tx0 * tiles_y + ty0;
+    return wx <= 1 && wy <= 1;
+}
+
+__global__ __launch_bounds__(EB) void eng_bin_count(Pick pos_p, BinJob f, BinJob j) {
+    extern __shared__ int hist[];   // planes x tiles of the grid
     const bool judge = (int)blockIdx.x >= f.blocks;
-    const MapJob& J = judge ? j : f;
-    const int base = ((int)blockIdx.x - (judge ? f.blocks : 0)) * MAP_NODES;
-    if (base >= J.num_nodes) {                                     // uniform over the workgroup
-        if (threadIdx.x == 0) J.boxes[blockIdx.x] = make_int4(0, 0, 0, 0);
-        return;
-    }
+    const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
+    const int base = ((int)blockIdx.x - (judge ? f.blocks : 0)) * BIN_NODES;
+    const int nbuck = J.planes * J.tiles;
+    for (int t = threadIdx.x; t < nbuck; t += EB) hist[t] = 0;
+    __syncthreads();
     const float2* pos = pos_p.get(true);
-    Span sx[MAP_PER_LANE], sy[MAP_PER_LANE];
-    float r[MAP_PER_LANE];
-    int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1;
-    // lane l of wave w takes nodes base + 16 l + 4 w + q: the lanes of one instruction are 16 nodes apart along the
-    // curve, so their LDS atomics rarely hit the same bin (same-address lanes serialise)
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int first = base + lane * (MAP_NODES / 64) + wv * MAP_PER_LANE;
+    int rank[BIN_PER_LANE][4], b0[BIN_PER_LANE];
 #pragma unroll
-    for (int q = 0; q < MAP_PER_LANE; q++) {
-        const int i = first + q;
-        r[q] = 0.f;
-        sx[q].b0 = 1;
-        sx[q].b1 = 0;                                             // empty unless a node is here
-        sy[q].b0 = 1;
-        sy[q].b1 = 0;
-        if (i < J.num_nodes) {
-            const float2 p = pos[i], sz = J.size[i];
-            r[q] = J.ratio[i];
-            sx[q] = make_span(p.x, sz.x, J.inv_bx, J.nbx);
-            sy[q] = make_span(p.y, sz.y, J.inv_by, J.nby);
-            if (sx[q].b0 <= sx[q].b1 && sy[q].b0 <= sy[q].b1) {
-                x0 = min(x0, sx[q].b0); x1 = max(x1, sx[q].b1);
-                y0 = min(y0, sy[q].b0); y1 = max(y1, sy[q].b1);
-            }
+    for (int q = 0; q < BIN_PER_LANE; q++) {
+        rank[q][0] = rank[q][1] = rank[q][2] = rank[q][3] = -1;
+        b0[q] = 0;
+        const int i = base + q * EB + threadIdx.x;
+        if (i >= J.num_nodes) continue;
+        const float2 p = pos[i], sz = J.size[i];
+        const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
+        if (sx.b0 > sx.b1 || sy.b0 > sy.b1) continue;
+        int t0, wx, wy;
+        if (!small_node(sx, sy, J.T, J.tiles_y, t0, wx, wy)) {
+            J.big[atomicAdd(J.big_count, 1)] = i;
+            continue;
         }
-    }
-    // bounding box of the workgroup's bins
-    for (int o = 32; o > 0; o >>= 1) {
-        x0 = min(x0, __shfl_xor(x0, o, 64)); y0 = min(y0, __shfl_xor(y0, o, 64));
-        x1 = max(x1, __shfl_xor(x1, o, 64)); y1 = max(y1, __shfl_xor(y1, o, 64));
-    }
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[0][wave] = x0; red[1][wave] = y0; red[2][wave] = x1; red[3][wave] = y1;
+        b0[q] = t0 + (i >= J.num_mov ? J.tiles : 0);
+        rank[q][0] = atomicAdd(&hist[b0[q]], 1);
+        if (wy) rank[q][1] = atomicAdd(&hist[b0[q] + 1], 1);
+        if (wx) rank[q][2] = atomicAdd(&hist[b0[q] + J.tiles_y], 1);
+        if (wx && wy) rank[q][3] = atomicAdd(&hist[b0[q] + J.tiles_y + 1], 1);
     }
     __syncthreads();
-    x0 = red[0][0]; y0 = red[1][0]; x1 = red[2][0]; y1 = red[3][0];
-    for (int w = 1; w < EB / 64; w++) {
-        x0 = min(x0, red[0][w]); y0 = min(y0, red[1][w]); x1 = max(x1, red[2][w]); y1 = max(y1, red[3][w]);
+    // this workgroup's share of every bucket it touched: hist[b] becomes its first place in bucket b
+    for (int t = threadIdx.x; t < nbuck; t += EB) {
+        const int c = hist[t];
+        if (c) hist[t] = atomicAdd(&J.count[t], c);
     }
-    const int last = min(base + MAP_NODES, J.num_nodes) - 1;
-    const bool one_plane = (base < J.num_mov) == (last < J.num_mov);
-    const long long plane = base < J.num_mov ? 0 : (long long)J.nbx * J.nby;
-    const int w = x1 - x0 + 1, h = y1 - y0 + 1;
-    const int gb = (int)blockIdx.x;                                // slot / box of this workgroup
-    if (x1 < x0) {                                                 // no bins at all (uniform)
-        if (threadIdx.x == 0) J.boxes[gb] = make_int4(0, 0, 0, 0);
-        return;
-    }
-    if (one_plane && (long long)w * h <= MAP_LDS) {
-        for (int e = threadIdx.x; e < w * h; e += EB) tile[e] = 0ull;
-        __syncthreads();
+    __syncthreads();
 #pragma unroll
-        for (int q = 0; q < MAP_PER_LANE; q++)
-            for (int bx = sx[q].b0; bx <= sx[q].b1; bx++) {
-                const float wx = r[q] * span_overlap(sx[q], bx);
-                unsigned long long* row = tile + (bx - x0) * h - y0;
-                for (int by = sy[q].b0; by <= sy[q].b1; by++) {
-                    const unsigned long long c = contribution(wx, sy[q], by);
-                    if (c) atomicAdd(row + by, c);
-                }
-            }
-        __syncthreads();
-        // the tile goes to this workgroup's own slot with plain stores; the merge kernel sums the slots per bin
-        unsigned long long* slot = J.slots + (long long)gb * MAP_LDS;
-        for (int e = threadIdx.x; e < w * h; e += EB) slot[e] = tile[e];
-        if (threadIdx.x == 0) J.boxes[gb] = make_int4(x0, y0, w, h | (plane ? (1 << 20) : 0));
-        return;
+    for (int q = 0; q < BIN_PER_LANE; q++) {
+        const int i = base + q * EB + threadIdx.x;
+        if (i >= J.num_nodes) continue;
+        int4 o;
+        o.x = rank[q][0] >= 0 ? hist[b0[q]] + rank[q][0] : -1;
+        o.y = rank[q][1] >= 0 ? hist[b0[q] + 1] + rank[q][1] : -1;
+        o.z = rank[q][2] >= 0 ? hist[b0[q] + J.tiles_y] + rank[q][2] : -1;
+        o.w = rank[q][3] >= 0 ? hist[b0[q] + J.tiles_y + 1] + rank[q][3] : -1;
+        J.off[i] = o;
     }
+}
+
+// one workgroup per grid
+__global__ __launch_bounds__(SCAN_T) void eng_bin_scan(BinJob f, BinJob j) {
+    __shared__ int wsum[SCAN_T / 64];
+    const BinJob J = blockIdx.x ? j : f;
+    const int n = J.planes * J.tiles;
+    const int per = (n + SCAN_T - 1) / SCAN_T;
+    const int lo = min(n, (int)threadIdx.x * per), hi = min(n, lo + per);
+    int s = 0;
+    for (int k = lo; k < hi; k++) s += J.count[k];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int run = x - s;
+    for (int w = 0; w < wave; w++) run += wsum[w];
+    for (int k = lo; k < hi; k++) {
+        const int c = J.count[k];
+        J.start[k] = run;
+        run += c;
+        J.count[k] = 0;
+    }
+    if (threadIdx.x == SCAN_T - 1) J.start[n] = run;   // the last lane's running sum is the total
     if (threadIdx.x == 0) {
-        J.boxes[gb] = make_int4(0, 0, 0, 0);
-        *J.fallback = 1;
+        *J.big_n = *J.big_count;
+        *J.big_count = 0;
     }
-#pragma unroll
-    for (int q = 0; q < MAP_PER_LANE; q++) {
-        const int i = first + q;
-        unsigned long long* m = J.map + (i < J.num_mov ? 0 : (long long)J.nbx * J.nby);
-        for (int bx = sx[q].b0; bx <= sx[q].b1; bx++) {
-            const float wx = r[q] * span_overlap(sx[q], bx);
-            unsigned long long* row = m + (long long)bx * J.nby;
-            for (int by = sy[q].b0; by <= sy[q].b1; by++) {
-                const unsigned long long c = contribution(wx, sy[q], by);
-                if (c) atomicAdd(row + by, c);
-            }
+}
+
+__global__ __launch_bounds__(EB) void eng_bin_scatter(Pick pos_p, BinJob f, BinJob j) {
+    const bool judge = (int)blockIdx.x >= f.blocks;
+    const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
+    const int base = ((int)blockIdx.x - (judge ? f.blocks : 0)) * BIN_NODES;
+    const float2* pos = pos_p.get(true);
+#pragma unroll 2
+    for (int q = 0; q < BIN_PER_LANE; q++) {
+        const int i = base + q * EB + threadIdx.x;
+        if (i >= J.num_nodes) continue;
+        const int4 o = J.off[i];
+        if (o.x < 0) continue;                                  // big node, or no bins
+        const float2 p = pos[i], sz = J.size[i];
+        const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
+        int t0, wx, wy;
+        small_node(sx, sy, J.T, J.tiles_y, t0, wx, wy);
+        const int b = t0 + (i >= J.num_mov ? J.tiles : 0);
+        const float4 g = make_float4(p.x, p.y, sz.x, sz.y);
+        const float r = J.ratio[i];
+        int e = J.start[b] + o.x;
+        J.geo[e] = g;
+        J.gr[e] = r;
+        if (o.y >= 0) {
+            e = J.start[b + 1] + o.y;
+            J.geo[e] = g;
+            J.gr[e] = r;
+        }
+        if (o.z >= 0) {
+            e = J.start[b + J.tiles_y] + o.z;
+            J.geo[e] = g;
+            J.gr[e] = r;
+        }
+        if (o.w >= 0) {
+            e = J.start[b + J.tiles_y + 1] + o.w;
+            J.geo[e] = g;
+            J.gr[e] = r;
         }
     }
 }
 
-// maps -> unmirrored total density (force grid) and the overflow partial sums; the maps are cleared for the next pass
-// Merge + post, one workgroup per 32 x 32 tile of bins: the tile's value is the sum of the slots of the map
-// workgroups whose box covers it (found by scanning the boxes in workgroup order: a fixed order, and integer sums
-// anyway) plus the fallback map when any workgroup used it; then density, overflow, and the map cleared.
-constexpr int MT = 32;
-struct PostJob {
-    unsigned long long* map;
-    int planes;                // 2: cells + fillers (force grid), 1: cells only
-    const float* fixed;
-    float* rho;                // null: overflow only
-    int nbx, nby;
-    int tiles_y;               // tiles per column of tiles
-    int blocks;                // tiles of this grid
-    int count_over;            // 1: this grid's overflow feeds the control
-    const unsigned long long* slots;
-    const int4* boxes;
-    int first_box, num_boxes;  // the map workgroups of this grid
-    const int* fallback;
-};
+// a node's contributions to the bins of the tile [X0, X1] x [Y0, Y1], into the tile's LDS accumulator a (row stride T)
+__device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0, int Y0, int X1, int Y1,
+                                            const Span& sx, const Span& sy, float r) {
+    const int x0 = max(sx.b0, X0), x1 = min(sx.b1, X1), y0 = max(sy.b0, Y0), y1 = min(sy.b1, Y1);
+    for (int bx = x0; bx <= x1; bx++) {
+        const float wx = r * span_overlap(sx, bx);
+        unsigned long long* row = a + (bx - X0) * T - Y0;
+        for (int by = y0; by <= y1; by++) {
+            const unsigned long long c = contribution(wx, sy, by);
+            if (c) atomicAdd(row + by, c);
+        }
+    }
+}
 
-__global__ __launch_bounds__(EB) void eng_density_merge(PostJob f, PostJob j, float target, double* part) {
+constexpr int TB = 512;   // lanes per workgroup of the tile kernel: a crowded tile's bucket spreads over 8 waves
+
+__global__ __launch_bounds__(TB) void eng_bin_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
+    extern __shared__ unsigned long long acc[];   // planes x T x T
     __shared__ double red[16];
-    __shared__ int list[EB + 1];
-    __shared__ int nlist;
-    __shared__ int wcount[EB / 64];
-    const bool judge = (int)blockIdx.x >= f.blocks;
-    const PostJob& J = judge ? j : f;
-    const int tile = (int)blockIdx.x - (judge ? f.blocks : 0);
-    const int X0 = (tile / J.tiles_y) * MT, Y0 = (tile % J.tiles_y) * MT;
-    const long long nb = (long long)J.nbx * J.nby;
-    const bool fb = *J.fallback != 0;
-    // each lane owns 4 bins of the tile: local e = lane + 256 k -> (e / 32, e % 32)
-    unsigned long long acc0[4], acc1[4];
-    long long bin[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int e = threadIdx.x + EB * k, bx = X0 + e / MT, by = Y0 + e % MT;
-        bin[k] = (bx < J.nbx && by < J.nby) ? (long long)bx * J.nby + by : -1;
-        acc0[k] = acc1[k] = 0ull;
-        if (fb && bin[k] >= 0) {
-            acc0[k] = J.map[bin[k]];
-            J.map[bin[k]] = 0ull;
-            if (J.planes == 2) {
-                acc1[k] = J.map[nb + bin[k]];
-                J.map[nb + bin[k]] = 0ull;
+    const bool judge = (int)blockIdx.x >= f.tiles;
+    const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
+    const int t = (int)blockIdx.x - (judge ? f.tiles : 0);
+    const int T = J.T, TT = T * T;
+    const int X0 = (t / J.tiles_y) * T, Y0 = (t % J.tiles_y) * T;
+    const int X1 = min(X0 + T, J.nbx) - 1, Y1 = min(Y0 + T, J.nby) - 1;
+    for (int e = threadIdx.x; e < J.planes * TT; e += TB) acc[e] = 0ull;
+    __syncthreads();
+    for (int pl = 0; pl < J.planes; pl++) {
+        const int b = pl * J.tiles + t;
+        const int e1 = J.start[b + 1];
+        unsigned long long* a = acc + pl * TT;
+        // two entries per lane per round, both loaded before either is used
+        for (int e = J.start[b] + threadIdx.x; e < e1; e += 2 * TB) {
+            const bool two = e + TB < e1;
+            const float4 g0 = J.geo[e];
+            const float r0 = J.gr[e];
+            float4 g1 = g0;
+            float r1 = 0.f;
+            if (two) {
+                g1 = J.geo[e + TB];
+                r1 = J.gr[e + TB];
+            }
+            add_clipped(a, T, X0, Y0, X1, Y1, make_span(g0.x, g0.z, J.inv_bx, J.nbx),
+                        make_span(g0.y, g0.w, J.inv_by, J.nby), r0);
+            if (two)
+                add_clipped(a, T, X0, Y0, X1, Y1, make_span(g1.x, g1.z, J.inv_bx, J.nbx),
+                            make_span(g1.y, g1.w, J.inv_by, J.nby), r1);
+        }
+    }
+    // big nodes: one at a time, their bins in this tile spread over the lanes
+    const int nbig = *J.big_n;
+    if (nbig) {
+        const float2* pos = pos_p.get(true);
+        for (int k = 0; k < nbig; k++) {
+            const int i = J.big[k];
+            const float2 p = pos[i], sz = J.size[i];
+            const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
+            const int x0 = max(sx.b0, X0), x1 = min(sx.b1, X1), y0 = max(sy.b0, Y0), y1 = min(sy.b1, Y1);
+            if (x0 > x1 || y0 > y1) continue;                   // uniform over the workgroup
+            const float r = J.ratio[i];
+            unsigned long long* a = acc + (i >= J.num_mov ? TT : 0);
+            const int h = y1 - y0 + 1, n = (x1 - x0 + 1) * h;
+            for (int e = threadIdx.x; e < n; e += TB) {
+                const int bx = x0 + e / h, by = y0 + e % h;
+                const unsigned long long c = contribution(r * span_overlap(sx, bx), sy, by);
+                if (c) atomicAdd(a + (bx - X0) * T + (by - Y0), c);
             }
         }
     }
-    // boxes overlapping the tile, in workgroup order, EB at a time
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int c0 = 0; c0 < J.num_boxes; c0 += EB) {
-        const int wgi = c0 + threadIdx.x;
-        bool hit = false;
-        int4 bx4 = make_int4(0, 0, 0, 0);
-        if (wgi < J.num_boxes) {
-            bx4 = J.boxes[J.first_box + wgi];
-            const int h = bx4.w & 0xFFFFF;
-            hit = bx4.z > 0 && bx4.x < X0 + MT && bx4.x + bx4.z > X0 && bx4.y < Y0 + MT && bx4.y + h > Y0;
-        }
-        const unsigned long long m = __ballot(hit);
-        if (lane == 0) wcount[wave] = __popcll(m);
-        __syncthreads();
-        int before = 0, total = 0;
-        for (int w = 0; w < EB / 64; w++) {
-            if (w < wave) before += wcount[w];
-            total += wcount[w];
-        }
-        if (hit) list[before + __popcll(m & ((1ull << lane) - 1))] = J.first_box + wgi;
-        if (threadIdx.x == 0) nlist = total;
-        __syncthreads();
-        for (int q = 0; q < nlist; q++) {
-            const int g = list[q];
-            const int4 b4 = J.boxes[g];
-            const int h = b4.w & 0xFFFFF, plane = b4.w >> 20;
-            const unsigned long long* slot = J.slots + (long long)g * MAP_LDS;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (bin[k] < 0) continue;
-                const int e = threadIdx.x + EB * k, bx = X0 + e / MT, by = Y0 + e % MT;
-                if (bx >= b4.x && bx < b4.x + b4.z && by >= b4.y && by < b4.y + h) {
-                    const unsigned long long v = slot[(bx - b4.x) * h + (by - b4.y)];
-                    if (plane) acc1[k] += v;
-                    else acc0[k] += v;
-                }
-            }
-        }
-        __syncthreads();
-    }
+    __syncthreads();
     double o[1] = {0.0};
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        if (bin[k] < 0) continue;
-        const float mov = (float)((double)acc0[k] / kDensityScale) + J.fixed[bin[k]];
-        const float r = J.planes == 2 ? mov + (float)((double)acc1[k] / kDensityScale) : mov;
-        if (J.rho) J.rho[bin[k]] = r;
+    for (int e = threadIdx.x; e < TT; e += TB) {
+        const int bx = X0 + e / T, by = Y0 + e % T;
+        if (bx > X1 || by > Y1) continue;
+        const long long bin = (long long)bx * J.nby + by;
+        const float mov = (float)((double)acc[e] / kDensityScale) + J.fixed[bin];
+        const float r = J.planes == 2 ? mov + (float)((double)acc[TT + e] / kDensityScale) : mov;
+        if (J.rho) J.rho[bin] = r;
         if (J.count_over) o[0] += (double)fmaxf(mov - target, 0.f);
     }
     block_sum<1>(o, red);
@@ -423,14 +467,12 @@ __global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) {
 }
 
 __global__ __launch_bounds__(64) void eng_control(const double* red5, double over_scale, double* S, float* kp,
-                                                  const double* c, int* flags, double* stats, int* fallback) {
+                                                  const double* c, int* flags, double* stats) {
     if (threadIdx.x == 0) {
-        *fallback = 0;             // the merge of this pass has read it
         double r[5] = {0, 0, 0, 0, 0};
         for (int b = 0; b < RED_BLOCKS; b++)
             for (int k = 0; k < 5; k++) r[k] += red5[b * 5 + k];
-        const Segs s{nullptr, 0, nullptr, 0, nullptr, 0, nullptr, 0, over_scale};
-        const double moved = sqrt(r[0]), turned = sqrt(r[1]), wa = r[2], hpwl = r[3], ovf = r[4] * s.over_scale;
+        const double moved = sqrt(r[0]), turned = sqrt(r[1]), wa = r[2], hpwl = r[3], ovf = r[4] * over_scale;
         if (stats) {
             stats[0] = wa; stats[1] = hpwl; stats[2] = ovf; stats[3] = moved; stats[4] = turned;
         }
@@ -477,6 +519,50 @@ __global__ void eng_finalize_flags(int* flags) {
 }
 
 // ------------------------------------------------------------------------------ the level
+// Tile side of a grid: the power of two from 8 to 64 at or below the side that gives BIN_TILE_NODES nodes per tile
+// on average (2048 x 2048 bins and 2.9M nodes: 32; 512 x 512 bins and 2M cells: 8), raised while more than
+// max(64, nodes / 1024) nodes could span more than T + 1 bins along an axis (those take the big-node path, which
+// every tile's workgroup scans) or while the buckets do not fit the count kernel's histogram. 0: no side fits.
+constexpr int BIN_TILE_NODES = 1024;
+static int tile_side(const std::vector<float2>& size, double bx, double by, int nbx, int nby, int planes) {
+    const long long n = (long long)size.size();
+    const double want = std::sqrt((double)BIN_TILE_NODES * nbx * nby / (double)std::max(n, 1LL));
+    int T = 8;
+    while (T < 64 && 2 * T <= want) T *= 2;
+    // bins a node can touch along an axis: floor(size / bin) + 2 at most
+    std::vector<int> reach(n);
+    for (long long i = 0; i < n; i++)
+        reach[i] = (int)std::min(1e9, std::max(std::floor(size[i].x / bx), std::floor(size[i].y / by)) + 2);
+    const long long allow = std::max(64LL, n / 1024);
+    int fits = 0;
+    for (; T <= 64; T *= 2) {
+        if ((long long)planes * grid_of(nbx, T) * grid_of(nby, T) > BIN_MAX_BUCKETS) continue;
+        fits = T;
+        long long big = 0;
+        for (long long i = 0; i < n; i++) big += reach[i] > T + 1;
+        if (big <= allow) return T;
+    }
+    return fits;   // the largest side that fits; its big nodes take the slower path
+}
+
+struct GraphEntry {            // a captured run of `passes` passes
+    int passes = 0;
+    int timing = -1;           // kernel class bracketed by events (-1: none)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> ev;   // timing: 2 per pass (before the class, after it)
+    void clear() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        exec = nullptr;
+        graph = nullptr;
+        passes = 0;
+        timing = -1;
+    }
+};
+constexpr int GRAPH_CACHE = 4;
 
 struct Level {
     ap_level_desc d{};
@@ -485,28 +571,55 @@ struct Level {
     float2 *u1 = nullptr, *v1 = nullptr, *g1 = nullptr, *pin_grad = nullptr;
     int* block_net = nullptr;  // WA workgroup -> first owned net
     int* pin_slot = nullptr;   // pin -> position in the cell CSR (-1: fixed pin)
-    unsigned long long *map = nullptr, *jmap = nullptr;
-    unsigned long long* slots = nullptr;   // per map workgroup: its LDS tile
-    int4* boxes = nullptr;                 // per map workgroup: the tile's box
-    int* fallback = nullptr;               // a map workgroup used the global atomics in this pass
+    BinJob fj{}, jj{};         // density binning of the force grid and of the overflow grid
+    size_t hist_lds = 0, acc_lds = 0;
+    std::vector<void*> bin_mem;
     float *rho = nullptr, *fx = nullptr, *fy = nullptr;
     double* part = nullptr;
     double* stats = nullptr;
     double* red5 = nullptr;    // first-stage sums, RED_BLOCKS x 5
     int* flags = nullptr;
-    int nblk_n = 0, nblk_wa = 0, nblk_fpost = 0, nblk_jpost = 0, nblk_fmap = 0, nblk_jmap = 0;
+    int nblk_n = 0, nblk_wa = 0;
     double *p_moved = nullptr, *p_wa = nullptr, *p_over = nullptr, *p_turned = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int graph_passes = 0;
+    GraphEntry graphs[GRAPH_CACHE];
+    int graph_next = 0;
+    // event timing of one kernel class: events before and after it in every pass of a run (graph or direct)
+    int timing_cls = -1;
+    hipEvent_t* rec_ev = nullptr;   // events of the run being captured or launched (null: none)
+    bool rec_capture = false;       // true: added to the graph being captured as event-record nodes
+    int rec_pass = 0;
+    hipError_t rec_err = hipSuccess;
+    std::vector<hipEvent_t> direct_ev;                 // events of direct runs
+    const std::vector<hipEvent_t>* last_ev = nullptr;  // events of the last run (null: it had none)
+    int last_ev_passes = 0;
     // profiling: one event before each kernel class and one at the end of the pass
     bool prof = false;
     hipEvent_t pev[AP_PROF_CLASSES + 1] = {};
     double prof_ms[AP_PROF_CLASSES] = {};
     void mark(int k, hipStream_t s) {
         if (prof) (void)hipEventRecord(pev[k], s);
+        if (rec_ev && (k == timing_cls || k == timing_cls + 1)) {
+            hipEvent_t ev = rec_ev[2 * rec_pass + (k == timing_cls ? 0 : 1)];
+            const hipError_t e = rec_capture ? record_in_capture(ev, s) : hipEventRecord(ev, s);
+            if (e != hipSuccess && rec_err == hipSuccess) rec_err = e;
+        }
+    }
+    // an event-record node after the nodes captured so far, which the next captured work depends on: the event of a
+    // stream-ordered hipEventRecord, inside the graph
+    static hipError_t record_in_capture(hipEvent_t ev, hipStream_t s) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        unsigned long long id = 0;
+        hipGraph_t g = nullptr;
+        const hipGraphNode_t* deps = nullptr;
+        size_t nd = 0;
+        hipError_t e = hipStreamGetCaptureInfo_v2(s, &cs, &id, &g, &deps, &nd);
+        if (e != hipSuccess) return e;
+        if (cs != hipStreamCaptureStatusActive || !g) return hipErrorIllegalState;
+        hipGraphNode_t node = nullptr;
+        if ((e = hipGraphAddEventRecordNode(&node, g, deps, nd, ev)) != hipSuccess) return e;
+        return hipStreamUpdateCaptureDependencies(s, &node, 1, hipStreamSetCaptureDependencies);
     }
 
     Bufs bufs() const {
@@ -519,13 +632,61 @@ struct Level {
         return B;
     }
 
+    template <typename T>
+    int alloc(T** p, long long n) {
+        AP_HIP(hipMalloc((void**)p, (size_t)(n > 0 ? n : 1) * sizeof(T)));
+        bin_mem.push_back((void*)*p);
+        return AP_OK;
+    }
+    int bin_job(BinJob& J, const float* size, const float* ratio, int num_nodes, int num_mov, double bx, double by,
+                int nbx, int nby, int planes, const float* fixed, float* out, int count_over);
     int create(const ap_level_desc& desc);
     void destroy();
     int objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev, const float2* g_prev_b,
                   float2* g_out_a, float2* g_out_b, hipStream_t s, double* S, float* kp_out, int use_density);
     int pass(hipStream_t s);
     int finalize(hipStream_t s);
+    int graph_for(int passes, int& slot);
 };
+
+int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nodes, int num_mov, double bx, double by,
+                   int nbx, int nby, int planes, const float* fixed, float* out, int count_over) {
+    J = BinJob{};
+    J.size = (const float2*)size;
+    J.ratio = ratio;
+    J.num_nodes = num_nodes;
+    J.num_mov = num_mov;
+    J.inv_bx = 1.f / (float)bx;   // the same float32 reciprocal as the reference's host code: 1.f / (float)bx
+    J.inv_by = 1.f / (float)by;
+    J.nbx = nbx;
+    J.nby = nby;
+    J.planes = planes;
+    std::vector<float2> host_size(num_nodes);
+    AP_HIP(hipMemcpy(host_size.data(), size, (size_t)num_nodes * sizeof(float2), hipMemcpyDeviceToHost));
+    J.T = tile_side(host_size, bx, by, nbx, nby, planes);
+    if (!J.T) {
+        set_error("level: density grid %d x %d is beyond the tile binning's %d buckets", nbx, nby, BIN_MAX_BUCKETS);
+        return AP_ERR_UNSUPPORTED;
+    }
+    J.tiles_y = (int)grid_of(nby, J.T);
+    J.tiles = (int)grid_of(nbx, J.T) * J.tiles_y;
+    J.blocks = (int)grid_of(num_nodes, BIN_NODES);
+    J.fixed = fixed;
+    J.rho = out;
+    J.count_over = count_over;
+    const long long nbuck = (long long)planes * J.tiles;
+    int st;
+    if ((st = alloc(&J.count, nbuck)) || (st = alloc(&J.start, nbuck + 1)) || (st = alloc(&J.off, num_nodes)) ||
+        (st = alloc(&J.geo, 4LL * num_nodes)) || (st = alloc(&J.gr, 4LL * num_nodes)) ||
+        (st = alloc(&J.big_count, 1)) || (st = alloc(&J.big_n, 1)) || (st = alloc(&J.big, num_nodes)))
+        return st;
+    AP_HIP(hipMemset(J.count, 0, nbuck * sizeof(int)));
+    AP_HIP(hipMemset(J.big_count, 0, sizeof(int)));
+    AP_HIP(hipMemset(J.big_n, 0, sizeof(int)));
+    hist_lds = std::max(hist_lds, (size_t)nbuck * sizeof(int));
+    acc_lds = std::max(acc_lds, (size_t)planes * J.T * J.T * sizeof(unsigned long long));
+    return AP_OK;
+}
 
 int Level::create(const ap_level_desc& desc) {
     d = desc;
@@ -540,7 +701,7 @@ int Level::create(const ap_level_desc& desc) {
                "level: null pointer");
     int st = poisson.create(d.nbx, d.nby, d.bx, d.by, d.poisson_method);
     if (st) return st;
-    const long long N = d.num_nodes, nb = (long long)d.nbx * d.nby, nj = d.has_judge ? (long long)d.njx * d.njy : 0;
+    const long long N = d.num_nodes, nb = (long long)d.nbx * d.nby;
     AP_HIP(hipMalloc(&u1, N * sizeof(float2)));
     AP_HIP(hipMalloc(&v1, N * sizeof(float2)));
     AP_HIP(hipMalloc(&g1, N * sizeof(float2)));
@@ -554,35 +715,31 @@ int Level::create(const ap_level_desc& desc) {
                            d.cell_start, d.cell_pin, d.num_mov, d.num_pins);
         AP_LAUNCHED();
     }
-    AP_HIP(hipMalloc(&map, 2 * nb * sizeof(unsigned long long)));
-    AP_HIP(hipMemset(map, 0, 2 * nb * sizeof(unsigned long long)));
-    if (nj) {
-        AP_HIP(hipMalloc(&jmap, nj * sizeof(unsigned long long)));
-        AP_HIP(hipMemset(jmap, 0, nj * sizeof(unsigned long long)));
-    }
     AP_HIP(hipMalloc(&rho, nb * sizeof(float)));
     AP_HIP(hipMalloc(&fx, nb * sizeof(float)));
     AP_HIP(hipMalloc(&fy, nb * sizeof(float)));
     nblk_n = grid_of(N, EB);
     if ((st = wa_plan.create(d.net_start, d.num_nets))) return st;
-    AP_HIP(hipFuncSetAttribute((const void*)eng_density_map, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               MAP_LDS * sizeof(unsigned long long)));
     nblk_wa = wa_plan.partial_count();
-    nblk_fmap = grid_of(N, MAP_NODES);
-    nblk_jmap = nj ? grid_of(d.num_mov, MAP_NODES) : 0;
-    nblk_fpost = grid_of(d.nbx, MT) * grid_of(d.nby, MT);
-    nblk_jpost = nj ? grid_of(d.njx, MT) * grid_of(d.njy, MT) : 0;
-    AP_HIP(hipMalloc(&slots, (long long)(nblk_fmap + nblk_jmap) * MAP_LDS * sizeof(unsigned long long)));
-    AP_HIP(hipMalloc(&boxes, (nblk_fmap + nblk_jmap + 1) * sizeof(int4)));
-    AP_HIP(hipMalloc(&fallback, sizeof(int)));
-    AP_HIP(hipMemset(fallback, 0, sizeof(int)));
-    const long long np = nblk_n + 2LL * nblk_wa + nblk_fpost + nblk_jpost + nblk_n;
+    if ((st = bin_job(fj, d.size, d.ratio, d.num_nodes, d.num_mov, d.bx, d.by, d.nbx, d.nby, 2, d.fixed, rho,
+                      d.has_judge ? 0 : 1)))
+        return st;
+    if (d.has_judge &&
+        (st = bin_job(jj, d.judge_size, d.judge_ratio, d.num_mov, d.num_mov, d.jbx, d.jby, d.njx, d.njy, 1,
+                      d.judge_fixed, nullptr, 1)))
+        return st;
+    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)hist_lds));
+    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)acc_lds));
+    const long long n_over = fj.tiles + jj.tiles;
+    const long long np = nblk_n + 2LL * nblk_wa + n_over + nblk_n;
     AP_HIP(hipMalloc(&part, np * sizeof(double)));
     AP_HIP(hipMemset(part, 0, np * sizeof(double)));
     p_moved = part;
     p_wa = p_moved + nblk_n;
     p_over = p_wa + 2LL * nblk_wa;
-    p_turned = p_over + nblk_fpost + nblk_jpost;
+    p_turned = p_over + n_over;
     AP_HIP(hipMalloc(&stats, 8 * sizeof(double)));
     AP_HIP(hipMalloc(&red5, RED_BLOCKS * 5 * sizeof(double)));
     AP_HIP(hipMalloc(&flags, 4 * sizeof(int)));
@@ -597,14 +754,17 @@ int Level::create(const ap_level_desc& desc) {
 }
 
 void Level::destroy() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (GraphEntry& G : graphs) G.clear();
+    for (hipEvent_t e : direct_ev) (void)hipEventDestroy(e);
+    direct_ev.clear();
+    last_ev = nullptr;
     for (hipEvent_t& ev : pev)
         if (ev) (void)hipEventDestroy(ev);
-    void* ptrs[] = {u1,    v1,    g1,       pin_grad, map,      jmap,  rho,   fx, fy, part, stats, flags, red5,
-                    block_net, pin_slot, slots, boxes, fallback};
+    void* ptrs[] = {u1, v1, g1, pin_grad, rho, fx, fy, part, stats, flags, red5, block_net, pin_slot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (void* p : bin_mem) (void)hipFree(p);
+    bin_mem.clear();
     poisson.destroy();
     wa_plan.destroy();
     if (stream) (void)hipStreamDestroy(stream);
@@ -629,28 +789,18 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     int n_over = 0;
     double over_scale = 0;
     if (use_density) {
-        // the same float32 reciprocal as the reference's host code: 1.f / (float)bx
-        MapJob f{(const float2*)d.size, d.ratio, d.num_nodes, d.num_mov, 1.f / (float)d.bx, 1.f / (float)d.by,
-                 d.nbx, d.nby, map, nblk_fmap, 0, slots, boxes, fallback};
-        MapJob j{(const float2*)d.judge_size, d.judge_ratio, d.num_mov, d.num_mov, 0.f, 0.f, d.njx, d.njy, jmap,
-                 nblk_jmap, nblk_fmap, slots, boxes, fallback};
-        if (d.has_judge) {
-            j.inv_bx = 1.f / (float)d.jbx;
-            j.inv_by = 1.f / (float)d.jby;
-        }
         mark(AP_PROF_MAP, s);
-        hipLaunchKernelGGL(eng_density_map, dim3(nblk_fmap + nblk_jmap), dim3(EB),
-                           MAP_LDS * sizeof(unsigned long long), s, pos, f, j);
+        hipLaunchKernelGGL(eng_bin_count, dim3(fj.blocks + jj.blocks), dim3(EB), hist_lds, s, pos, fj, jj);
         AP_LAUNCHED();
-        PostJob pf{map, 2, d.fixed, rho, d.nbx, d.nby, (int)grid_of(d.nby, MT), nblk_fpost, d.has_judge ? 0 : 1,
-                   slots, boxes, 0, nblk_fmap, fallback};
-        PostJob pj{jmap, 1, d.judge_fixed, nullptr, d.njx, d.njy, d.has_judge ? (int)grid_of(d.njy, MT) : 1,
-                   nblk_jpost, 1, slots, boxes, nblk_fmap, nblk_jmap, fallback};
+        hipLaunchKernelGGL(eng_bin_scan, dim3(d.has_judge ? 2 : 1), dim3(SCAN_T), 0, s, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_bin_scatter, dim3(fj.blocks + jj.blocks), dim3(EB), 0, s, pos, fj, jj);
+        AP_LAUNCHED();
         mark(AP_PROF_POST, s);
-        hipLaunchKernelGGL(eng_density_merge, dim3(nblk_fpost + nblk_jpost), dim3(EB), 0, s, pf, pj, d.target,
+        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.tiles + jj.tiles), dim3(TB), acc_lds, s, pos, fj, jj, d.target,
                            p_over);
         AP_LAUNCHED();
-        n_over = nblk_fpost + nblk_jpost;
+        n_over = fj.tiles + jj.tiles;
         over_scale = d.has_judge ? d.judge_bin_area / d.mov_area : d.bin_area / d.mov_area;
         mark(AP_PROF_POISSON, s);
         if ((st = poisson.solve(rho, d.nby, fx, fy, d.nby, s))) return st;
@@ -688,8 +838,7 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
             over_scale};
     hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(EB), 0, s, sg, red5);
     AP_LAUNCHED();
-    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red5, over_scale, S, kp_out, d.c, flags, stats,
-                       fallback);
+    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red5, over_scale, S, kp_out, d.c, flags, stats);
     AP_LAUNCHED();
     mark(AP_PROF_CLASSES, s);
     return AP_OK;
@@ -710,6 +859,50 @@ int Level::finalize(hipStream_t s) {
     AP_LAUNCHED();
     hipLaunchKernelGGL(eng_finalize_flags, dim3(1), dim3(64), 0, s, flags);
     AP_LAUNCHED();
+    return AP_OK;
+}
+
+// the captured graph of `passes` passes with the current timing class, captured now when not cached
+int Level::graph_for(int passes, int& slot) {
+    for (int k = 0; k < GRAPH_CACHE; k++)
+        if (graphs[k].exec && graphs[k].passes == passes && graphs[k].timing == timing_cls) {
+            slot = k;
+            return AP_OK;
+        }
+    slot = graph_next;
+    graph_next = (graph_next + 1) % GRAPH_CACHE;
+    GraphEntry& G = graphs[slot];
+    if (last_ev == &G.ev) last_ev = nullptr;
+    G.clear();
+    if (timing_cls >= 0) {
+        G.ev.resize(2 * (size_t)passes, nullptr);
+        for (hipEvent_t& e : G.ev) AP_HIP(hipEventCreate(&e));
+    }
+    rec_ev = timing_cls >= 0 ? G.ev.data() : nullptr;
+    rec_capture = true;
+    rec_err = hipSuccess;
+    AP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int st = AP_OK;
+    for (int k = 0; k < passes && !st; k++) {
+        rec_pass = k;
+        st = pass(stream);
+    }
+    if (!st) st = finalize(stream);
+    rec_ev = nullptr;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &g);
+    if (st || e != hipSuccess || rec_err != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        G.clear();
+        (void)hipGetLastError();
+        if (st) return st;
+        set_error("level: graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : rec_err));
+        return AP_ERR_HIP;
+    }
+    G.graph = g;
+    AP_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+    G.passes = passes;
+    G.timing = timing_cls;
     return AP_OK;
 }
 
@@ -745,47 +938,80 @@ int ap_level_destroy(ap_level* level) {
     return AP_OK;
 }
 
+int ap_level_prepare(ap_level* level, int passes) {
+    AP_REQUIRE(level && passes > 0, "level_prepare: bad arguments");
+    int slot;
+    return level->L.graph_for(passes, slot);
+}
+
 int ap_level_run(ap_level* level, int passes, int use_graph, void* stream) {
     AP_REQUIRE(level && passes >= 0, "level_run: bad arguments");
     Level& L = level->L;
     hipStream_t caller = (hipStream_t)stream;
+    int st = AP_OK, slot = -1;
+    if (use_graph && passes > 0 && (st = L.graph_for(passes, slot))) return st;
     AP_HIP(hipEventRecord(L.ev_in, caller));
     AP_HIP(hipStreamWaitEvent(L.stream, L.ev_in, 0));
     AP_HIP(hipEventRecord(L.t0, L.stream));
-    int st = AP_OK;
-    if (use_graph) {
-        if (!L.exec || L.graph_passes != passes) {
-            if (L.exec) {
-                (void)hipGraphExecDestroy(L.exec);
-                L.exec = nullptr;
-            }
-            if (L.graph) {
-                (void)hipGraphDestroy(L.graph);
-                L.graph = nullptr;
-            }
-            AP_HIP(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < passes && !st; k++) st = L.pass(L.stream);
-            if (!st) st = L.finalize(L.stream);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(L.stream, &g);
-            if (st) {
-                if (g) (void)hipGraphDestroy(g);
-                return st;
-            }
-            AP_HIP(e);
-            L.graph = g;
-            AP_HIP(hipGraphInstantiate(&L.exec, L.graph, nullptr, nullptr, 0));
-            L.graph_passes = passes;
-        }
-        AP_HIP(hipGraphLaunch(L.exec, L.stream));
+    if (slot >= 0) {
+        GraphEntry& G = L.graphs[slot];
+        AP_HIP(hipGraphLaunch(G.exec, L.stream));
+        L.last_ev = G.timing >= 0 ? &G.ev : nullptr;
+        L.last_ev_passes = G.passes;
     } else {
-        for (int k = 0; k < passes && !st; k++) st = L.pass(L.stream);
+        const bool timed = L.timing_cls >= 0 && passes > 0;
+        if (timed) {
+            if (L.last_ev == &L.direct_ev) L.last_ev = nullptr;
+            while (L.direct_ev.size() < 2 * (size_t)passes) {
+                hipEvent_t e = nullptr;
+                AP_HIP(hipEventCreate(&e));
+                L.direct_ev.push_back(e);
+            }
+            L.rec_ev = L.direct_ev.data();
+            L.rec_capture = false;
+            L.rec_err = hipSuccess;
+        }
+        for (int k = 0; k < passes && !st; k++) {
+            L.rec_pass = k;
+            st = L.pass(L.stream);
+        }
+        L.rec_ev = nullptr;
         if (!st) st = L.finalize(L.stream);
         if (st) return st;
+        if (L.rec_err != hipSuccess) {
+            set_error("level_run: timing event: %s", hipGetErrorString(L.rec_err));
+            return AP_ERR_HIP;
+        }
+        L.last_ev = timed ? &L.direct_ev : nullptr;
+        L.last_ev_passes = passes;
     }
     AP_HIP(hipEventRecord(L.t1, L.stream));
     AP_HIP(hipEventRecord(L.ev_out, L.stream));
     AP_HIP(hipStreamWaitEvent(caller, L.ev_out, 0));
+    return AP_OK;
+}
+
+int ap_level_time_class(ap_level* level, int cls) {
+    AP_REQUIRE(level && cls >= -1 && cls < AP_PROF_CLASSES, "level_time_class: bad arguments");
+    level->L.timing_cls = cls;
+    return AP_OK;
+}
+
+int ap_level_class_ms(ap_level* level, double* ms, int* passes) {
+    AP_REQUIRE(level && ms && passes, "level_class_ms: null argument");
+    Level& L = level->L;
+    AP_REQUIRE(L.last_ev && L.last_ev_passes > 0, "level_class_ms: the last run had no timed class");
+    const std::vector<hipEvent_t>& ev = *L.last_ev;
+    const int n = L.last_ev_passes;
+    AP_HIP(hipEventSynchronize(ev[2 * n - 1]));
+    double sum = 0;
+    for (int k = 0; k < n; k++) {
+        float t = 0;
+        AP_HIP(hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]));
+        sum += t;
+    }
+    *ms = sum / n;
+    *passes = n;
     return AP_OK;
 }
 
@@ -809,9 +1035,7 @@ int ap_level_profile(ap_level* level, int passes, double* ms, void* stream) {
         }
         for (int k = 0; k < AP_PROF_CLASSES; k++) {
             float t = 0;
-            int nxt = k + 1;
-            if (k == AP_PROF_POISSON) nxt = AP_PROF_GATHER;
-            (void)hipEventElapsedTime(&t, L.pev[k], L.pev[nxt]);
+            (void)hipEventElapsedTime(&t, L.pev[k], L.pev[k + 1]);
             L.prof_ms[k] += t;
         }
     }
@@ -842,6 +1066,14 @@ int ap_level_objective(ap_level* level, const float* pos, float* grad, double* s
     int st = L.objective(p, L.d.kp, false, nullptr, nullptr, (float2*)grad, (float2*)grad, s, nullptr, nullptr, 1);
     if (st) return st;
     AP_HIP(hipMemcpyAsync(stats, L.stats, 5 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    return AP_OK;
+}
+
+int ap_level_density(ap_level* level, float* rho, void* stream) {
+    AP_REQUIRE(level && rho, "level_density: null argument");
+    Level& L = level->L;
+    AP_HIP(hipMemcpyAsync(rho, L.rho, (size_t)L.d.nbx * L.d.nby * sizeof(float), hipMemcpyDeviceToDevice,
+                          (hipStream_t)stream));
     return AP_OK;
 }
 

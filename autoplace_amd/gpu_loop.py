@@ -98,12 +98,35 @@ class Engine(object):
         for mine, theirs in zip((self.u, self.v, self.g, self.best), self.user):
             theirs[self.perm] = mine
 
+    def prepare(self, passes):
+        """Capture the graph of `passes` passes now (outside any timed region)."""
+        _lib.check(_lib.lib().ap_level_prepare(self._h, passes), "level_prepare")
+
     def profile(self, passes):
         """Average ms per pass of every kernel class (direct launches with HIP events). -> dict."""
         ms = (ctypes.c_double * len(_lib.PROF_CLASSES))()
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         _lib.check(_lib.lib().ap_level_profile(self._h, passes, ms, stream), "level_profile")
         return dict(zip(_lib.PROF_CLASSES, list(ms)))
+
+    def time_class(self, name):
+        """Graphs captured from now on carry HIP events around the kernels of class `name` (None: no events)."""
+        cls = -1 if name is None else _lib.PROF_CLASSES.index(name)
+        _lib.check(_lib.lib().ap_level_time_class(self._h, cls), "level_time_class")
+
+    def class_ms(self):
+        """(average ms per pass of the timed class over the last graph run, passes of that run)."""
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        _lib.check(_lib.lib().ap_level_class_ms(self._h, ctypes.byref(ms), ctypes.byref(n)), "level_class_ms")
+        return ms.value, n.value
+
+    def density(self):
+        """Force-grid density [nbx, nby] of the last objective evaluated, in the engine's bin layout."""
+        d = self.desc
+        out = torch.empty((d.nbx, d.nby), dtype=torch.float32, device=self.v.device)
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(_lib.lib().ap_level_density(self._h, _p(out), stream), "level_density")
+        return out
 
     def last_ms(self):
         ms = ctypes.c_float()

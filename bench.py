@@ -1,19 +1,26 @@
 """Benchmark of the GP iteration on the 2M-cell synthetic design, one GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C] [--no-cpu] [--no-full]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C] [--no-cpu]
+                    [--save-snapshot F] [--snapshot F] [--copy-calibration]
 
 Workload: the 2,000,000-cell design (2,100,000 nets, ~8.2M pins, 16 fixed macros on 15 % of the die, generator
 seed 0, placer seed 0): force grid 2048 x 2048, overflow grid 512 x 512, default options.
 
 1. A whole two-level global placement to overflow <= 0.07 (untimed by this script's step clock; its own GP clock
    is reported): GP seconds, iterations, gradient evaluations, HPWL, overflow -- and it captures the state the
-   cell level's device loop starts from (cells just expanded from their clusters, overflow ~0.5).
-2. The timed region: from that captured state, W warm-up passes, then exactly K passes of the GP iteration on the
-   device (Nesterov step, WA, both density scatters, Poisson solve, gather, control; replayed from a HIP graph),
-   bracketed by synchronisations. value = K / time = gradient evaluations per second.
-3. Per-kernel times: the same K passes replayed with HIP events on the engine's stream; the dominant kernel's
-   roofline fraction uses its algorithmic bytes (every array element once, node data gathered once per pin).
+   cell level's device loop starts from (cells just expanded from their clusters).
+2. W warm-up passes from that state with direct launches and HIP events around every kernel class (the per-class
+   breakdown `kernel_ms`); the class with the largest single-kernel time is the roofline kernel.
+3. The timed region: exactly K passes of the GP iteration on the device (Nesterov step, WA, density binning of both
+   grids, Poisson solve, gather, control), one replay of a HIP graph captured beforehand, bracketed by
+   synchronisations. value = K / time = gradient evaluations per second. The graph carries a HIP event before and
+   after the roofline kernel in every pass (on the engine's stream): its average duration over the K passes gives
+   `roofline.achieved` = its algorithmic bytes per launch / that duration.
 4. cpu_baseline: the CPU oracle (numpy, one thread) running passes from the same captured state, bounded to ~25 s.
+
+--snapshot F starts from a state written by --save-snapshot (no full run, no CPU baseline): the form used under
+rocprofv3, together with --copy-calibration (three 256 MiB device copies first, the scale of the FETCH_SIZE /
+WRITE_SIZE counters).
 """
 import argparse
 import json
@@ -27,23 +34,30 @@ sys.path.insert(0, ROOT)
 PEAK_HBM = 8.0e12            # B/s
 PEAK_FP32_MFMA = 157.3e12    # flop/s
 
+# kernel classes made of one kernel (the others launch several: binning = count + scan + scatter, Poisson = 5,
+# control = reduce + control)
+SINGLE_KERNEL = {"nesterov": "eng_nesterov", "wa": "wa_wave_kernel", "density_tiles": "eng_bin_tiles",
+                 "gather": "eng_gather"}
+
 
 def algorithmic_bytes(cls, dims):
-    """Compulsory HBM bytes of one launch of a kernel class."""
+    """Compulsory HBM bytes of one launch of a kernel class: every array element read or written once, positions
+    gathered once per pin."""
     N, M, P, nets = dims["nodes"], dims["mov"], dims["pins"], dims["nets"]
     nb2, nj2 = dims["nb"] ** 2, dims["nj"] ** 2
     return {
         # u, v, g, lo, hi read; u', v' written (8 B each per node)
         "nesterov": 56 * N,
-        # per pin: cell 4 + offset 8 + position 8 + slot 4 read, gradient 8 written; net_start per net
-        "wa": 32 * P + 4 * nets,
-        # position 8 + size 8 + ratio 4 per node and grid (force: all nodes, overflow: cells); map written 8 B/bin
-        "density_map": 20 * N + 20 * M + 8 * (2 * nb2 + nj2),
-        # maps read 8 B/bin/plane, fixed 4, rho written 4 (force); overflow map 8 + fixed 4
-        "density_post": (2 * 8 + 4 + 4) * nb2 + 12 * nj2,
-        # per node: position, size, ratio, pins, area, g_prev read, g written; pin gradients 8 B/pin, cell_start;
-        # field planes 8 B/bin
-        "gather": (8 + 8 + 4 + 4 + 4 + 8 + 8) * N + 8 * P + 4 * M + 8 * nb2,
+        # per pin: cell 4 + offset 8 + position 8 read, gradient 8 written (the cell sum is the gather's);
+        # per net: its start 4
+        "wa": 28 * P + 4 * nets,
+        # per node and grid: position 8 + size 8 + ratio 4 read (force grid: all nodes; overflow grid: cells)
+        "density_bin": 20 * N + 20 * M,
+        # the maps: fixed 4 read and density 4 written per force bin, fixed 4 read per overflow bin
+        "density_tiles": 8 * nb2 + 4 * nj2,
+        # per node: position 8, size 8, ratio 4, pins 4, area 4, previous gradient 8 read, gradient 8 written;
+        # per cell: cell_start 4; pin gradients 8 per pin; the field 8 per bin
+        "gather": 44 * N + 4 * M + 8 * P + 8 * nb2,
     }.get(cls)
 
 
@@ -65,8 +79,8 @@ def full_run(data, log):
 
     def hook(objective, u, v, g, S, kp, c):
         snap.clear()
-        snap.update(objective=objective, u=u.clone(), v=v.clone(), g=g.clone(), S=S.clone(), kp=kp.clone(),
-                    c=c.clone())
+        snap.update(wl=objective.wirelength, den=objective.density, u=u.clone(), v=v.clone(), g=g.clone(),
+                    S=S.clone(), kp=kp.clone(), c=c.clone())
 
     args = default_args()
     _, m = place(data, args, "cuda:0", log=log, opts={"hook": hook})
@@ -79,35 +93,74 @@ def full_run(data, log):
     return out, snap
 
 
+def load_snapshot(data, path):
+    """State written by --save-snapshot, with the cell level's operators rebuilt as the full run builds them."""
+    import numpy as np
+    import torch
+
+    from autoplace_amd.problem import build_problem, default_args
+    from autoplace_amd.run_placement import build_operators
+
+    z = np.load(path)
+    args = default_args()
+    prob = build_problem(data, args)
+    wl, den = build_operators(prob, args, torch.device("cuda:0"))
+    dt = {"S": torch.float64, "c": torch.float64}
+    snap = {k: torch.tensor(z[k], dtype=dt.get(k, torch.float32), device="cuda:0") for k in
+            ("u", "v", "g", "S", "kp", "c")}
+    snap.update(wl=wl, den=den)
+    return snap
+
+
+def copy_calibration():
+    """Three plain 256 MiB device-to-device copies: the FETCH_SIZE / WRITE_SIZE reading of known traffic."""
+    import torch
+    src = torch.ones(64 * 1024 * 1024, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def gpu_passes(snap, steps, warmup):
     import torch
 
     from autoplace_amd import gpu_loop
 
-    obj = snap["objective"]
-    wl, den = obj.wirelength, obj.density
+    wl, den = snap["wl"], snap["den"]
     S, kp, c = snap["S"].clone(), snap["kp"].clone(), snap["c"].clone()
     c[9] = 1e12                                      # inner_iter: no max_iter stop inside the timed window
     u, v, g = snap["u"].clone(), snap["v"].clone(), snap["g"].clone()
     eng = gpu_loop.Engine(wl, den, u, v, g, v.clone(), S, kp, c)
-    chunk = 16 if steps % 16 == 0 and warmup % 16 == 0 else 1
-    for _ in range(warmup // chunk):
-        eng.run(chunk, True)
+    prof = eng.profile(max(warmup, 1))               # the warm-up passes, per kernel class
+    dom = max(SINGLE_KERNEL, key=lambda k: prof[k])
+    eng.time_class(dom)
+    try:
+        eng.prepare(steps)                           # graph with event-record nodes around the roofline kernel
+        in_graph = True
+    except RuntimeError as e:
+        print("event-record nodes in the graph failed (%s): the roofline kernel is timed over K direct passes after "
+              "the timed region" % e, file=sys.stderr, flush=True)
+        eng.time_class(None)
+        eng.prepare(steps)
+        in_graph = False
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps // chunk):
-        eng.run(chunk, True)
+    eng.run(steps, True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    prof = eng.profile(steps)
-    torch.cuda.synchronize()
+    if not in_graph:
+        eng.time_class(dom)
+        eng.run(steps, False)
+    dom_ms, n = eng.class_ms()
+    assert n == steps
     stop = int(S[gpu_loop.K["stop"]].item())
     d = den
     dims = {"nodes": d.num_nodes, "mov": d.num_mov, "pins": int(wl.opt["cell32"].numel()), "nets": wl.num_opt_nets,
             "nb": d.nbx, "nj": d.judge.nbx if d.judge is not None else 0}
     method = d.plan.method
     eng.close()
-    return elapsed, prof, dims, method, stop
+    return elapsed, prof, dom, dom_ms, dims, method, stop, in_graph
 
 
 def cpu_baseline(data, snap, budget_s=25.0):
@@ -150,7 +203,9 @@ def main():
     p.add_argument("--warmup", type=int, default=16)
     p.add_argument("--config", default="C")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--save-snapshot", default="", help="write the captured state (npz) for kernel studies")
+    p.add_argument("--save-snapshot", default="", help="write the captured state (npz)")
+    p.add_argument("--snapshot", default="", help="start from a saved state: no full run, no CPU baseline")
+    p.add_argument("--copy-calibration", action="store_true", help="three 256 MiB device copies first")
     a = p.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the GP iteration of one design runs on one device here "
@@ -161,34 +216,37 @@ def main():
     data = make_design(a.config)
     log("design %s: %d cells, %d nets, %d pins (%.1fs)" % (a.config, data.num_mov, data.num_nets, data.num_pins,
                                                            time.time() - t0))
-    run, snap = full_run(data, log)
-    log("full run: %s" % json.dumps(run))
+    if a.copy_calibration:
+        copy_calibration()
+    if a.snapshot:
+        run, snap = None, load_snapshot(data, a.snapshot)
+    else:
+        run, snap = full_run(data, log)
+        log("full run: %s" % json.dumps(run))
     if a.save_snapshot:
         import numpy as np
         np.savez(a.save_snapshot, **{k: snap[k].cpu().numpy() for k in ("u", "v", "g", "S", "kp", "c")})
-    elapsed, prof, dims, method, stop = gpu_passes(snap, a.steps, a.warmup)
-    log("per-pass kernel ms: " + ", ".join("%s %.3f" % kv for kv in prof.items()))
-    dom = max(prof, key=prof.get)
-    if dom == "poisson" and method == "gemm":
-        roof = {"kernel": "poisson (fp32 MFMA GEMM DCT)", "bound": "mfma",
-                "achieved": 12.0 * dims["nb"] ** 3 / (prof[dom] * 1e-3) / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
-                "unit": "TFLOP/s", "traffic": None}
-    else:
-        by = algorithmic_bytes(dom, dims)
-        roof = {"kernel": dom, "bound": "hbm", "achieved": by / (prof[dom] * 1e-3) / 1e9 if by else None,
-                "peak": PEAK_HBM / 1e9, "unit": "GB/s", "traffic": None}
-    if roof["achieved"] is not None:
-        roof["frac"] = roof["achieved"] / roof["peak"]
-    out = {"metric": "grad evals/s @2M cells (GP iteration, 1 GPU)", "value": a.steps / elapsed,
-           "unit": "grad evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": elapsed / a.steps * 1e3,
+    elapsed, prof, dom, dom_ms, dims, method, stop, in_graph = gpu_passes(snap, a.steps, a.warmup)
+    log("per-pass kernel ms (warm-up, direct launches): " + ", ".join("%s %.3f" % kv for kv in prof.items()))
+    log("roofline kernel %s (%s): %.4f ms per launch over the timed passes" % (dom, SINGLE_KERNEL[dom], dom_ms))
+    by = algorithmic_bytes(dom, dims)
+    roof = {"kernel": SINGLE_KERNEL[dom], "bound": "hbm", "achieved": by / (dom_ms * 1e-3) / 1e9,
+            "peak": PEAK_HBM / 1e9, "unit": "GB/s", "traffic": None, "algorithmic_bytes": by, "launch_ms": dom_ms,
+            "events": "event-record nodes in the timed graph" if in_graph else "direct launches after the timed region"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    out = {"metric": "grad evals/s @2M cells", "value": a.steps / elapsed,
+           "unit": "grad evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": elapsed / a.steps * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (generator seed 0), state captured from a real run",
            "config": {"workload": "synthetic %s: %d cells, %d nets, %d pins, force grid %d, overflow grid %d, "
                                   "poisson %s" % (a.config, data.num_mov, data.num_nets, data.num_pins, dims["nb"],
                                                   dims["nj"] or dims["nb"], method),
                       "parallelism": "single GPU"},
-           "roofline": roof, "kernel_ms": prof, "stopped_in_window": stop, "full_run": run}
-    if not a.no_cpu:
+           "roofline": roof, "kernel_ms": prof, "stopped_in_window": stop}
+    if run is not None:
+        out["full_run"] = run
+    if not a.no_cpu and not a.snapshot:
         out["cpu_baseline"] = cpu_baseline(data, snap)
     print(json.dumps(out))
 

@@ -147,14 +147,17 @@ typedef struct ap_level_desc {
 AP_API int ap_level_create(const ap_level_desc* desc, ap_level** level);
 AP_API int ap_level_destroy(ap_level* level);
 /* Run `passes` passes (each: Nesterov step, objective, control, select). With use_graph the passes are replayed from
- * a HIP graph captured on the first call. On return (stream-ordered) u / v / g / best / S / kp hold the state after
- * the passes, as the reference's loop leaves them. */
+ * a HIP graph captured on the first call with that pass count (the last 4 captures are kept). On return
+ * (stream-ordered) u / v / g / best / S / kp hold the state after the passes, as the reference's loop leaves them. */
 AP_API int ap_level_run(ap_level* level, int passes, int use_graph, void* stream);
-/* Kernel classes of a pass, in launch order, for ap_level_profile. */
+/* Capture (and instantiate) the graph of `passes` passes now, so that the first ap_level_run with that count only
+ * launches it. Optional. */
+AP_API int ap_level_prepare(ap_level* level, int passes);
+/* Kernel classes of a pass, in launch order, for ap_level_profile and ap_level_time_class. */
 #define AP_PROF_NESTEROV 0
 #define AP_PROF_WA 1
-#define AP_PROF_MAP 2
-#define AP_PROF_POST 3
+#define AP_PROF_MAP 2      /* density: tile binning (count, scan, scatter) */
+#define AP_PROF_POST 3     /* density: per-tile sums, density and overflow */
 #define AP_PROF_POISSON 4
 #define AP_PROF_GATHER 5
 #define AP_PROF_CONTROL 6
@@ -162,12 +165,19 @@ AP_API int ap_level_run(ap_level* level, int passes, int use_graph, void* stream
 /* Run `passes` passes with direct launches and HIP events around every kernel class; ms[AP_PROF_CLASSES] receives
  * the average milliseconds per pass of each class. Blocks the calling thread until the passes are done. */
 AP_API int ap_level_profile(ap_level* level, int passes, double* ms, void* stream);
+/* Graphs captured from now on record a HIP event before and after the kernels of class cls in every pass (-1: none).
+ * After a graph run, ap_level_class_ms gives the class's average ms per pass over that run (blocks until it ends). */
+AP_API int ap_level_time_class(ap_level* level, int cls);
+AP_API int ap_level_class_ms(ap_level* level, double* ms, int* passes);
 /* Timing of the last ap_level_run (ms between its first and last kernel), for the caller's instrumentation. */
 AP_API int ap_level_last_ms(ap_level* level, float* ms);
 
 /* One gradient evaluation of the objective at pos (Objective.on_gpu, src/calculator.py:37-43) with the parameters
- * in kp (gamma, weight, precond): grad f32[N][2]; stats f64[4] = (WA, HPWL, overflow, 0). */
+ * in kp (gamma, weight, precond): grad f32[N][2]; stats f64[5] = (WA, HPWL, overflow, 0, 0). */
 AP_API int ap_level_objective(ap_level* level, const float* pos, float* grad, double* stats, void* stream);
+/* The force grid's total density f32[nbx][nby] (unmirrored) of the last objective evaluated (pass or
+ * ap_level_objective): the top-left quadrant of the reference's density_map result. */
+AP_API int ap_level_density(ap_level* level, float* rho, void* stream);
 
 #ifdef __cplusplus
 }

@@ -70,7 +70,68 @@ def test_engine_objective_vs_oracle(small_design):
     eng.close()
 
 
-def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm"):
+@pytest.mark.parametrize("case", ["small", "grid1024"])
+def test_engine_density_bit_exact(small_design, case):
+    """The engine's tile-binned density map equals the oracle's fixed-point map bit for bit, and its overflow the
+    oracle's: on the small design (16 x 16-bin tiles) and on a 1024 x 1024 force grid (32 x 32-bin tiles) where
+    nodes span several tiles and the biggest go through the big-node path."""
+    from autoplace_amd import synth
+    from autoplace_amd.database import PlaceData
+    from autoplace_amd.problem import build_problem, default_args
+    from autoplace_amd.run_placement import build_operators
+    from oracle import density as oden
+    if case == "small":
+        data = small_design[2]
+        args = default_args()
+    else:
+        lef, d = synth.generate(6000, num_macros=3, macro_frac=0.1, seed=11, io_pins=8, name="g1024")
+        data = PlaceData(lef, d)
+        args = default_args(force_num_bin=1024)
+    prob = build_problem(data, args)
+    if case == "grid1024":
+        prob.node_size = prob.node_size.copy()
+        prob.node_size[:40] *= 6.0                   # 40 cells over 2 tiles wide: the big-node path
+    wl, den = build_operators(prob, args, DEV, "fft")
+    rng = np.random.default_rng(5)
+    lo, hi = den.pos_lo.cpu().numpy(), den.pos_hi.cpu().numpy()
+    pos = (lo + rng.random(lo.shape) * (hi - lo)).astype(np.float32)
+    pos[: len(pos) // 3] = (lo[: len(pos) // 3] + 0.5 * (hi - lo)[: len(pos) // 3] * rng.random((len(pos) // 3, 2)))
+    S = np.zeros(21)
+    kp = np.array([30.0, 1e-3, 1.0, 0.0, 0.0], np.float32)
+    c = np.zeros(13)
+    eng, _ = _engine(wl, den, pos, S, kp, c, np.zeros_like(pos), reorder=True, poisson="fft")
+    grad = torch.empty_like(eng.v)
+    stats = torch.zeros(8, dtype=torch.float64, device=DEV)
+    eng.objective(eng.v, grad, stats)
+    rho_dev = eng.density().cpu().numpy()
+    ovf_dev = float(stats[2])
+    eng.close()
+    smooth, ratio = den.smooth["size"].cpu().numpy(), den.smooth["ratio"].cpu().numpy()
+    bx, by = float(den.bin_np[0]), float(den.bin_np[1])
+    m0, m1 = oden.density_maps(pos, smooth, ratio, data.num_mov, bx, by, den.nbx, den.nby)
+    rho, over = oden.density_post(m0, m1, den.fixed_map.cpu().numpy(), den.target, den.nbx, den.nby)
+    if den.judge is not None:
+        j = den.judge
+        js, jr = j.smooth["size"].cpu().numpy(), j.smooth["ratio"].cpu().numpy()
+        q0, q1 = oden.density_maps(pos[:data.num_mov], js, jr, data.num_mov, float(j.bin_np[0]), float(j.bin_np[1]),
+                                   j.nbx, j.nby)
+        _, over = oden.density_post(q0, q1, j.fixed_map.cpu().numpy(), j.target, j.nbx, j.nby)
+        # the engine divides by the force layer's movable area (equal to the overflow layer's whenever the cells'
+        # sizes are the same in both, as in every run; the enlarged cells above make them differ here)
+        ovf = oden.overflow(over, j.bin_area, den.mov_area)
+    else:
+        ovf = oden.overflow(over, den.bin_area, den.mov_area)
+    if case == "grid1024":
+        span = smooth / np.array([bx, by], np.float32)
+        print("1024 grid: nodes over 33 bins: %d of %d" % (int((span.max(1) > 33).sum()), len(span)))
+        assert (span.max(1) > 33).sum() >= 40
+    print("engine density (%s, %d x %d): max |diff| %.3g, overflow rel %.2e"
+          % (case, den.nbx, den.nby, float(np.abs(rho_dev - rho).max()), rel_err(ovf_dev, ovf)))
+    assert np.array_equal(rho_dev, rho)
+    assert rel_err(ovf_dev, ovf) < 1e-12
+
+
+def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cls=None):
     args, prob, wl, den, pos, S, kp, c = _setup(data)
     g0 = np.zeros_like(pos)
     eng, bufs = _engine(wl, den, pos, S, kp, c, g0, reorder=False, poisson=poisson)
@@ -81,13 +142,16 @@ def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm"):
     eng.close()
     g0 = grad.cpu().numpy()
     eng, bufs = _engine(wl, den, pos, S, kp, c, g0, reorder=reorder, poisson=poisson)
+    if time_cls:
+        eng.time_class(time_cls)
     for _ in range(calls):
         eng.run(passes, use_graph)
+    timing = eng.class_ms() if time_cls else None
     eng.finish()
     torch.cuda.synchronize()
     out = [b.cpu().numpy().copy() for b in bufs]
     eng.close()
-    return out, (args, prob, pos, S, kp, c, grad.cpu().numpy())
+    return out, (args, prob, pos, S, kp, c, grad.cpu().numpy(), timing)
 
 
 def test_engine_graph_equals_direct_and_is_deterministic(small_design):
@@ -107,12 +171,27 @@ def test_engine_graph_equals_direct_and_is_deterministic(small_design):
     assert a[4][K["loop_it"]] == 40 + 32 and a[4][K["evals"]] == 32
 
 
+def test_engine_timing_events_change_nothing(small_design):
+    """Event-record nodes around a kernel class (captured graph) and event records (direct launches) leave the results
+    bit for bit as they are, and time the class."""
+    _, _, data = small_design
+    ref, _ = _run(data, 16, True)
+    for use_graph in (True, False):
+        for cls in ("density_tiles", "wa"):
+            out, (_, _, _, _, _, _, _, timing) = _run(data, 16, use_graph, time_cls=cls)
+            ms, n = timing
+            print("timed %s (%s): %.4f ms per pass over %d passes" % (cls, "graph" if use_graph else "direct", ms, n))
+            assert n == 16 and 0 < ms < 50
+            for x, y in zip(out, ref):
+                assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("poisson", ["gemm", "fft"])
 def test_engine_trajectory_vs_oracle(small_design, poisson):
     """16 passes of the engine against 16 oracle passes from the same state: the control decisions agree and the
     points stay within FP32 drift of each other."""
     _, _, data = small_design
-    (u, v, g, best, S_dev, kp_dev, _), (args, prob, pos, S, kp, c, g0) = _run(data, 16, True, poisson=poisson)
+    (u, v, g, best, S_dev, kp_dev, _), (args, prob, pos, S, kp, c, g0, _) = _run(data, 16, True, poisson=poisson)
     lvl = _oracle_level(prob, args, data)
     S = list(S)
     kp = np.array(kp, np.float32)
