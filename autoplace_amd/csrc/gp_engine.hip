@@ -137,9 +137,10 @@ __device__ __forceinline__ bool small_node(const Span& sx, const Span& sy, int T
 
 __global__ __launch_bounds__(EB) void eng_bin_count(Pick pos_p, BinJob f, BinJob j) {
     extern __shared__ int hist[];   // planes x tiles of the grid
-    const bool judge = (int)blockIdx.x >= f.blocks;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const bool judge = blk >= f.blocks;
     const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
-    const int base = ((int)blockIdx.x - (judge ? f.blocks : 0)) * BIN_NODES;
+    const int base = (blk - (judge ? f.blocks : 0)) * BIN_NODES;
     const int nbuck = J.planes * J.tiles;
     for (int t = threadIdx.x; t < nbuck; t += EB) hist[t] = 0;
     __syncthreads();
@@ -218,9 +219,10 @@ __global__ __launch_bounds__(SCAN_T) void eng_bin_scan(BinJob f, BinJob j) {
 }
 
 __global__ __launch_bounds__(EB) void eng_bin_scatter(Pick pos_p, BinJob f, BinJob j) {
-    const bool judge = (int)blockIdx.x >= f.blocks;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const bool judge = blk >= f.blocks;
     const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
-    const int base = ((int)blockIdx.x - (judge ? f.blocks : 0)) * BIN_NODES;
+    const int base = (blk - (judge ? f.blocks : 0)) * BIN_NODES;
     const float2* pos = pos_p.get(true);
 #pragma unroll 2
     for (int q = 0; q < BIN_PER_LANE; q++) {
@@ -367,7 +369,7 @@ struct GatherArgs {
 
 __global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part) {
     __shared__ double red[16];
-    const int i = blockIdx.x * EB + threadIdx.x;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * EB + threadIdx.x;
     double t2[1] = {0.0};
     if (i < a.num_nodes) {
         int cur = a.flags ? a.flags[0] : 0;
@@ -719,7 +721,7 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipMalloc(&fx, nb * sizeof(float)));
     AP_HIP(hipMalloc(&fy, nb * sizeof(float)));
     nblk_n = grid_of(N, EB);
-    if ((st = wa_plan.create(d.net_start, d.num_nets))) return st;
+    if ((st = wa_plan.create(d.net_start, d.num_nets, d.pin_cell, d.pin_off, pin_slot, d.num_pins))) return st;
     nblk_wa = wa_plan.partial_count();
     if ((st = bin_job(fj, d.size, d.ratio, d.num_nodes, d.num_mov, d.bx, d.by, d.nbx, d.nby, 2, d.fixed, rho,
                       d.has_judge ? 0 : 1)))

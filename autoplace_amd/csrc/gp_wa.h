@@ -1,4 +1,4 @@
-// The engine's WA plan: nets packed whole into 64-lane wavefronts (gp_wa.hip).
+// The engine's WA plan: nets in aligned lane groups of 2^k lanes, one degree class per wavefront (gp_wa.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -8,13 +8,24 @@ namespace ap {
 
 struct WaPlan {
     int num_waves = 0, num_big = 0;
-    int4* d_waves = nullptr;   // per wavefront: first pin, pins, first net, nets
-    int* d_big = nullptr;      // nets of more than 64 pins
-    int create(const int* d_net_start, int num_nets);
+    unsigned char* d_lg = nullptr;   // per wavefront: log2 of its lanes per net
+    int4* d_lanes = nullptr;         // per lane, in wavefront order: pin cell (-1: no pin), gradient slot (-1: none),
+                                     // pin offset x, y (float bits)
+    int* d_big = nullptr;            // nets of more than 64 pins
+    // pin_slot: where pin p's gradient goes (null: p)
+    int create(const int* d_net_start, int num_nets, const int* d_pin_cell, const float* d_pin_off,
+               const int* d_pin_slot, int num_pins);
     void destroy();
     int wave_blocks() const { return (num_waves + 3) / 4; }
     int partial_count() const { return wave_blocks() + num_big; }   // (WA, HPWL) pairs written per launch
     int launch(const WaArgs& a, double* partial, hipStream_t s) const;
 };
+
+// Work item of workgroup b among nb when consecutive workgroups land on the 8 XCDs in turn: XCD x gets a contiguous
+// range of work items, so the workgroups sharing an XCD's L2 work on neighbouring data.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
 
 }  // namespace ap

@@ -1,12 +1,15 @@
-// WA wirelength of the engine: one lane per pin, nets packed whole into 64-lane wavefronts.
+// WA wirelength of the engine: one lane per pin, every net in an aligned group of 2^k lanes.
 //
 // The drop-in kernel (gp_kernels.hip) gives each net one lane that loops over the net's pins: the lanes of a
 // wavefront then wait for its longest net (degrees 2..50 here, up to --ignore_net_degree), and the per-net phase is
-// VALU-bound. Here the nets of a level are packed, once, into wavefronts of at most 64 pins (no net straddles two
-// wavefronts; nets of more than 64 pins go to a workgroup-wide path). Every lane holds one pin; each net is a
-// segment of lanes, and max / min / the four exponential sums per axis are segmented tree reductions over the
-// lanes (shuffles), with the results broadcast from the segment's first lane. The sums are taken in a fixed tree
-// order, so the results are deterministic (and within FP32 rounding of the sequential sums).
+// VALU-bound. Here the nets of a level are sorted once, on the host, into degree classes: a net of d <= 64 pins gets
+// a group of G = 2^k >= d lanes (G >= 2), aligned to G inside a wavefront, and each wavefront holds 64 / G nets of one
+// class, in net order. Every lane holds one pin (or nothing, past the net's degree); max / min and the four
+// exponential sums per axis are butterfly reductions over the group whose steps inside a row of 16 lanes are DPP
+// lane permutations (quad_perm, row_half_mirror, row_mirror: plain VALU operands, no LDS crossbar), and the steps
+// across rows (G = 32, 64) are shuffles. After a butterfly every lane of the group holds the same bits (each step
+// adds the same two partial results, in either order), so no broadcast is needed and the results are
+// deterministic. Nets of more than 64 pins take a workgroup-wide path.
 #include <vector>
 
 #include "gp_common.h"
@@ -20,107 +23,100 @@ struct WaveArgs {
     const float2* pos;
     const float2* pos_other;
     const int* flag;
-    const int* pin_cell;
-    const float2* pin_off;
-    const int* net_start;
-    const int4* waves;       // pin_begin, pin_count, net_begin, net_count
+    const unsigned char* lg;   // per wavefront: log2 of its lanes per net
+    const int4* lanes;         // per lane: cell (-1: none), gradient slot, offset x, y
     int num_waves;
     int num_mov;
     const float* gamma;
-    const int* pin_slot;
     float2* pin_grad;
-    double* partial;         // per workgroup (WA, HPWL)
+    double* partial;           // per workgroup (WA, HPWL)
 };
 
-__device__ __forceinline__ float seg_max(float v, int lane, int end) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const float o = __shfl_down(v, off, 64);
-        if (lane + off < end) v = fmaxf(v, o);
-    }
+// DPP lane permutations (gfx9 encodings of the dpp_ctrl operand)
+constexpr int DPP_QUAD_XOR1 = 0xB1;        // quad_perm [1, 0, 3, 2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;        // quad_perm [2, 3, 0, 1]
+constexpr int DPP_ROW_HALF_MIRROR = 0x141; // lane i of each 8 <- lane 7 - i
+constexpr int DPP_ROW_MIRROR = 0x140;      // lane i of each 16 <- lane 15 - i
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+struct MaxOp {
+    __device__ __forceinline__ float operator()(float a, float b) const { return fmaxf(a, b); }
+};
+struct MinOp {
+    __device__ __forceinline__ float operator()(float a, float b) const { return fminf(a, b); }
+};
+struct AddOp {
+    __device__ __forceinline__ float operator()(float a, float b) const { return a + b; }
+};
+
+// reduction over the aligned group of 2^LG lanes, the result in every lane of the group. After the two quad steps
+// every lane of a quad holds the quad's value, so the mirror partner (in the other quad of the half row, the other
+// half of the row) holds the other half's value.
+template <int LG, typename Op>
+__device__ __forceinline__ float group_reduce(float v, Op op) {
+    v = op(v, dpp<DPP_QUAD_XOR1>(v));
+    if (LG >= 2) v = op(v, dpp<DPP_QUAD_XOR2>(v));
+    if (LG >= 3) v = op(v, dpp<DPP_ROW_HALF_MIRROR>(v));
+    if (LG >= 4) v = op(v, dpp<DPP_ROW_MIRROR>(v));
+    if (LG >= 5) v = op(v, __shfl_xor(v, 16, 64));
+    if (LG >= 6) v = op(v, __shfl_xor(v, 32, 64));
     return v;
 }
-__device__ __forceinline__ float seg_min(float v, int lane, int end) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const float o = __shfl_down(v, off, 64);
-        if (lane + off < end) v = fminf(v, o);
+
+template <int LG>
+__device__ __forceinline__ void wa_group(const WaveArgs& a, const float2* pos, const int4& ln, int lane,
+                                         double (&acc)[2]) {
+    constexpr int G = 1 << LG;
+    const int sub = lane & (G - 1);
+    const int c = ln.x, slot = ln.y;
+    const bool live = c >= 0;
+    float2 x = make_float2(__int_as_float(ln.z), __int_as_float(ln.w));
+    if (live && c < a.num_mov) {
+        const float2 q = pos[c];
+        x.x += q.x;
+        x.y += q.y;
     }
-    return v;
-}
-__device__ __forceinline__ float seg_sum(float v, int lane, int end) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const float o = __shfl_down(v, off, 64);
-        if (lane + off < end) v += o;
+    const float ig = 1.0f / *a.gamma;
+    const float NEG = -3.402823466e38f, POS = 3.402823466e38f;
+    const float hx = group_reduce<LG>(live ? x.x : NEG, MaxOp()), hy = group_reduce<LG>(live ? x.y : NEG, MaxOp());
+    const float lx = group_reduce<LG>(live ? x.x : POS, MinOp()), ly = group_reduce<LG>(live ? x.y : POS, MinOp());
+    const float epx = live ? expf((x.x - hx) * ig) : 0.f, epy = live ? expf((x.y - hy) * ig) : 0.f;
+    const float enx = live ? expf((lx - x.x) * ig) : 0.f, eny = live ? expf((ly - x.y) * ig) : 0.f;
+    const float spx = group_reduce<LG>(epx, AddOp()), spy = group_reduce<LG>(epy, AddOp());
+    const float snx = group_reduce<LG>(enx, AddOp()), sny = group_reduce<LG>(eny, AddOp());
+    const float xpx = group_reduce<LG>(x.x * epx, AddOp()) / spx, xpy = group_reduce<LG>(x.y * epy, AddOp()) / spy;
+    const float xnx = group_reduce<LG>(x.x * enx, AddOp()) / snx, xny = group_reduce<LG>(x.y * eny, AddOp()) / sny;
+    if (live) {
+        float2 gr;
+        gr.x = epx * (1 + (x.x - xpx) * ig) / spx - enx * (1 - (x.x - xnx) * ig) / snx;
+        gr.y = epy * (1 + (x.y - xpy) * ig) / spy - eny * (1 - (x.y - xny) * ig) / sny;
+        if (slot >= 0) a.pin_grad[slot] = gr;
+        if (sub == 0) {
+            acc[0] = (double)((xpx - xnx) + (xpy - xny));
+            acc[1] = (double)((hx - lx) + (hy - ly));
+        }
     }
-    return v;
 }
 
 __global__ __launch_bounds__(256) void wa_wave_kernel(WaveArgs a) {
     __shared__ double red[16 * 2];
     const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     const float2* pos = a.flag ? (*a.flag == 0 ? a.pos_other : a.pos) : a.pos;
     double acc[2] = {0.0, 0.0};
     if (w < a.num_waves) {
-        const int4 wv = a.waves[w];
-        const int pin_begin = wv.x, npin = wv.y, net_begin = wv.z, nnet = wv.w;
-        // segment heads: lane j (< nnet) marks the first pin of net j
-        unsigned long long mask = 0ull;
-        {
-            int s = lane < nnet ? a.net_start[net_begin + lane] - pin_begin : 64;
-            const bool head = s < npin;             // an empty last net has no pin to mark
-            // OR of (1 << s) over the head lanes
-            unsigned long long bit = head ? (1ull << s) : 0ull;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const unsigned lo = __shfl_xor((unsigned)bit, off, 64), hi = __shfl_xor((unsigned)(bit >> 32), off, 64);
-                bit |= ((unsigned long long)hi << 32) | lo;
-            }
-            mask = bit;
-        }
-        const bool live = lane < npin;
-        const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-        const int seg_start = 63 - __clzll(mask & upto);
-        const unsigned long long above = mask & ~upto;
-        const int seg_end = above ? __ffsll(above) - 1 : npin;
-        const int p = pin_begin + lane;
-        float2 x = make_float2(0.f, 0.f);
-        if (live) {
-            const int c = a.pin_cell[p];
-            x = a.pin_off[p];
-            if (c < a.num_mov) {
-                const float2 q = pos[c];
-                x.x += q.x;
-                x.y += q.y;
-            }
-        }
-        const float ig = 1.0f / *a.gamma;
-        const float hx = __shfl(seg_max(x.x, lane, seg_end), seg_start, 64);
-        const float hy = __shfl(seg_max(x.y, lane, seg_end), seg_start, 64);
-        const float lx = __shfl(seg_min(x.x, lane, seg_end), seg_start, 64);
-        const float ly = __shfl(seg_min(x.y, lane, seg_end), seg_start, 64);
-        const float epx = live ? expf((x.x - hx) * ig) : 0.f, epy = live ? expf((x.y - hy) * ig) : 0.f;
-        const float enx = live ? expf((lx - x.x) * ig) : 0.f, eny = live ? expf((ly - x.y) * ig) : 0.f;
-        const float spx = __shfl(seg_sum(epx, lane, seg_end), seg_start, 64);
-        const float spy = __shfl(seg_sum(epy, lane, seg_end), seg_start, 64);
-        const float snx = __shfl(seg_sum(enx, lane, seg_end), seg_start, 64);
-        const float sny = __shfl(seg_sum(eny, lane, seg_end), seg_start, 64);
-        const float xpx = __shfl(seg_sum(x.x * epx, lane, seg_end), seg_start, 64) / spx;
-        const float xpy = __shfl(seg_sum(x.y * epy, lane, seg_end), seg_start, 64) / spy;
-        const float xnx = __shfl(seg_sum(x.x * enx, lane, seg_end), seg_start, 64) / snx;
-        const float xny = __shfl(seg_sum(x.y * eny, lane, seg_end), seg_start, 64) / sny;
-        if (live) {
-            float2 gr;
-            gr.x = epx * (1 + (x.x - xpx) * ig) / spx - enx * (1 - (x.x - xnx) * ig) / snx;
-            gr.y = epy * (1 + (x.y - xpy) * ig) / spy - eny * (1 - (x.y - xny) * ig) / sny;
-            const int slot = a.pin_slot ? a.pin_slot[p] : p;
-            if (slot >= 0) a.pin_grad[slot] = gr;
-            if (lane == seg_start) {
-                acc[0] = (double)((xpx - xnx) + (xpy - xny));
-                acc[1] = (double)((hx - lx) + (hy - ly));
-            }
+        const int4 ln = a.lanes[(long long)w * 64 + lane];
+        switch (a.lg[w]) {   // uniform over the wavefront
+            case 1: wa_group<1>(a, pos, ln, lane, acc); break;
+            case 2: wa_group<2>(a, pos, ln, lane, acc); break;
+            case 3: wa_group<3>(a, pos, ln, lane, acc); break;
+            case 4: wa_group<4>(a, pos, ln, lane, acc); break;
+            case 5: wa_group<5>(a, pos, ln, lane, acc); break;
+            default: wa_group<6>(a, pos, ln, lane, acc); break;
         }
     }
     block_sum<2>(acc, red);
@@ -144,36 +140,72 @@ __global__ __launch_bounds__(WA_BLOCK) void wa_big_nets_kernel(WaArgs a, const i
     }
 }
 
-int WaPlan::create(const int* d_net_start, int num_nets) {
+// set-up: the lane-ordered copy of the pins' cell, gradient slot and offset
+__global__ __launch_bounds__(256) void wa_lanes_fill(const int* lane_pin, long long n, const int* pin_cell,
+                                                     const float2* pin_off, const int* pin_slot, int4* lanes) {
+    const long long l = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (l >= n) return;
+    const int p = lane_pin[l];
+    int4 v = make_int4(-1, -1, 0, 0);
+    if (p >= 0) {
+        const float2 o = pin_off[p];
+        v = make_int4(pin_cell[p], pin_slot ? pin_slot[p] : p, __float_as_int(o.x), __float_as_int(o.y));
+    }
+    lanes[l] = v;
+}
+
+// Nets in net order; each degree class keeps one open wavefront, closed when its 64 / G groups are taken, so the
+// wavefronts stay close to net order (and to the cells' order) within every class. The pins' data is then copied
+// into lane order once, so that a wavefront reads one contiguous 1 KB block.
+int WaPlan::create(const int* d_net_start, int num_nets, const int* d_pin_cell, const float* d_pin_off,
+                   const int* d_pin_slot, int num_pins) {
     std::vector<int> ns(num_nets + 1);
     AP_HIP(hipMemcpy(ns.data(), d_net_start, (num_nets + 1) * sizeof(int), hipMemcpyDeviceToHost));
-    std::vector<int4> waves;
-    std::vector<int> big;
-    int cur_pins = 0, cur_begin = -1;
-    auto flush = [&](int next_net) {
-        if (cur_begin >= 0) waves.push_back(make_int4(ns[cur_begin], cur_pins, cur_begin, next_net - cur_begin));
-        cur_begin = -1;
-        cur_pins = 0;
+    AP_REQUIRE(ns[0] >= 0 && ns[num_nets] <= num_pins, "wa plan: net_start out of the pin range");
+    std::vector<unsigned char> lgs;
+    std::vector<int> big, lane_pin;
+    std::vector<int> open[7];
+    auto close = [&](int lg) {
+        if (open[lg].empty()) return;
+        const int G = 1 << lg;
+        lgs.push_back((unsigned char)lg);
+        const size_t base = lane_pin.size();
+        lane_pin.resize(base + 64, -1);
+        for (size_t k = 0; k < open[lg].size(); k++) {
+            const int net = open[lg][k];
+            for (int q = ns[net]; q < ns[net + 1]; q++) lane_pin[base + k * G + (q - ns[net])] = q;
+        }
+        open[lg].clear();
     };
     for (int i = 0; i < num_nets; i++) {
         const int deg = ns[i + 1] - ns[i];
         if (deg > 64) {
-            flush(i);
             big.push_back(i);
             continue;
         }
-        if (cur_begin < 0 || cur_pins + deg > 64 || i - cur_begin >= 64) {
-            flush(i);
-            cur_begin = i;
-        }
-        cur_pins += deg;
+        int lg = 1;
+        while ((1 << lg) < deg) lg++;
+        open[lg].push_back(i);
+        if ((int)open[lg].size() == (64 >> lg)) close(lg);
     }
-    flush(num_nets);
-    num_waves = (int)waves.size();
+    for (int lg = 1; lg <= 6; lg++) close(lg);
+    num_waves = (int)lgs.size();
     num_big = (int)big.size();
     if (num_waves) {
-        AP_HIP(hipMalloc(&d_waves, waves.size() * sizeof(int4)));
-        AP_HIP(hipMemcpy(d_waves, waves.data(), waves.size() * sizeof(int4), hipMemcpyHostToDevice));
+        const long long n = (long long)num_waves * 64;
+        int* d_lane_pin = nullptr;
+        AP_HIP(hipMalloc(&d_lg, lgs.size()));
+        AP_HIP(hipMemcpy(d_lg, lgs.data(), lgs.size(), hipMemcpyHostToDevice));
+        AP_HIP(hipMalloc(&d_lanes, n * sizeof(int4)));
+        AP_HIP(hipMalloc(&d_lane_pin, n * sizeof(int)));
+        AP_HIP(hipMemcpy(d_lane_pin, lane_pin.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(wa_lanes_fill, dim3(grid_of(n, 256)), dim3(256), 0, nullptr, d_lane_pin, n, d_pin_cell,
+                           (const float2*)d_pin_off, d_pin_slot, d_lanes);
+        const hipError_t e = hipGetLastError();
+        const hipError_t e2 = hipDeviceSynchronize();
+        (void)hipFree(d_lane_pin);
+        AP_HIP(e);
+        AP_HIP(e2);
     }
     if (num_big) {
         AP_HIP(hipMalloc(&d_big, big.size() * sizeof(int)));
@@ -183,16 +215,17 @@ int WaPlan::create(const int* d_net_start, int num_nets) {
 }
 
 void WaPlan::destroy() {
-    if (d_waves) (void)hipFree(d_waves);
+    if (d_lg) (void)hipFree(d_lg);
+    if (d_lanes) (void)hipFree(d_lanes);
     if (d_big) (void)hipFree(d_big);
-    d_waves = nullptr;
+    d_lg = nullptr;
+    d_lanes = nullptr;
     d_big = nullptr;
 }
 
 int WaPlan::launch(const WaArgs& a, double* partial, hipStream_t s) const {
     if (num_waves) {
-        WaveArgs w{a.pos, a.pos_other, a.flag, a.pin_cell, a.pin_off, a.net_start, d_waves, num_waves, a.num_mov,
-                   a.gamma, a.pin_slot, a.pin_grad, partial};
+        WaveArgs w{a.pos, a.pos_other, a.flag, d_lg, d_lanes, num_waves, a.num_mov, a.gamma, a.pin_grad, partial};
         hipLaunchKernelGGL(wa_wave_kernel, dim3(wave_blocks()), dim3(256), 0, s, w);
         AP_LAUNCHED();
     }
