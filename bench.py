@@ -61,6 +61,21 @@ def algorithmic_bytes(cls, dims):
     }.get(cls)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "engine_2M_cells_pmc_traffic.md")
+
+
+def profile_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE summary of the 2M-cell design
+    (copy-calibrated), or None."""
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == "ap::" + kernel:
+            return float(cells[4]) * 1e6
+    return None
+
+
 def make_design(config, seed=0):
     from autoplace_amd import synth
     from autoplace_amd.database import PlaceData
@@ -234,6 +249,10 @@ def main():
             "peak": PEAK_HBM / 1e9, "unit": "GB/s", "traffic": None, "algorithmic_bytes": by, "launch_ms": dom_ms,
             "events": "event-record nodes in the timed graph" if in_graph else "direct launches after the timed region"}
     roof["frac"] = roof["achieved"] / roof["peak"]
+    if a.config == "C":
+        roof["traffic"] = profile_traffic(SINGLE_KERNEL[dom])
+        if roof["traffic"] is not None:
+            roof["traffic_source"] = os.path.relpath(TRAFFIC_PROFILE, ROOT)
     out = {"metric": "grad evals/s @2M cells", "value": a.steps / elapsed,
            "unit": "grad evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "ms_per_step": elapsed / a.steps * 1e3,

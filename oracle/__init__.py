@@ -11,8 +11,8 @@ Modules and what pins each one (tests/test_oracle_*.py):
                                                              patterns, linearity, reference goldens
     poisson.py   spectral field (mirrored FFT and DCT)     DCT closed form vs mirrored FFT, cosine mode with an
                                                              analytic field, uniform map -> zero field
-    control.py   BB step, scheduler, stop rules (FP64)     reference ParamScheduler goldens, hand-derived cases
-    nesterov.py  Nesterov step, BB estimate                elementwise algebra against hand-computed values
+    control.py   BB step, scheduler, stop rules (FP64),    reference ParamScheduler goldens, hand-derived cases;
+                 Nesterov step, BB norms                   elementwise algebra against hand-computed values
     cluster.py   star-matching clustering (torch CPU)      reference cluster-id goldens, integer invariants
     place.py     the whole level / two-level flow (CPU)    reference full-run goldens on the 10k-cell design
 """
