@@ -43,6 +43,13 @@ _SIGNATURES = {
     "ap_level_class_ms": ([P, ctypes.POINTER(D), ctypes.POINTER(I)], I),
     "ap_level_objective": ([P, P, P, P, P], I),
     "ap_level_density": ([P, P, P], I),
+    "ap_mlp_param_count": ([], I),
+    "ap_mlp_work_floats": ([I], ctypes.c_longlong),
+    "ap_mlp_forward": ([P, P, I, P, P], I),
+    "ap_mlp_grad": ([P, P, P, P, I, P, P, P, P], I),
+    "ap_mlp_adam": ([P, P, P, P, I, F, F, F, F, P], I),
+    "ap_level_set_controller": ([P, P, F, ctypes.c_ulonglong, I, F, P, I, I], I),
+    "ap_level_controller_state": ([P, ctypes.POINTER(D)], I),
 }
 
 

@@ -24,6 +24,7 @@
 #include "gp_common.h"
 #include "gp_control.h"
 #include "gp_kernels.h"
+#include "gp_mlp.h"
 #include "gp_poisson.h"
 #include "gp_wa.h"
 
@@ -367,10 +368,12 @@ struct GatherArgs {
     int use_density;          // 0: grad = wl / max(pins, 1) (the weight-0 objective)
 };
 
-__global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part) {
-    __shared__ double red[16];
+// part[b]: |g' - g|^2 of workgroup b; part_wl[b], part_dn[b]: squared norms of the wirelength part and of the
+// weighted density part of the (unpreconditioned) gradient, the learned controller's gradient-ratio feature
+__global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part, double* part_wl, double* part_dn) {
+    __shared__ double red[16 * 3];
     const int i = xcd_remap(blockIdx.x, gridDim.x) * EB + threadIdx.x;
-    double t2[1] = {0.0};
+    double t2[3] = {0.0, 0.0, 0.0};
     if (i < a.num_nodes) {
         int cur = a.flags ? a.flags[0] : 0;
         const float2* pos = a.pos.get(true);
@@ -392,6 +395,7 @@ __global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part) {
             }
             gx *= weight;
             gy *= weight;
+            t2[2] = (double)gx * gx + (double)gy * gy;
         }
         if (i < a.num_mov) {
             float wx = 0, wy = 0;
@@ -403,6 +407,7 @@ __global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part) {
             }
             gx += wx;
             gy += wy;
+            t2[1] = (double)wx * wx + (double)wy * wy;
         }
         float scale;
         if (a.use_density) {
@@ -420,8 +425,12 @@ __global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part) {
             t2[0] = dx * dx + dy * dy;
         }
     }
-    block_sum<1>(t2, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = t2[0];
+    block_sum<3>(t2, red);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = t2[0];
+        part_wl[blockIdx.x] = t2[1];
+        part_dn[blockIdx.x] = t2[2];
+    }
 }
 
 // ------------------------------------------------------------------------------ control (+ reductions)
@@ -434,13 +443,15 @@ struct Segs {
     int n_over;
     const double* turned;
     int n_turned;
-    double over_scale;       // bin_area / mov_area of the grid the overflow is measured on
+    const double* wl2;       // squared norms of the gradient's wirelength / density parts
+    const double* dn2;
+    int n_norm;
 };
 
 // First stage of the reduction of the partials: workgroup b sums the b-th fixed slice of every segment
-// (moved^2, turned^2, WA, HPWL, overflow) -> out[b][5]. The slices and the order inside them do not depend on the
-// launch, so the sums are deterministic.
-constexpr int RED_BLOCKS = 64;
+// (moved^2, turned^2, WA, HPWL, overflow, wl^2, density^2) -> out[b][RED_VALS]. The slices and the order inside
+// them do not depend on the launch, so the sums are deterministic.
+constexpr int RED_BLOCKS = 64, RED_VALS = 7;
 
 __device__ __forceinline__ void slice(int n, int& lo, int& hi) {
     const int per = (n + RED_BLOCKS - 1) / RED_BLOCKS;
@@ -449,8 +460,8 @@ __device__ __forceinline__ void slice(int n, int& lo, int& hi) {
 }
 
 __global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) {
-    __shared__ double red[16 * 5];
-    double v[5] = {0, 0, 0, 0, 0};
+    __shared__ double red[16 * RED_VALS];
+    double v[RED_VALS] = {0, 0, 0, 0, 0, 0, 0};
     int lo, hi;
     slice(s.n_moved, lo, hi);
     for (int k = lo + threadIdx.x; k < hi; k += EB) v[0] += s.moved[k];
@@ -463,23 +474,91 @@ __global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) {
     }
     slice(s.n_over, lo, hi);
     for (int k = lo + threadIdx.x; k < hi; k += EB) v[4] += s.over[k];
-    block_sum<5>(v, red);
+    slice(s.n_norm, lo, hi);
+    for (int k = lo + threadIdx.x; k < hi; k += EB) {
+        v[5] += s.wl2[k];
+        v[6] += s.dn2[k];
+    }
+    block_sum<RED_VALS>(v, red);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 5; k++) out[blockIdx.x * 5 + k] = v[k];
+        for (int k = 0; k < RED_VALS; k++) out[blockIdx.x * RED_VALS + k] = v[k];
 }
 
-__global__ __launch_bounds__(64) void eng_control(const double* red5, double over_scale, double* S, float* kp,
-                                                  const double* c, int* flags, double* stats) {
+// Second stage and the control. With a learned controller (ctl.params) the 64 lanes evaluate the MLP on the pass's
+// features first; its clipped outputs (plus the exploration noise, when asked) scale the rule's density-weight
+// growth and smoothing, until the overflow stalls for ctl.window passes (then the rule alone, for the level).
+__global__ __launch_bounds__(64) void eng_control(const double* red, double over_scale, double* S, float* kp,
+                                                  const double* c, int* flags, double* stats, Ctl ctl) {
+    __shared__ double r[RED_VALS];
+    __shared__ float x[MLP_IN];
+    __shared__ float h[MLP_H];
+    __shared__ int learn;
     if (threadIdx.x == 0) {
-        double r[5] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < RED_VALS; k++) r[k] = 0;
         for (int b = 0; b < RED_BLOCKS; b++)
-            for (int k = 0; k < 5; k++) r[k] += red5[b * 5 + k];
+            for (int k = 0; k < RED_VALS; k++) r[k] += red[b * RED_VALS + k];
+        learn = 0;
+        if (S && ctl.params && S[STOP] == 0) {
+            const double moved = sqrt(r[0]), turned = sqrt(r[1]), wa = r[2], hpwl = r[3], ovf = r[4] * over_scale;
+            double* X = ctl.X;
+            if (X[X_STARTED] == 0) {
+                X[X_STARTED] = 1;
+                X[X_PREV_OVF] = ovf;
+                X[X_HPWL0] = hpwl;
+                X[X_ALPHA0] = turned > 0 ? moved / turned : S[ALPHA];
+                X[X_BEST_OVF] = ovf;
+                X[X_LAST_IMPROVE] = S[ITER];
+            }
+            if (X[X_FALLBACK] == 0 && isfinite(moved) && isfinite(turned) && isfinite(hpwl) && isfinite(ovf)) {
+                mlp_features(S, c, X, moved, turned, hpwl, ovf, wa, r[5], r[6], ctl.phase, x);
+                learn = 1;
+            }
+        }
+    }
+    __syncthreads();
+    float o0 = 0.f, o1 = 0.f;
+    if (learn) mlp_forward_block(ctl.params, x, h, o0, o1);      // uniform: learn is one LDS word
+    if (threadIdx.x == 0) {
         const double moved = sqrt(r[0]), turned = sqrt(r[1]), wa = r[2], hpwl = r[3], ovf = r[4] * over_scale;
         if (stats) {
             stats[0] = wa; stats[1] = hpwl; stats[2] = ovf; stats[3] = moved; stats[4] = turned;
+            stats[5] = sqrt(r[5]); stats[6] = sqrt(r[6]);
         }
         if (S) {
-            control_step(S, kp, c, moved, turned, hpwl, ovf, wa);
+            ScheduleAdjust adj;
+            if (learn) {
+                double* X = ctl.X;
+                const unsigned long long n = (unsigned long long)X[X_COUNT];
+                if (ctl.sigma > 0) {
+                    const unsigned long long d = n / (unsigned long long)ctl.hold;
+                    o0 += ctl.sigma * normal_draw(ctl.seed, d, 0);
+                    o1 += ctl.sigma * normal_draw(ctl.seed, d, 1);
+                }
+                o0 = fminf(MLP_CLIP, fmaxf(-MLP_CLIP, o0));
+                o1 = fminf(MLP_CLIP, fmaxf(-MLP_CLIP, o1));
+                adj.lam_factor = exp((double)o0);
+                adj.gam_factor = exp((double)o1);
+                if (ctl.log && n < (unsigned long long)ctl.log_cap) {
+                    float* L = ctl.log + n * MLP_LOG;
+                    for (int k = 0; k < MLP_IN; k++) L[k] = x[k];
+                    L[12] = o0;
+                    L[13] = o1;
+                    L[14] = (float)ovf;
+                    L[15] = (float)hpwl;
+                }
+                X[X_COUNT] = (double)(n + 1);
+            }
+            control_step(S, kp, c, moved, turned, hpwl, ovf, wa, adj);
+            if (ctl.params && S[ACCEPT] != 0) {
+                double* X = ctl.X;
+                X[X_PREV_OVF] = ovf;
+                if (ovf < X[X_BEST_OVF] * 0.999) {
+                    X[X_BEST_OVF] = ovf;
+                    X[X_LAST_IMPROVE] = S[ITER];
+                } else if (S[ITER] - X[X_LAST_IMPROVE] >= ctl.window) {
+                    X[X_FALLBACK] = 1;
+                }
+            }
             if (S[ACCEPT] != 0) flags[0] = 1 - flags[0];
             flags[1] = S[IS_BEST] != 0 ? 1 : 0;
         }
@@ -579,10 +658,13 @@ struct Level {
     float *rho = nullptr, *fx = nullptr, *fy = nullptr;
     double* part = nullptr;
     double* stats = nullptr;
-    double* red5 = nullptr;    // first-stage sums, RED_BLOCKS x 5
+    double* red = nullptr;     // first-stage sums, RED_BLOCKS x RED_VALS
+    double* ctl_x = nullptr;   // learned controller state, NUM_X
+    Ctl ctl{};                 // the learned controller (params null: the plain rule)
     int* flags = nullptr;
     int nblk_n = 0, nblk_wa = 0;
-    double *p_moved = nullptr, *p_wa = nullptr, *p_over = nullptr, *p_turned = nullptr;
+    double *p_moved = nullptr, *p_wa = nullptr, *p_over = nullptr, *p_turned = nullptr, *p_wl2 = nullptr,
+           *p_dn2 = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr;
     GraphEntry graphs[GRAPH_CACHE];
@@ -735,15 +817,20 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipFuncSetAttribute((const void*)eng_bin_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)acc_lds));
     const long long n_over = fj.tiles + jj.tiles;
-    const long long np = nblk_n + 2LL * nblk_wa + n_over + nblk_n;
+    const long long np = nblk_n + 2LL * nblk_wa + n_over + 3LL * nblk_n;
     AP_HIP(hipMalloc(&part, np * sizeof(double)));
     AP_HIP(hipMemset(part, 0, np * sizeof(double)));
     p_moved = part;
     p_wa = p_moved + nblk_n;
     p_over = p_wa + 2LL * nblk_wa;
     p_turned = p_over + n_over;
+    p_wl2 = p_turned + nblk_n;
+    p_dn2 = p_wl2 + nblk_n;
     AP_HIP(hipMalloc(&stats, 8 * sizeof(double)));
-    AP_HIP(hipMalloc(&red5, RED_BLOCKS * 5 * sizeof(double)));
+    AP_HIP(hipMalloc(&red, RED_BLOCKS * RED_VALS * sizeof(double)));
+    AP_HIP(hipMalloc(&ctl_x, NUM_X * sizeof(double)));
+    AP_HIP(hipMemset(ctl_x, 0, NUM_X * sizeof(double)));
+    ctl.X = ctl_x;
     AP_HIP(hipMalloc(&flags, 4 * sizeof(int)));
     AP_HIP(hipMemset(flags, 0, 4 * sizeof(int)));
     AP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -762,7 +849,7 @@ void Level::destroy() {
     last_ev = nullptr;
     for (hipEvent_t& ev : pev)
         if (ev) (void)hipEventDestroy(ev);
-    void* ptrs[] = {u1, v1, g1, pin_grad, rho, fx, fy, part, stats, flags, red5, block_net, pin_slot};
+    void* ptrs[] = {u1, v1, g1, pin_grad, rho, fx, fy, part, stats, flags, red, ctl_x, block_net, pin_slot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : bin_mem) (void)hipFree(p);
@@ -833,14 +920,17 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     g.nby = d.nby;
     g.use_density = use_density;
     mark(AP_PROF_GATHER, s);
-    hipLaunchKernelGGL(eng_gather, dim3(nblk_n), dim3(EB), 0, s, g, p_turned);
+    hipLaunchKernelGGL(eng_gather, dim3(nblk_n), dim3(EB), 0, s, g, p_turned, p_wl2, p_dn2);
     AP_LAUNCHED();
     mark(AP_PROF_CONTROL, s);
     Segs sg{p_moved, with_state ? nblk_n : 0, p_wa, nblk_wa, p_over, n_over, p_turned, g_prev ? nblk_n : 0,
-            over_scale};
-    hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(EB), 0, s, sg, red5);
+            p_wl2, p_dn2, nblk_n};
+    hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(EB), 0, s, sg, red);
     AP_LAUNCHED();
-    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red5, over_scale, S, kp_out, d.c, flags, stats);
+    Ctl ctl_pass = ctl;
+    if (!S) ctl_pass.params = nullptr;                       // a plain objective evaluation: no controller
+    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red, over_scale, S, kp_out, d.c, flags, stats,
+                       ctl_pass);
     AP_LAUNCHED();
     mark(AP_PROF_CLASSES, s);
     return AP_OK;
@@ -1076,6 +1166,37 @@ int ap_level_density(ap_level* level, float* rho, void* stream) {
     Level& L = level->L;
     AP_HIP(hipMemcpyAsync(rho, L.rho, (size_t)L.d.nbx * L.d.nby * sizeof(float), hipMemcpyDeviceToDevice,
                           (hipStream_t)stream));
+    return AP_OK;
+}
+
+int ap_level_set_controller(ap_level* level, const float* params, float sigma, unsigned long long seed, int hold,
+                            float phase, float* log, int log_cap, int window) {
+    AP_REQUIRE(level && sigma >= 0 && hold > 0 && log_cap >= 0 && window > 0 && (!log || log_cap > 0),
+               "level_set_controller: bad arguments");
+    Level& L = level->L;
+    AP_HIP(hipDeviceSynchronize());
+    for (GraphEntry& G : L.graphs) {                 // the kernels' arguments change: capture again
+        if (L.last_ev == &G.ev) L.last_ev = nullptr;
+        G.clear();
+    }
+    L.ctl.params = params;
+    L.ctl.sigma = sigma;
+    L.ctl.seed = seed;
+    L.ctl.hold = hold;
+    L.ctl.phase = phase;
+    L.ctl.log = log;
+    L.ctl.log_cap = log ? log_cap : 0;
+    L.ctl.window = window;
+    AP_HIP(hipMemset(L.ctl_x, 0, NUM_X * sizeof(double)));
+    AP_HIP(hipDeviceSynchronize());
+    return AP_OK;
+}
+
+int ap_level_controller_state(ap_level* level, double* x) {
+    AP_REQUIRE(level && x, "level_controller_state: null argument");
+    Level& L = level->L;
+    AP_HIP(hipStreamSynchronize(L.stream));
+    AP_HIP(hipMemcpy(x, L.ctl_x, NUM_X * sizeof(double), hipMemcpyDeviceToHost));
     return AP_OK;
 }
 

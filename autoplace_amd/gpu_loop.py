@@ -120,6 +120,25 @@ class Engine(object):
         _lib.check(_lib.lib().ap_level_class_ms(self._h, ctypes.byref(ms), ctypes.byref(n)), "level_class_ms")
         return ms.value, n.value
 
+    def set_controller(self, ctl, phase):
+        """Attach the learned controller (dict: params device f32 tensor or None, sigma, seed, hold, window, log_cap).
+        -> the log tensor [log_cap, 16] (None without a log)."""
+        params = ctl.get("params")
+        cap = int(ctl.get("log_cap", 0))
+        log = torch.zeros((cap, 16), dtype=torch.float32, device=self.v.device) if cap > 0 else None
+        self._ctl = (params, log)
+        seed = (int(ctl.get("seed", 0)) * 2 + (1 if phase else 0)) & 0xFFFFFFFFFFFFFFFF
+        _lib.check(_lib.lib().ap_level_set_controller(
+            self._h, _p(params) if params is not None else None, float(ctl.get("sigma", 0.0)), seed,
+            int(ctl.get("hold", 1)), float(phase),
+            _p(log) if log is not None else None, cap, int(ctl.get("window", 50))), "level_set_controller")
+        return log
+
+    def controller_state(self):
+        st = (ctypes.c_double * 8)()
+        _lib.check(_lib.lib().ap_level_controller_state(self._h, st), "level_controller_state")
+        return list(st)
+
     def density(self):
         """Force-grid density [nbx, nby] of the last objective evaluated, in the engine's bin layout."""
         d = self.desc
@@ -167,10 +186,12 @@ def control_state(ps, optimizer, iteration, args):
 
 
 def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every=16, use_graph=True,
-                     poisson="auto", reorder=True, hook=None):
+                     poisson="auto", reorder=True, hook=None, controller=None, phase=0.0):
     """Continue the level from `iteration` (density weight on) until it stops; updates optimizer, ps and
     objective.evals as the host loop would. -> (reason, iterations, hpwl, overflow) of the kept placement.
-    hook(objective, u, v, g, S, kp, c), if given, sees the state the device loop starts from (instrumentation)."""
+    hook(objective, u, v, g, S, kp, c), if given, sees the state the device loop starts from (instrumentation).
+    controller: None (the plain rule) or a dict for the learned controller (controller.py: `attach`), whose "levels"
+    list receives this level's record (passes under the controller, fallback, the logged passes)."""
     device, pos = optimizer.v.device, optimizer.v
     S, kp, c = control_state(ps, optimizer, iteration, args)
     S = torch.tensor(S, dtype=torch.float64, device=device)
@@ -182,6 +203,9 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
         hook(objective, u, pos, g, S, kp, c)
     best = pos.clone()
     eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
+    ctl_log = None
+    if controller is not None:
+        ctl_log = eng.set_controller(controller, phase)
     evals_before, logged = objective.evals, iteration // args.log_freq
     watch = torch.tensor([K["stop"], K["loop_it"], K["lam"], K["gam"], K["res_hpwl"], K["res_ovf"]], device=device)
     try:
@@ -194,6 +218,12 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
             if stop:
                 break
         eng.finish()
+        if controller is not None:
+            st = eng.controller_state()
+            n = int(st[7])
+            controller.setdefault("levels", []).append(
+                {"phase": phase, "passes": n, "fallback": bool(st[6]),
+                 "log": ctl_log[:min(n, len(ctl_log))].cpu().numpy() if ctl_log is not None else None})
     finally:
         torch.cuda.synchronize(device)
         eng.close()

@@ -30,7 +30,7 @@ def _scheduler(density, num_mov, mov_area, total_pins, total_area, num_nodes, ar
                                           refine=refine(density)), args)
 
 
-def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None):
+def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None, phase=0.0):
     """One level (run_placement.py:41-76): host loop through the wirelength-only start, then the device loop.
     -> namespace(reason, iters, start_iters, hpwl, overflow)."""
     opts = opts or {}
@@ -44,7 +44,8 @@ def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None):
         if ps.density_weight > 0 and gpu_loop.usable(optimizer):
             reason, iters, hpwl, overflow = gpu_loop.run_level_on_gpu(
                 optimizer, objective, ps, args, log, iteration, use_graph=opts.get("graph", True),
-                poisson=opts.get("poisson", "auto"), reorder=opts.get("reorder", True), hook=opts.get("hook"))
+                poisson=opts.get("poisson", "auto"), reorder=opts.get("reorder", True), hook=opts.get("hook"),
+                controller=opts.get("controller"), phase=phase)
             return SimpleNamespace(reason=reason, iters=iters, start_iters=start_iters, hpwl=hpwl, overflow=overflow)
         optimizer.step()
         if optimizer.diverged:
@@ -91,7 +92,7 @@ def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=N
         cps.handover_gamma = ps.wa_coeff
         cobjective = Objective(coarse.wirelength, coarse.density, cps)
         cpos = torch.cat([pos[:coarse.num], pos[n:]])
-        level = _place_level(cpos, cobjective, cps, cargs, log, 0, opts)
+        level = _place_level(cpos, cobjective, cps, cargs, log, 0, opts, phase=1.0)
         pos.copy_(expand(cpos, coarse, node_size, args.seed)).clamp_(min=density.pos_lo, max=density.pos_hi)
         weight = cps.density_weight * objective.tension_unit(pos) / cobjective.tension_unit(cpos)
         overflow = density.forward(pos)[1].item() if args.wa_rule == "overflow" else None

@@ -179,6 +179,40 @@ AP_API int ap_level_objective(ap_level* level, const float* pos, float* grad, do
  * ap_level_objective): the top-left quadrant of the reference's density_map result. */
 AP_API int ap_level_density(ap_level* level, float* rho, void* stream);
 
+/* ---------------------------------------------------------------- part 4: learned schedule controller
+ * No counterpart in the reference: a 12 -> 64 -> 64 -> 2 ReLU MLP (FP32, parameters f32[ap_mlp_param_count()] laid
+ * out W1[64][12], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]) whose outputs, clipped to +-0.2, are the logs of
+ * factors on the density-weight growth and on the smoothing of every pass of the engine; the reference's rule
+ * (gp_cuda_kernel.cu:198-209) is what they scale. Features (12, each clamped to [-10, 10]): overflow,
+ * 100 x its change, log(HPWL / HPWL at the controller's start), 10 x relative HPWL change, WA model error
+ * 1 - WA / HPWL, log(gamma / base gamma), density share lambda A / (pins + lambda A), log(step / first step),
+ * 0.2 x log(|wirelength gradient| / |weighted density gradient|), iteration / 1000, log(1 + motion), phase
+ * (1 coarse level, 0 cell level). */
+
+AP_API int ap_mlp_param_count(void);
+/* Workspace (floats) ap_mlp_grad needs for n samples. */
+AP_API long long ap_mlp_work_floats(int n);
+/* out f32[n][2] = the MLP (unclipped) on x f32[n][12]. */
+AP_API int ap_mlp_forward(const float* params, const float* x, int n, float* out, void* stream);
+/* Weighted squared loss sum_s weight[s] |mlp(x_s) - target_s|^2 (target f32[n][2]) into loss f64[1] and its
+ * gradient into grad f32[ap_mlp_param_count()]; deterministic. work: f32[ap_mlp_work_floats(n)]. */
+AP_API int ap_mlp_grad(const float* params, const float* x, const float* target, const float* weight, int n,
+                       float* grad, double* loss, float* work, void* stream);
+/* One Adam step (bias-corrected, step >= 1) on params with moments m, v. */
+AP_API int ap_mlp_adam(float* params, float* m, float* v, const float* grad, int step, float lr, float beta1,
+                       float beta2, float eps, void* stream);
+/* Attach the controller to a level's passes (params null: the plain rule). sigma: Gaussian exploration noise added
+ * to the outputs before the clip (counter-based draws from seed, a new draw every `hold` learned passes); phase: 1
+ * for a coarse level; log: null or
+ * f32[log_cap][16] receiving, per learned pass, the 12 features, the 2 applied outputs, overflow and HPWL; window:
+ * the plain rule takes over for the rest of the level after `window` passes without a 0.1 % overflow improvement.
+ * Resets the controller's state and drops the level's captured graphs. */
+AP_API int ap_level_set_controller(ap_level* level, const float* params, float sigma, unsigned long long seed,
+                                   int hold, float phase, float* log, int log_cap, int window);
+/* The controller's state f64[8]: started, previous overflow, first HPWL, first step, best overflow, pass of the last
+ * improvement, fallback (1: the rule took over), learned passes so far. Waits for the level's stream. */
+AP_API int ap_level_controller_state(ap_level* level, double* state);
+
 #ifdef __cplusplus
 }
 #endif

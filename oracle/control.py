@@ -22,8 +22,9 @@ def next_coef(a):
     return (a - 1) / a_next
 
 
-def control_pass(S, kp, c, moved, turned, hpwl, ovf, wa):
-    """Mutates S (list of 21 floats) and kp (float32 [5]) as control_kernel does."""
+def control_pass(S, kp, c, moved, turned, hpwl, ovf, wa, lam_factor=1.0, gam_factor=1.0):
+    """Mutates S (list of 21 floats) and kp (float32 [5]) as control_kernel does. lam_factor / gam_factor: the learned
+    controller's factors on the density-weight growth and on the new smoothing (1: the plain rule)."""
     S[K["accept"]] = 0.0
     S[K["is_best"]] = 0.0
     if S[K["stop"]] != 0:
@@ -45,16 +46,17 @@ def control_pass(S, kp, c, moved, turned, hpwl, ovf, wa):
         mu = 1.05 * 0.98
     else:
         mu = 1.05 * max(math.pow(1.05, -S[K["motion"]] * c[C["pitch"]] / S[K["gam"]]), 0.95)
+    mu *= lam_factor
     S[K["hpwl_prev"]] = hpwl
     S[K["mu"]] = mu
     S[K["lam"]] *= mu
     if c[C["accuracy"]] != 0:
         err = max(1e-6, 1 - wa / max(hpwl, 1e-12))
         m = max(mu, 1 / mu)
-        g = S[K["gam"]] * min(m, max(1 / m, math.sqrt(max(ovf, 1e-3) / c[C["refine"]] / err)))
+        g = S[K["gam"]] * min(m, max(1 / m, math.sqrt(max(ovf, 1e-3) / c[C["refine"]] / err))) * gam_factor
         S[K["gam"]] = min(100 * c[C["base_gamma"]], max(0.01 * c[C["base_gamma"]], g))
     else:
-        S[K["gam"]] = c[C["base_gamma"]] * math.pow(10.0, (ovf - 0.1) * 20 / 9 - 1)
+        S[K["gam"]] = c[C["base_gamma"]] * math.pow(10.0, (ovf - 0.1) * 20 / 9 - 1) * gam_factor
     it = it + 1
     S[K["iter"]] = it
     stop, is_best = 0.0, 0.0

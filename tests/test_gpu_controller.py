@@ -1,0 +1,145 @@
+"""The learned schedule controller on the GPU: the MLP kernels against the FP64 oracle, the zero network against the
+plain rule (bit for bit), the logged passes against the oracle's MLP + noise + clip, the fallback, and what the
+trained network gains over the rule on held-out designs."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import norm_err, rel_err
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _t(a):
+    return torch.tensor(np.asarray(a, np.float32), device=DEV)
+
+
+def test_mlp_forward_gradient_and_adam_vs_fp64_oracle():
+    from autoplace_amd import controller
+    rng = np.random.default_rng(3)
+    p = rng.normal(0, 0.2, omlp.PARAMS).astype(np.float32)
+    n = 1000                                              # not a multiple of the 64-sample chunk
+    x = rng.normal(0, 1, (n, 12)).astype(np.float32)
+    t = rng.normal(0, 0.1, (n, 2)).astype(np.float32)
+    w = (rng.random(n) / n).astype(np.float32)
+    out = controller.forward(_t(p), _t(x)).cpu().numpy()
+    ref = omlp.forward(p, x)
+    e_f = norm_err(out, ref)
+    loss, grad = controller.loss_grad(_t(p), _t(x), _t(t), _t(w))
+    l64, g64 = omlp.loss_grad(p, x.astype(np.float64), t.astype(np.float64), w.astype(np.float64))
+    e_l, e_g = rel_err(float(loss), l64), norm_err(grad.cpu().numpy(), g64)
+    print("mlp: forward %.2e, loss %.2e, gradient %.2e against FP64" % (e_f, e_l, e_g))
+    assert e_f < 1e-5 and e_l < 1e-5 and e_g < 1e-4
+    # deterministic: the same call gives the same bits
+    _, grad2 = controller.loss_grad(_t(p), _t(x), _t(t), _t(w))
+    assert torch.equal(grad, grad2)
+    # one Adam step
+    opt = controller.Adam(_t(p), lr=1e-2)
+    opt.step(grad)
+    g = grad.cpu().numpy().astype(np.float64)
+    m, v = 0.1 * g, 0.001 * g * g
+    want = p - 1e-2 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    assert norm_err(opt.params.cpu().numpy(), want) < 1e-6
+
+
+def test_fit_learns_a_target():
+    from autoplace_amd import controller
+    rng = np.random.default_rng(4)
+    x = rng.normal(0, 1, (20000, 12)).astype(np.float32)
+    t = np.stack([0.15 * np.tanh(x[:, 0] - 0.5 * x[:, 3]), -0.1 * np.tanh(x[:, 1] * x[:, 2])], 1).astype(np.float32)
+    w = np.ones(len(x), np.float32)
+    params = controller.init_params(0)
+    base, _ = controller.loss_grad(params, _t(x), _t(t), _t(w / len(x)))
+    loss = controller.fit(params, _t(x), _t(t), _t(w), epochs=60, lr=3e-3, batch=4096)
+    print("fit: weighted loss %.3e -> %.3e" % (float(base), loss))
+    assert loss < 0.1 * float(base)
+
+
+def _place(data, ctl):
+    from autoplace_amd.problem import default_args
+    from autoplace_amd.run_placement import place
+    opts = {"controller": ctl} if ctl is not None else {}
+    pos, m = place(data, default_args(), DEV, opts=opts)
+    return pos.cpu().numpy(), m
+
+
+def test_zero_network_is_the_plain_rule(small_design):
+    from autoplace_amd import controller
+    data = small_design[2]
+    p0, m0 = _place(data, None)
+    ctl = controller.options(controller.init_params(5), log_cap=4096)
+    p1, m1 = _place(data, ctl)
+    assert np.array_equal(p0, p1) and m0["iters"] == m1["iters"] and m0["hpwl_sites"] == m1["hpwl_sites"]
+    levels = ctl["levels"]
+    assert len(levels) == 2 and levels[0]["phase"] == 1.0 and levels[1]["phase"] == 0.0
+    for lv in levels:
+        assert lv["passes"] > 0 and np.all(lv["log"][:, 12:14] == 0)
+
+
+def test_logged_passes_match_the_oracle_network(small_design):
+    """Every logged pass: the applied outputs are the oracle MLP on the logged features, plus the counter-based
+    noise draw, clipped; the logged overflow is the first feature."""
+    from autoplace_amd import controller
+    data = small_design[2]
+    rng = np.random.default_rng(6)
+    p = rng.normal(0, 0.05, omlp.PARAMS).astype(np.float32)
+    p[omlp.W3:] *= 0.5
+    ctl = controller.options(_t(p), sigma=0.05, seed=11, hold=7, log_cap=4096)
+    _, m = _place(data, ctl)
+    checked = 0
+    for lv in ctl["levels"]:
+        seed = (11 * 2 + (1 if lv["phase"] else 0))
+        log = lv["log"]
+        o = omlp.forward(p, log[:, :12])
+        for n in range(len(log)):
+            d = n // 7
+            want = [min(0.2, max(-0.2, o[n, k] + np.float32(0.05) * np.float32(omlp.normal_draw(seed, d, k))))
+                    for k in (0, 1)]
+            assert np.allclose(log[n, 12:14], want, atol=2e-5), (n, log[n, 12:14], want)
+            checked += 1
+        assert np.allclose(log[:, 0], np.clip(log[:, 14], -10, 10), rtol=1e-6)
+    print("controller: %d logged passes match the oracle, %d iterations, stop %s" % (checked, m["iters"],
+                                                                                  m["stop_reason"]))
+    assert checked > 50
+
+
+def test_fallback_hands_over_to_the_rule(small_design):
+    """A window of 3 passes without an overflow improvement: the controller stops after it and the rule finishes."""
+    from autoplace_amd import controller
+    data = small_design[2]
+    rng = np.random.default_rng(7)
+    p = rng.normal(0, 0.05, omlp.PARAMS).astype(np.float32)
+    ctl = controller.options(_t(p), window=3, log_cap=4096)
+    _, m = _place(data, ctl)
+    assert any(lv["fallback"] for lv in ctl["levels"])
+    assert m["stop_reason"] == "converged"
+    total = sum(lv["passes"] for lv in ctl["levels"])
+    assert total < m["iters"]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                                    "autoplace_amd", "controller_weights.npz")),
+                    reason="no trained controller weights in the package")
+def test_trained_controller_gains_on_held_out_designs():
+    """The committed network against the plain rule on two held-out designs (seeds 1001, 1002; 60k cells): fewer
+    iterations, HPWL within 0.5 %."""
+    from autoplace_amd import controller, synth
+    from autoplace_amd.database import PlaceData
+    params, meta = controller.load(device=DEV)
+    rows = []
+    for seed in (1001, 1002):
+        lef, d = synth.generate(60000, seed=seed, name="heldout%d" % seed)
+        data = PlaceData(lef, d)
+        _, rule = _place(data, None)
+        _, learned = _place(data, controller.options(params))
+        rows.append((learned["iters"] / rule["iters"], learned["hpwl_sites"] / rule["hpwl_sites"],
+                     learned["stop_reason"]))
+        print("held-out %d: iterations %d -> %d, HPWL x%.4f, %s" % (seed, rule["iters"], learned["iters"],
+                                                                   rows[-1][1], learned["stop_reason"]))
+    assert all(r[2] == "converged" for r in rows)
+    assert np.mean([r[0] for r in rows]) < 0.95
+    assert max(r[1] for r in rows) < 1.005

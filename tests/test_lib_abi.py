@@ -9,7 +9,7 @@ from conftest import ROOT
 def _declared():
     with open(os.path.join(ROOT, "include", "autoplace_gp.h")) as f:
         text = f.read()
-    return sorted(set(re.findall(r"AP_API\s+(?:const\s+)?\w+\s*\*?\s*(ap_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"AP_API\s+[\w\s*]*?\b(ap_\w+)\s*\(", text)))
 
 
 def test_header_declares_entry_points():
