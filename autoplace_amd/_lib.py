@@ -50,6 +50,14 @@ _SIGNATURES = {
     "ap_mlp_adam": ([P, P, P, P, I, F, F, F, F, P], I),
     "ap_level_set_controller": ([P, P, F, ctypes.c_ulonglong, I, F, P, I, I], I),
     "ap_level_controller_state": ([P, ctypes.POINTER(D)], I),
+    "ap_level_autotune": ([P, I, P], I),
+    "ap_batch_create": ([P, I, ctypes.POINTER(P)], I),
+    "ap_batch_destroy": ([P], I),
+    "ap_batch_prepare": ([P, I], I),
+    "ap_batch_run": ([P, I, I, P], I),
+    "ap_batch_last_ms": ([P, ctypes.POINTER(F)], I),
+    "ap_batch_autotune": ([P, I, P], I),
+    "ap_batch_tile_sides": ([P, P], I),
 }
 
 

@@ -57,16 +57,25 @@ struct Bufs {          // the two point sets: set 0 = the caller's u / v / g, se
 };
 
 // ------------------------------------------------------------------------------ Nesterov (+ deferred best copy)
-__global__ __launch_bounds__(EB) void eng_nesterov(Bufs B, const float2* lo, const float2* hi, const float* kp, int n,
-                                                   double* part) {
+struct NestArgs {
+    Bufs B;
+    const float2* lo;
+    const float2* hi;
+    const float* kp;
+    int n;
+    double* part;
+};
+
+__device__ __forceinline__ void nesterov_body(const NestArgs& a, int wg) {
     __shared__ double red[16];
+    const Bufs& B = a.B;
     const int cur = B.flags[0], nx = 1 - cur;
     const bool copy_best = B.flags[1] != 0;
-    const float alpha = kp[3], coef = kp[4];
-    const int i = blockIdx.x * EB + threadIdx.x;
+    const float alpha = a.kp[3], coef = a.kp[4];
+    const int i = wg * EB + threadIdx.x;
     double m2[1] = {0.0};
-    if (i < n) {
-        const float2 u = B.u[cur][i], v = B.v[cur][i], g = B.g[cur][i], l = lo[i], h = hi[i];
+    if (i < a.n) {
+        const float2 u = B.u[cur][i], v = B.v[cur][i], g = B.g[cur][i], l = a.lo[i], h = a.hi[i];
         float2 un, vn;
         un.x = v.x - alpha * g.x;
         un.y = v.y - alpha * g.y;
@@ -79,8 +88,9 @@ __global__ __launch_bounds__(EB) void eng_nesterov(Bufs B, const float2* lo, con
         m2[0] = dx * dx + dy * dy;
     }
     block_sum<1>(m2, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = m2[0];
+    if (threadIdx.x == 0) a.part[wg] = m2[0];
 }
+__global__ __launch_bounds__(EB) void eng_nesterov(NestArgs a) { nesterov_body(a, blockIdx.x); }
 
 // ------------------------------------------------------------------------------ density: tile binning
 // A grid of nbx x nby bins is cut into tiles of T x T bins. Every pass:
@@ -136,9 +146,8 @@ __device__ __forceinline__ bool small_node(const Span& sx, const Span& sy, int T
     return wx <= 1 && wy <= 1;
 }
 
-__global__ __launch_bounds__(EB) void eng_bin_count(Pick pos_p, BinJob f, BinJob j) {
+__device__ __forceinline__ void bin_count_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk) {
     extern __shared__ int hist[];   // planes x tiles of the grid
-    const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const bool judge = blk >= f.blocks;
     const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
     const int base = (blk - (judge ? f.blocks : 0)) * BIN_NODES;
@@ -186,11 +195,14 @@ __global__ __launch_bounds__(EB) void eng_bin_count(Pick pos_p, BinJob f, BinJob
         J.off[i] = o;
     }
 }
+__global__ __launch_bounds__(EB) void eng_bin_count(Pick pos_p, BinJob f, BinJob j) {
+    bin_count_body(pos_p, f, j, xcd_remap(blockIdx.x, gridDim.x));
+}
 
 // one workgroup per grid
-__global__ __launch_bounds__(SCAN_T) void eng_bin_scan(BinJob f, BinJob j) {
+__device__ __forceinline__ void bin_scan_body(const BinJob& f, const BinJob& j, int which) {
     __shared__ int wsum[SCAN_T / 64];
-    const BinJob J = blockIdx.x ? j : f;
+    const BinJob J = which ? j : f;
     const int n = J.planes * J.tiles;
     const int per = (n + SCAN_T - 1) / SCAN_T;
     const int lo = min(n, (int)threadIdx.x * per), hi = min(n, lo + per);
@@ -218,9 +230,9 @@ __global__ __launch_bounds__(SCAN_T) void eng_bin_scan(BinJob f, BinJob j) {
         *J.big_count = 0;
     }
 }
+__global__ __launch_bounds__(SCAN_T) void eng_bin_scan(BinJob f, BinJob j) { bin_scan_body(f, j, blockIdx.x); }
 
-__global__ __launch_bounds__(EB) void eng_bin_scatter(Pick pos_p, BinJob f, BinJob j) {
-    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void bin_scatter_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk) {
     const bool judge = blk >= f.blocks;
     const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
     const int base = (blk - (judge ? f.blocks : 0)) * BIN_NODES;
@@ -258,6 +270,9 @@ __global__ __launch_bounds__(EB) void eng_bin_scatter(Pick pos_p, BinJob f, BinJ
         }
     }
 }
+__global__ __launch_bounds__(EB) void eng_bin_scatter(Pick pos_p, BinJob f, BinJob j) {
+    bin_scatter_body(pos_p, f, j, xcd_remap(blockIdx.x, gridDim.x));
+}
 
 // a node's contributions to the bins of the tile [X0, X1] x [Y0, Y1], into the tile's LDS accumulator a (row stride T)
 __device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0, int Y0, int X1, int Y1,
@@ -275,12 +290,13 @@ __device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0
 
 constexpr int TB = 512;   // lanes per workgroup of the tile kernel: a crowded tile's bucket spreads over 8 waves
 
-__global__ __launch_bounds__(TB) void eng_bin_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
+__device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& f, const BinJob& j, float target,
+                                               double* part, int wg) {
     extern __shared__ unsigned long long acc[];   // planes x T x T
     __shared__ double red[16];
-    const bool judge = (int)blockIdx.x >= f.tiles;
+    const bool judge = wg >= f.tiles;
     const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
-    const int t = (int)blockIdx.x - (judge ? f.tiles : 0);
+    const int t = wg - (judge ? f.tiles : 0);
     const int T = J.T, TT = T * T;
     const int X0 = (t / J.tiles_y) * T, Y0 = (t % J.tiles_y) * T;
     const int X1 = min(X0 + T, J.nbx) - 1, Y1 = min(Y0 + T, J.nby) - 1;
@@ -340,7 +356,10 @@ __global__ __launch_bounds__(TB) void eng_bin_tiles(Pick pos_p, BinJob f, BinJob
         if (J.count_over) o[0] += (double)fmaxf(mov - target, 0.f);
     }
     block_sum<1>(o, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = o[0];
+    if (threadIdx.x == 0) part[wg] = o[0];
+}
+__global__ __launch_bounds__(TB) void eng_bin_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
+    bin_tiles_body(pos_p, f, j, target, part, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------ gather (+ cell WA sum, |g' - g|)
@@ -370,9 +389,10 @@ struct GatherArgs {
 
 // part[b]: |g' - g|^2 of workgroup b; part_wl[b], part_dn[b]: squared norms of the wirelength part and of the
 // weighted density part of the (unpreconditioned) gradient, the learned controller's gradient-ratio feature
-__global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part, double* part_wl, double* part_dn) {
+__device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, double* part_wl, double* part_dn, int wg,
+                                            int nwg) {
     __shared__ double red[16 * 3];
-    const int i = xcd_remap(blockIdx.x, gridDim.x) * EB + threadIdx.x;
+    const int i = xcd_remap(wg, nwg) * EB + threadIdx.x;
     double t2[3] = {0.0, 0.0, 0.0};
     if (i < a.num_nodes) {
         int cur = a.flags ? a.flags[0] : 0;
@@ -427,10 +447,13 @@ __global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part, dou
     }
     block_sum<3>(t2, red);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = t2[0];
-        part_wl[blockIdx.x] = t2[1];
-        part_dn[blockIdx.x] = t2[2];
+        part[wg] = t2[0];
+        part_wl[wg] = t2[1];
+        part_dn[wg] = t2[2];
     }
+}
+__global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part, double* part_wl, double* part_dn) {
+    gather_body(a, part, part_wl, part_dn, blockIdx.x, gridDim.x);
 }
 
 // ------------------------------------------------------------------------------ control (+ reductions)
@@ -453,42 +476,43 @@ struct Segs {
 // them do not depend on the launch, so the sums are deterministic.
 constexpr int RED_BLOCKS = 64, RED_VALS = 7;
 
-__device__ __forceinline__ void slice(int n, int& lo, int& hi) {
+__device__ __forceinline__ void slice(int n, int wg, int& lo, int& hi) {
     const int per = (n + RED_BLOCKS - 1) / RED_BLOCKS;
-    lo = min(n, (int)blockIdx.x * per);
+    lo = min(n, wg * per);
     hi = min(n, lo + per);
 }
 
-__global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) {
+__device__ __forceinline__ void reduce_body(const Segs& s, double* out, int wg) {
     __shared__ double red[16 * RED_VALS];
     double v[RED_VALS] = {0, 0, 0, 0, 0, 0, 0};
     int lo, hi;
-    slice(s.n_moved, lo, hi);
+    slice(s.n_moved, wg, lo, hi);
     for (int k = lo + threadIdx.x; k < hi; k += EB) v[0] += s.moved[k];
-    slice(s.n_turned, lo, hi);
+    slice(s.n_turned, wg, lo, hi);
     for (int k = lo + threadIdx.x; k < hi; k += EB) v[1] += s.turned[k];
-    slice(s.n_wa, lo, hi);
+    slice(s.n_wa, wg, lo, hi);
     for (int k = lo + threadIdx.x; k < hi; k += EB) {
         v[2] += s.wa[2 * k];
         v[3] += s.wa[2 * k + 1];
     }
-    slice(s.n_over, lo, hi);
+    slice(s.n_over, wg, lo, hi);
     for (int k = lo + threadIdx.x; k < hi; k += EB) v[4] += s.over[k];
-    slice(s.n_norm, lo, hi);
+    slice(s.n_norm, wg, lo, hi);
     for (int k = lo + threadIdx.x; k < hi; k += EB) {
         v[5] += s.wl2[k];
         v[6] += s.dn2[k];
     }
     block_sum<RED_VALS>(v, red);
     if (threadIdx.x == 0)
-        for (int k = 0; k < RED_VALS; k++) out[blockIdx.x * RED_VALS + k] = v[k];
+        for (int k = 0; k < RED_VALS; k++) out[wg * RED_VALS + k] = v[k];
 }
+__global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) { reduce_body(s, out, blockIdx.x); }
 
 // Second stage and the control. With a learned controller (ctl.params) the 64 lanes evaluate the MLP on the pass's
 // features first; its clipped outputs (plus the exploration noise, when asked) scale the rule's density-weight
 // growth and smoothing, until the overflow stalls for ctl.window passes (then the rule alone, for the level).
-__global__ __launch_bounds__(64) void eng_control(const double* red, double over_scale, double* S, float* kp,
-                                                  const double* c, int* flags, double* stats, Ctl ctl) {
+__device__ __forceinline__ void control_body(const double* red, double over_scale, double* S, float* kp,
+                                             const double* c, int* flags, double* stats, const Ctl& ctl) {
     __shared__ double r[RED_VALS];
     __shared__ float x[MLP_IN];
     __shared__ float h[MLP_H];
@@ -552,17 +576,20 @@ __global__ __launch_bounds__(64) void eng_control(const double* red, double over
             if (ctl.params && S[ACCEPT] != 0) {
                 double* X = ctl.X;
                 X[X_PREV_OVF] = ovf;
-                if (ovf < X[X_BEST_OVF] * 0.999) {
-                    X[X_BEST_OVF] = ovf;
-                    X[X_LAST_IMPROVE] = S[ITER];
-                } else if (S[ITER] - X[X_LAST_IMPROVE] >= ctl.window) {
-                    X[X_FALLBACK] = 1;
-                }
+                // a stall counts, as in the rule's own stall test, once the density share is 0.95 or more
+                const bool better = ovf < X[X_BEST_OVF] * 0.999;
+                if (better) X[X_BEST_OVF] = ovf;
+                if (better || S[WW] < 0.95) X[X_LAST_IMPROVE] = S[ITER];
+                else if (S[ITER] - X[X_LAST_IMPROVE] >= ctl.window) X[X_FALLBACK] = 1;
             }
             if (S[ACCEPT] != 0) flags[0] = 1 - flags[0];
             flags[1] = S[IS_BEST] != 0 ? 1 : 0;
         }
     }
+}
+__global__ __launch_bounds__(64) void eng_control(const double* red, double over_scale, double* S, float* kp,
+                                                  const double* c, int* flags, double* stats, Ctl ctl) {
+    control_body(red, over_scale, S, kp, c, flags, stats, ctl);
 }
 
 // level set-up: pin -> its position in the cell CSR, so the WA kernel writes pin gradients in cell order
@@ -724,10 +751,23 @@ struct Level {
     }
     int bin_job(BinJob& J, const float* size, const float* ratio, int num_nodes, int num_mov, double bx, double by,
                 int nbx, int nby, int planes, const float* fixed, float* out, int count_over);
+    bool side_ok(const BinJob& J, int T) const;
+    void set_side(BinJob& J, int T);
+    void update_lds();
+    int time_density(int reps, hipStream_t s, float& ms);
+    int autotune(int reps, hipStream_t s, int* sides);
     int create(const ap_level_desc& desc);
     void destroy();
     int objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev, const float2* g_prev_b,
                   float2* g_out_a, float2* g_out_b, hipStream_t s, double* S, float* kp_out, int use_density);
+    // the kernels' arguments (shared by the single-level launches and the batch's tables)
+    WaArgs wa_args(const Pick& pos, const float* kp_ptr) const;
+    GatherArgs gather_args(const Pick& pos, const float* kp_ptr, const float2* g_prev, const float2* g_prev_b,
+                           float2* g_out_a, float2* g_out_b, int use_density) const;
+    Segs segs(bool with_state, bool has_prev, int use_density) const;
+    double over_scale() const;
+    NestArgs nest_args() const;
+    Pick pass_pos() const;
     int pass(hipStream_t s);
     int finalize(hipStream_t s);
     int graph_for(int passes, int& slot);
@@ -752,23 +792,112 @@ int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nod
         set_error("level: density grid %d x %d is beyond the tile binning's %d buckets", nbx, nby, BIN_MAX_BUCKETS);
         return AP_ERR_UNSUPPORTED;
     }
-    J.tiles_y = (int)grid_of(nby, J.T);
-    J.tiles = (int)grid_of(nbx, J.T) * J.tiles_y;
     J.blocks = (int)grid_of(num_nodes, BIN_NODES);
     J.fixed = fixed;
     J.rho = out;
     J.count_over = count_over;
-    const long long nbuck = (long long)planes * J.tiles;
+    set_side(J, J.T);
+    // bucket arrays for the largest bucket count any side may use, so that a new side needs no allocation (the
+    // scan leaves the counts it used at zero; the others were never used)
     int st;
-    if ((st = alloc(&J.count, nbuck)) || (st = alloc(&J.start, nbuck + 1)) || (st = alloc(&J.off, num_nodes)) ||
-        (st = alloc(&J.geo, 4LL * num_nodes)) || (st = alloc(&J.gr, 4LL * num_nodes)) ||
-        (st = alloc(&J.big_count, 1)) || (st = alloc(&J.big_n, 1)) || (st = alloc(&J.big, num_nodes)))
+    if ((st = alloc(&J.count, BIN_MAX_BUCKETS)) || (st = alloc(&J.start, BIN_MAX_BUCKETS + 1)) ||
+        (st = alloc(&J.off, num_nodes)) || (st = alloc(&J.geo, 4LL * num_nodes)) ||
+        (st = alloc(&J.gr, 4LL * num_nodes)) || (st = alloc(&J.big_count, 1)) || (st = alloc(&J.big_n, 1)) ||
+        (st = alloc(&J.big, num_nodes)))
         return st;
-    AP_HIP(hipMemset(J.count, 0, nbuck * sizeof(int)));
+    AP_HIP(hipMemset(J.count, 0, BIN_MAX_BUCKETS * sizeof(int)));
     AP_HIP(hipMemset(J.big_count, 0, sizeof(int)));
     AP_HIP(hipMemset(J.big_n, 0, sizeof(int)));
-    hist_lds = std::max(hist_lds, (size_t)nbuck * sizeof(int));
-    acc_lds = std::max(acc_lds, (size_t)planes * J.T * J.T * sizeof(unsigned long long));
+    update_lds();
+    return AP_OK;
+}
+
+// a tile side the binning can use for J: 8..64 and the buckets within the count kernel's histogram
+bool Level::side_ok(const BinJob& J, int T) const {
+    return T >= 8 && T <= 64 && (long long)J.planes * grid_of(J.nbx, T) * grid_of(J.nby, T) <= BIN_MAX_BUCKETS;
+}
+
+void Level::set_side(BinJob& J, int T) {
+    J.T = T;
+    J.tiles_y = (int)grid_of(J.nby, T);
+    J.tiles = (int)grid_of(J.nbx, T) * J.tiles_y;
+}
+
+void Level::update_lds() {
+    hist_lds = acc_lds = 0;
+    for (const BinJob* J : {&fj, &jj}) {
+        if (!J->count) continue;
+        hist_lds = std::max(hist_lds, (size_t)J->planes * J->tiles * sizeof(int));
+        acc_lds = std::max(acc_lds, (size_t)J->planes * J->T * J->T * sizeof(unsigned long long));
+    }
+}
+
+// The density stage (count, scan, scatter, tiles) at the caller's current point v (between runs the current point is
+// always set 0), `reps` times: ms per run. It changes nothing a pass reads before writing it again (bucket counts end
+// at zero, rho is rewritten).
+int Level::time_density(int reps, hipStream_t s, float& ms) {
+    const Pick pos{(const float2*)d.v, nullptr, nullptr};
+    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)hist_lds));
+    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)acc_lds));
+    AP_HIP(hipEventRecord(t0, s));
+    for (int r = 0; r < reps; r++) {
+        hipLaunchKernelGGL(eng_bin_count, dim3(fj.blocks + jj.blocks), dim3(EB), hist_lds, s, pos, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_bin_scan, dim3(d.has_judge ? 2 : 1), dim3(SCAN_T), 0, s, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_bin_scatter, dim3(fj.blocks + jj.blocks), dim3(EB), 0, s, pos, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.tiles + jj.tiles), dim3(TB), acc_lds, s, pos, fj, jj, d.target,
+                           p_over);
+        AP_LAUNCHED();
+    }
+    AP_HIP(hipEventRecord(t1, s));
+    AP_HIP(hipEventSynchronize(t1));
+    AP_HIP(hipEventElapsedTime(&ms, t0, t1));
+    ms /= reps;
+    return AP_OK;
+}
+
+// Autotuner of the tile sides: for each grid (force, then overflow), every admissible side from 8 to 64 is timed on
+// the density stage at the current positions and the fastest kept. The map does not depend on the side (integer
+// sums), so the choice changes the time only, never the results.
+int Level::autotune(int reps, hipStream_t s, int* sides) {
+    AP_HIP(hipDeviceSynchronize());
+    for (BinJob* J : {&fj, &jj}) {
+        if (!J->count) continue;
+        int best = J->T;
+        float best_ms = 1e30f;
+        for (int T = 8; T <= 64; T *= 2) {
+            if (!side_ok(*J, T)) continue;
+            set_side(*J, T);
+            update_lds();
+            float ms = 0;
+            int st = time_density(1, s, ms);                  // warm-up at this side
+            if (!st) st = time_density(reps, s, ms);
+            if (st) return st;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = T;
+            }
+        }
+        set_side(*J, best);
+        update_lds();
+    }
+    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)hist_lds));
+    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)acc_lds));
+    for (GraphEntry& G : graphs) {                    // the kernels' arguments change: capture again
+        if (last_ev == &G.ev) last_ev = nullptr;
+        G.clear();
+    }
+    if (sides) {
+        sides[0] = fj.T;
+        sides[1] = jj.count ? jj.T : 0;
+    }
+    AP_HIP(hipDeviceSynchronize());
     return AP_OK;
 }
 
@@ -816,7 +945,9 @@ int Level::create(const ap_level_desc& desc) {
                                (int)hist_lds));
     AP_HIP(hipFuncSetAttribute((const void*)eng_bin_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)acc_lds));
-    const long long n_over = fj.tiles + jj.tiles;
+    // one overflow partial per tile: room for the most tiles any side the autotuner may pick gives (a grid has at
+    // most BIN_MAX_BUCKETS tiles)
+    const long long n_over = 2LL * BIN_MAX_BUCKETS;
     const long long np = nblk_n + 2LL * nblk_wa + n_over + 3LL * nblk_n;
     AP_HIP(hipMalloc(&part, np * sizeof(double)));
     AP_HIP(hipMemset(part, 0, np * sizeof(double)));
@@ -862,38 +993,14 @@ void Level::destroy() {
         if (e) (void)hipEventDestroy(e);
 }
 
-// objective at pos with the parameters kp; g_prev (optional) gives |g' - g|. With S the control pass follows.
-int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev,
-                     const float2* g_prev_b, float2* g_out_a, float2* g_out_b, hipStream_t s, double* S, float* kp_out,
-                     int use_density) {
-    int st;
-    // WA
-    {
-        // the evaluation point: set 1 when the current point is set 0, and the reverse (resolved on the device)
-        WaArgs w{pos.a, d.pin_cell, (const float2*)d.pin_off, d.net_start, d.num_nets, d.num_mov, kp_ptr,
-                 pin_grad, nullptr, p_wa, pos.b, pos.flag, block_net, pin_slot};
-        mark(AP_PROF_WA, s);
-        if ((st = wa_plan.launch(w, p_wa, s))) return st;
-    }
-    int n_over = 0;
-    double over_scale = 0;
-    if (use_density) {
-        mark(AP_PROF_MAP, s);
-        hipLaunchKernelGGL(eng_bin_count, dim3(fj.blocks + jj.blocks), dim3(EB), hist_lds, s, pos, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_bin_scan, dim3(d.has_judge ? 2 : 1), dim3(SCAN_T), 0, s, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_bin_scatter, dim3(fj.blocks + jj.blocks), dim3(EB), 0, s, pos, fj, jj);
-        AP_LAUNCHED();
-        mark(AP_PROF_POST, s);
-        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.tiles + jj.tiles), dim3(TB), acc_lds, s, pos, fj, jj, d.target,
-                           p_over);
-        AP_LAUNCHED();
-        n_over = fj.tiles + jj.tiles;
-        over_scale = d.has_judge ? d.judge_bin_area / d.mov_area : d.bin_area / d.mov_area;
-        mark(AP_PROF_POISSON, s);
-        if ((st = poisson.solve(rho, d.nby, fx, fy, d.nby, s))) return st;
-    }
+WaArgs Level::wa_args(const Pick& pos, const float* kp_ptr) const {
+    // the evaluation point: set 1 when the current point is set 0, and the reverse (resolved on the device)
+    return WaArgs{pos.a, d.pin_cell, (const float2*)d.pin_off, d.net_start, d.num_nets, d.num_mov, kp_ptr,
+                  pin_grad, nullptr, p_wa, pos.b, pos.flag, block_net, pin_slot};
+}
+
+GatherArgs Level::gather_args(const Pick& pos, const float* kp_ptr, const float2* g_prev, const float2* g_prev_b,
+                              float2* g_out_a, float2* g_out_b, int use_density) const {
     GatherArgs g{};
     g.pos = pos;
     g.g_prev = g_prev;
@@ -919,31 +1026,66 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     g.nbx = d.nbx;
     g.nby = d.nby;
     g.use_density = use_density;
+    return g;
+}
+
+Segs Level::segs(bool with_state, bool has_prev, int use_density) const {
+    return Segs{p_moved, with_state ? nblk_n : 0, p_wa, nblk_wa, p_over, use_density ? fj.tiles + jj.tiles : 0,
+                p_turned, has_prev ? nblk_n : 0, p_wl2, p_dn2, nblk_n};
+}
+
+double Level::over_scale() const { return d.has_judge ? d.judge_bin_area / d.mov_area : d.bin_area / d.mov_area; }
+
+NestArgs Level::nest_args() const {
+    return NestArgs{bufs(), (const float2*)d.lo, (const float2*)d.hi, d.kp, d.num_nodes, p_moved};
+}
+
+Pick Level::pass_pos() const { return Pick{(const float2*)d.v, (const float2*)v1, flags}; }
+
+// objective at pos with the parameters kp; g_prev (optional) gives |g' - g|. With S the control pass follows.
+int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev,
+                     const float2* g_prev_b, float2* g_out_a, float2* g_out_b, hipStream_t s, double* S, float* kp_out,
+                     int use_density) {
+    int st;
+    mark(AP_PROF_WA, s);
+    if ((st = wa_plan.launch(wa_args(pos, kp_ptr), p_wa, s))) return st;
+    if (use_density) {
+        mark(AP_PROF_MAP, s);
+        hipLaunchKernelGGL(eng_bin_count, dim3(fj.blocks + jj.blocks), dim3(EB), hist_lds, s, pos, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_bin_scan, dim3(d.has_judge ? 2 : 1), dim3(SCAN_T), 0, s, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_bin_scatter, dim3(fj.blocks + jj.blocks), dim3(EB), 0, s, pos, fj, jj);
+        AP_LAUNCHED();
+        mark(AP_PROF_POST, s);
+        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.tiles + jj.tiles), dim3(TB), acc_lds, s, pos, fj, jj, d.target,
+                           p_over);
+        AP_LAUNCHED();
+        mark(AP_PROF_POISSON, s);
+        if ((st = poisson.solve(rho, d.nby, fx, fy, d.nby, s))) return st;
+    }
+    const GatherArgs g = gather_args(pos, kp_ptr, g_prev, g_prev_b, g_out_a, g_out_b, use_density);
     mark(AP_PROF_GATHER, s);
     hipLaunchKernelGGL(eng_gather, dim3(nblk_n), dim3(EB), 0, s, g, p_turned, p_wl2, p_dn2);
     AP_LAUNCHED();
     mark(AP_PROF_CONTROL, s);
-    Segs sg{p_moved, with_state ? nblk_n : 0, p_wa, nblk_wa, p_over, n_over, p_turned, g_prev ? nblk_n : 0,
-            p_wl2, p_dn2, nblk_n};
-    hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(EB), 0, s, sg, red);
+    hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(EB), 0, s, segs(with_state, g_prev != nullptr, use_density),
+                       red);
     AP_LAUNCHED();
     Ctl ctl_pass = ctl;
     if (!S) ctl_pass.params = nullptr;                       // a plain objective evaluation: no controller
-    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red, over_scale, S, kp_out, d.c, flags, stats,
-                       ctl_pass);
+    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red, use_density ? over_scale() : 0.0, S, kp_out, d.c,
+                       flags, stats, ctl_pass);
     AP_LAUNCHED();
     mark(AP_PROF_CLASSES, s);
     return AP_OK;
 }
 
 int Level::pass(hipStream_t s) {
-    Bufs B = bufs();
     mark(AP_PROF_NESTEROV, s);
-    hipLaunchKernelGGL(eng_nesterov, dim3(nblk_n), dim3(EB), 0, s, B, (const float2*)d.lo, (const float2*)d.hi, d.kp,
-                       d.num_nodes, p_moved);
+    hipLaunchKernelGGL(eng_nesterov, dim3(nblk_n), dim3(EB), 0, s, nest_args());
     AP_LAUNCHED();
-    Pick pos{(const float2*)d.v, (const float2*)v1, flags};
-    return objective(pos, d.kp, true, (const float2*)d.g, g1, (float2*)d.g, g1, s, d.S, d.kp, 1);
+    return objective(pass_pos(), d.kp, true, (const float2*)d.g, g1, (float2*)d.g, g1, s, d.S, d.kp, 1);
 }
 
 int Level::finalize(hipStream_t s) {
@@ -995,6 +1137,297 @@ int Level::graph_for(int passes, int& slot) {
     AP_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
     G.passes = passes;
     G.timing = timing_cls;
+    return AP_OK;
+}
+
+// ------------------------------------------------------------------------------ batch: many levels per launch
+// The passes of many independent levels (blocks) run as one sequence of grouped launches: every stage of the pass
+// is one launch over the workgroups of all blocks, each workgroup reading its block's arguments from a table on the
+// device through a work map (block, workgroup, workgroups of the block's stage). Each block keeps its own control
+// state; a block that has stopped (S[STOP] != 0) is skipped by every stage but the control (which freezes it) and
+// the end-of-run copy -- exactly what its own passes would leave. A run of passes is one HIP graph.
+struct PassTab {
+    NestArgs nest;
+    Pick pos;
+    BinJob fj, jj;
+    float target;
+    double* p_over;
+    GatherArgs gather;
+    double *p_turned, *p_wl2, *p_dn2;
+    Segs segs;
+    double* red;
+    double over_scale;
+    double* S;
+    float* kp;
+    const double* c;
+    int* flags;
+    double* stats;
+    Ctl ctl;
+    Bufs bufs;
+    int num_nodes;
+};
+
+enum { ST_NEST, ST_COUNT, ST_SCAN, ST_SCATTER, ST_TILES, ST_GATHER, ST_REDUCE, ST_CONTROL, ST_FINAL, ST_FLAGS, ST_WA,
+       NUM_ST };
+
+__device__ __forceinline__ bool stopped(const PassTab* tab, int b) { return tab[b].S[STOP] != 0; }
+
+__global__ __launch_bounds__(EB) void eng_nesterov_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    if (stopped(tab, m.x)) return;
+    const NestArgs a = tab[m.x].nest;
+    nesterov_body(a, m.y);
+}
+__global__ __launch_bounds__(EB) void eng_bin_count_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    if (stopped(tab, m.x)) return;
+    const PassTab& T = tab[m.x];
+    bin_count_body(T.pos, T.fj, T.jj, m.y);
+}
+__global__ __launch_bounds__(SCAN_T) void eng_bin_scan_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    if (stopped(tab, m.x)) return;
+    const PassTab& T = tab[m.x];
+    bin_scan_body(T.fj, T.jj, m.y);
+}
+__global__ __launch_bounds__(EB) void eng_bin_scatter_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    if (stopped(tab, m.x)) return;
+    const PassTab& T = tab[m.x];
+    bin_scatter_body(T.pos, T.fj, T.jj, m.y);
+}
+__global__ __launch_bounds__(TB) void eng_bin_tiles_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    if (stopped(tab, m.x)) return;
+    const PassTab& T = tab[m.x];
+    bin_tiles_body(T.pos, T.fj, T.jj, T.target, T.p_over, m.y);
+}
+__global__ __launch_bounds__(EB) void eng_gather_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    if (stopped(tab, m.x)) return;
+    const PassTab& T = tab[m.x];
+    const GatherArgs g = T.gather;
+    gather_body(g, T.p_turned, T.p_wl2, T.p_dn2, m.y, m.z);
+}
+__global__ __launch_bounds__(EB) void eng_reduce_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    if (stopped(tab, m.x)) return;
+    const PassTab& T = tab[m.x];
+    const Segs sg = T.segs;
+    reduce_body(sg, T.red, m.y);
+}
+__global__ __launch_bounds__(64) void eng_control_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    const PassTab& T = tab[m.x];
+    const Ctl ctl = T.ctl;
+    control_body(T.red, T.over_scale, T.S, T.kp, T.c, T.flags, T.stats, ctl);
+}
+__global__ __launch_bounds__(EB) void eng_finalize_copy_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    const PassTab& T = tab[m.x];
+    const Bufs B = T.bufs;
+    const int i = m.y * EB + threadIdx.x;
+    if (i >= T.num_nodes) return;
+    const int cur = B.flags[0];
+    if (B.flags[1]) B.best[i] = B.v[cur][i];
+    if (cur == 1) {
+        B.u[0][i] = B.u[1][i];
+        B.v[0][i] = B.v[1][i];
+        B.g[0][i] = B.g[1][i];
+    }
+}
+__global__ void eng_finalize_flags_grouped(const PassTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    if (threadIdx.x == 0) {
+        int* f = tab[m.x].flags;
+        f[0] = 0;
+        f[1] = 0;
+    }
+}
+
+struct Batch {
+    std::vector<Level*> levels;
+    PassTab* d_tab = nullptr;
+    WaveArgs* d_wave = nullptr;
+    FftTab* d_fft = nullptr;
+    double** d_S = nullptr;
+    int4* map[NUM_ST] = {};
+    int count[NUM_ST] = {};
+    FftGroup fft;
+    size_t hist_lds = 0, acc_lds = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr;
+    GraphEntry graphs[GRAPH_CACHE];
+    int graph_next = 0;
+
+    int build();          // tables and maps from the levels' current set-up
+    void release_tables();
+    void destroy();
+    int pass(hipStream_t s);
+    int finalize(hipStream_t s);
+    int graph_for(int passes, int& slot);
+};
+
+void Batch::release_tables() {
+    void* ptrs[] = {d_tab, d_wave, d_fft, d_S};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    d_tab = nullptr;
+    d_wave = nullptr;
+    d_fft = nullptr;
+    d_S = nullptr;
+    for (int k = 0; k < NUM_ST; k++) {
+        if (map[k]) (void)hipFree(map[k]);
+        map[k] = nullptr;
+        count[k] = 0;
+    }
+    fft.destroy();
+    for (GraphEntry& G : graphs) G.clear();
+}
+
+int Batch::build() {
+    release_tables();
+    const int nb = (int)levels.size();
+    std::vector<PassTab> tab(nb);
+    std::vector<WaveArgs> wave(nb);
+    std::vector<FftTab> ffts(nb);
+    std::vector<double*> S(nb);
+    std::vector<int4> m[NUM_ST];
+    hist_lds = acc_lds = 0;
+    for (int b = 0; b < nb; b++) {
+        Level& L = *levels[b];
+        AP_REQUIRE(L.poisson.method == AP_POISSON_FFT, "batch: block %d does not use the FFT Poisson solve", b);
+        AP_REQUIRE(L.wa_plan.num_big == 0, "batch: block %d has nets of more than 64 pins", b);
+        const Pick pos = L.pass_pos();
+        PassTab& T = tab[b];
+        T.nest = L.nest_args();
+        T.pos = pos;
+        T.fj = L.fj;
+        T.jj = L.jj;
+        T.target = L.d.target;
+        T.p_over = L.p_over;
+        T.gather = L.gather_args(pos, L.d.kp, (const float2*)L.d.g, L.g1, (float2*)L.d.g, L.g1, 1);
+        T.p_turned = L.p_turned;
+        T.p_wl2 = L.p_wl2;
+        T.p_dn2 = L.p_dn2;
+        T.segs = L.segs(true, true, 1);
+        T.red = L.red;
+        T.over_scale = L.over_scale();
+        T.S = L.d.S;
+        T.kp = L.d.kp;
+        T.c = L.d.c;
+        T.flags = L.flags;
+        T.stats = L.stats;
+        T.ctl = L.ctl;
+        T.bufs = L.bufs();
+        T.num_nodes = L.d.num_nodes;
+        wave[b] = L.wa_plan.wave_args(L.wa_args(pos, L.d.kp), L.p_wa);
+        ffts[b] = L.poisson.fft_tab(L.rho, L.d.nby, L.fx, L.fy, L.d.nby, L.d.S);
+        S[b] = L.d.S;
+        hist_lds = std::max(hist_lds, L.hist_lds);
+        acc_lds = std::max(acc_lds, L.acc_lds);
+        auto add = [&](int st, int n) {
+            for (int k = 0; k < n; k++) m[st].push_back(make_int4(b, k, n, 0));
+        };
+        add(ST_NEST, L.nblk_n);
+        add(ST_COUNT, L.fj.blocks + L.jj.blocks);
+        add(ST_SCAN, L.d.has_judge ? 2 : 1);
+        add(ST_SCATTER, L.fj.blocks + L.jj.blocks);
+        add(ST_TILES, L.fj.tiles + L.jj.tiles);
+        add(ST_GATHER, L.nblk_n);
+        add(ST_REDUCE, RED_BLOCKS);
+        add(ST_CONTROL, 1);
+        add(ST_FINAL, L.nblk_n);
+        add(ST_FLAGS, 1);
+        add(ST_WA, L.wa_plan.wave_blocks());
+    }
+    AP_HIP(hipMalloc(&d_tab, nb * sizeof(PassTab)));
+    AP_HIP(hipMemcpy(d_tab, tab.data(), nb * sizeof(PassTab), hipMemcpyHostToDevice));
+    AP_HIP(hipMalloc(&d_wave, nb * sizeof(WaveArgs)));
+    AP_HIP(hipMemcpy(d_wave, wave.data(), nb * sizeof(WaveArgs), hipMemcpyHostToDevice));
+    AP_HIP(hipMalloc(&d_fft, nb * sizeof(FftTab)));
+    AP_HIP(hipMemcpy(d_fft, ffts.data(), nb * sizeof(FftTab), hipMemcpyHostToDevice));
+    AP_HIP(hipMalloc(&d_S, nb * sizeof(double*)));
+    AP_HIP(hipMemcpy(d_S, S.data(), nb * sizeof(double*), hipMemcpyHostToDevice));
+    for (int k = 0; k < NUM_ST; k++) {
+        count[k] = (int)m[k].size();
+        if (!count[k]) continue;
+        AP_HIP(hipMalloc(&map[k], m[k].size() * sizeof(int4)));
+        AP_HIP(hipMemcpy(map[k], m[k].data(), m[k].size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    int st = fft.create(ffts);
+    if (st) return st;
+    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_count_grouped, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)hist_lds));
+    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_tiles_grouped, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)acc_lds));
+    return AP_OK;
+}
+
+void Batch::destroy() {
+    release_tables();
+    for (Level* L : levels) {
+        L->destroy();
+        delete L;
+    }
+    levels.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+    hipEvent_t evs[] = {ev_in, ev_out, t0, t1};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+}
+
+int Batch::pass(hipStream_t s) {
+    int st;
+#define AP_GROUPED(kernel, stage, threads, lds)                                                       \
+    if (count[stage]) {                                                                            \
+        hipLaunchKernelGGL(kernel, dim3(count[stage]), dim3(threads), lds, s, d_tab, map[stage]);  \
+        AP_LAUNCHED();                                                                             \
+    }
+    AP_GROUPED(eng_nesterov_grouped, ST_NEST, EB, 0);
+    if ((st = launch_wa_grouped(d_wave, map[ST_WA], count[ST_WA], d_S, s))) return st;
+    AP_GROUPED(eng_bin_count_grouped, ST_COUNT, EB, hist_lds);
+    AP_GROUPED(eng_bin_scan_grouped, ST_SCAN, SCAN_T, 0);
+    AP_GROUPED(eng_bin_scatter_grouped, ST_SCATTER, EB, 0);
+    AP_GROUPED(eng_bin_tiles_grouped, ST_TILES, TB, acc_lds);
+    if ((st = fft.launch(d_fft, s))) return st;
+    AP_GROUPED(eng_gather_grouped, ST_GATHER, EB, 0);
+    AP_GROUPED(eng_reduce_grouped, ST_REDUCE, EB, 0);
+    AP_GROUPED(eng_control_grouped, ST_CONTROL, 64, 0);
+    return AP_OK;
+}
+
+int Batch::finalize(hipStream_t s) {
+    AP_GROUPED(eng_finalize_copy_grouped, ST_FINAL, EB, 0);
+    AP_GROUPED(eng_finalize_flags_grouped, ST_FLAGS, 64, 0);
+#undef AP_GROUPED
+    return AP_OK;
+}
+
+int Batch::graph_for(int passes, int& slot) {
+    for (int k = 0; k < GRAPH_CACHE; k++)
+        if (graphs[k].exec && graphs[k].passes == passes) {
+            slot = k;
+            return AP_OK;
+        }
+    slot = graph_next;
+    graph_next = (graph_next + 1) % GRAPH_CACHE;
+    GraphEntry& G = graphs[slot];
+    G.clear();
+    AP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int st = AP_OK;
+    for (int k = 0; k < passes && !st; k++) st = pass(stream);
+    if (!st) st = finalize(stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &g);
+    if (st || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (st) return st;
+        AP_HIP(e);
+    }
+    G.graph = g;
+    AP_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
+    G.passes = passes;
     return AP_OK;
 }
 
@@ -1189,6 +1622,114 @@ int ap_level_set_controller(ap_level* level, const float* params, float sigma, u
     L.ctl.window = window;
     AP_HIP(hipMemset(L.ctl_x, 0, NUM_X * sizeof(double)));
     AP_HIP(hipDeviceSynchronize());
+    return AP_OK;
+}
+
+struct ap_batch {
+    Batch B;
+};
+
+int ap_batch_create(const ap_level_desc* blocks, int count, ap_batch** batch) {
+    AP_REQUIRE(blocks && batch && count > 0, "batch_create: bad arguments");
+    ap_batch* h = new ap_batch();
+    Batch& B = h->B;
+    int st = AP_OK;
+    for (int b = 0; b < count && !st; b++) {
+        Level* L = new Level();
+        st = L->create(blocks[b]);
+        if (st) {
+            L->destroy();
+            delete L;
+            break;
+        }
+        B.levels.push_back(L);
+    }
+    if (!st) st = B.build();
+    if (!st) {
+        const hipError_t e = hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking);
+        if (e == hipSuccess && hipEventCreateWithFlags(&B.ev_in, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&B.ev_out, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreate(&B.t0) == hipSuccess && hipEventCreate(&B.t1) == hipSuccess) {
+            st = AP_OK;
+        } else {
+            set_error("batch_create: stream / events");
+            st = AP_ERR_HIP;
+        }
+    }
+    if (st) {
+        B.destroy();
+        delete h;
+        return st;
+    }
+    *batch = h;
+    return AP_OK;
+}
+
+int ap_batch_destroy(ap_batch* batch) {
+    if (batch) {
+        (void)hipDeviceSynchronize();
+        batch->B.destroy();
+        delete batch;
+    }
+    return AP_OK;
+}
+
+int ap_batch_prepare(ap_batch* batch, int passes) {
+    AP_REQUIRE(batch && passes > 0, "batch_prepare: bad arguments");
+    int slot;
+    return batch->B.graph_for(passes, slot);
+}
+
+int ap_batch_run(ap_batch* batch, int passes, int use_graph, void* stream) {
+    AP_REQUIRE(batch && passes >= 0, "batch_run: bad arguments");
+    Batch& B = batch->B;
+    hipStream_t caller = (hipStream_t)stream;
+    int st = AP_OK, slot = -1;
+    if (use_graph && passes > 0 && (st = B.graph_for(passes, slot))) return st;
+    AP_HIP(hipEventRecord(B.ev_in, caller));
+    AP_HIP(hipStreamWaitEvent(B.stream, B.ev_in, 0));
+    AP_HIP(hipEventRecord(B.t0, B.stream));
+    if (slot >= 0) {
+        AP_HIP(hipGraphLaunch(B.graphs[slot].exec, B.stream));
+    } else {
+        for (int k = 0; k < passes && !st; k++) st = B.pass(B.stream);
+        if (!st) st = B.finalize(B.stream);
+        if (st) return st;
+    }
+    AP_HIP(hipEventRecord(B.t1, B.stream));
+    AP_HIP(hipEventRecord(B.ev_out, B.stream));
+    AP_HIP(hipStreamWaitEvent(caller, B.ev_out, 0));
+    return AP_OK;
+}
+
+int ap_batch_last_ms(ap_batch* batch, float* ms) {
+    AP_REQUIRE(batch && ms, "batch_last_ms: null argument");
+    AP_HIP(hipEventSynchronize(batch->B.t1));
+    AP_HIP(hipEventElapsedTime(ms, batch->B.t0, batch->B.t1));
+    return AP_OK;
+}
+
+int ap_level_autotune(ap_level* level, int reps, int* sides) {
+    AP_REQUIRE(level && reps > 0, "level_autotune: bad arguments");
+    Level& L = level->L;
+    return L.autotune(reps, L.stream, sides);
+}
+
+int ap_batch_autotune(ap_batch* batch, int reps, int* sides) {
+    AP_REQUIRE(batch && reps > 0, "batch_autotune: bad arguments");
+    Batch& B = batch->B;
+    for (size_t b = 0; b < B.levels.size(); b++) {
+        int two[2];
+        const int st = B.levels[b]->autotune(reps, B.stream, two);
+        if (st) return st;
+        if (sides) sides[b] = two[0];
+    }
+    return B.build();
+}
+
+int ap_batch_tile_sides(ap_batch* batch, int* sides) {
+    AP_REQUIRE(batch && sides, "batch_tile_sides: null argument");
+    for (size_t b = 0; b < batch->B.levels.size(); b++) sides[b] = batch->B.levels[b]->fj.T;
     return AP_OK;
 }
 

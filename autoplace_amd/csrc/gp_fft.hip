@@ -14,6 +14,7 @@
 // reversed coefficients. Each pass reads and writes every element once; the map stays unmirrored.
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "gp_common.h"
@@ -30,16 +31,42 @@ __device__ __forceinline__ int bitrev(int n, int lg) { return (int)(__brev((unsi
 // position of input element m in Makhoul's order: even m -> m / 2, odd m -> L - 1 - (m - 1) / 2
 __device__ __forceinline__ int makhoul(int m, int L) { return (m & 1) ? L - 1 - (m >> 1) : (m >> 1); }
 
-// in-place radix-2 decimation in time on bit-reversed input; tw[j] = e^{-2 pi i j / L} (j < L / 2) in LDS;
-// inverse: conjugate twiddles (no 1 / L scale)
+// in-place decimation in time on bit-reversed input; tw[j] = e^{-2 pi i j / L} (j < L / 2) in LDS; inverse:
+// conjugate twiddles (no 1 / L scale). Two radix-2 stages (half-sizes h and 2h) are done together as one radix-4
+// butterfly in registers on x[i], x[i+h], x[i+2h], x[i+3h]: one barrier and one LDS round trip per stage pair; an odd
+// stage count ends with one radix-2 stage.
 __device__ void fft_lds(float2* s, const float2* tw, int L, int lg, bool inverse) {
-    for (int st = 1; st <= lg; st++) {
+    const float sg = inverse ? -1.f : 1.f;
+    int st = 1;
+    for (; st + 1 <= lg; st += 2) {
+        const int h = 1 << (st - 1);
+        const int s1 = L >> st, s2 = L >> (st + 1);           // twiddle strides of the two stages
+        for (int b = threadIdx.x; b < (L >> 2); b += blockDim.x) {
+            const int j = b & (h - 1);
+            const int i0 = ((b >> (st - 1)) << (st + 1)) + j;
+            float2 w1 = tw[j * s1], w2 = tw[j * s2];
+            w1.y *= sg;
+            w2.y *= sg;
+            // W_{4h}^{j+h} = W_{4h}^j * W_4^1, and W_4^1 = -i (forward) or +i (inverse)
+            const float2 w3 = make_float2(sg * w2.y, -sg * w2.x);
+            const float2 a0 = s[i0], a1 = cmul(s[i0 + h], w1), a2 = s[i0 + 2 * h], a3 = cmul(s[i0 + 3 * h], w1);
+            const float2 b0 = make_float2(a0.x + a1.x, a0.y + a1.y), b1 = make_float2(a0.x - a1.x, a0.y - a1.y);
+            const float2 b2 = cmul(make_float2(a2.x + a3.x, a2.y + a3.y), w2);
+            const float2 b3 = cmul(make_float2(a2.x - a3.x, a2.y - a3.y), w3);
+            s[i0] = make_float2(b0.x + b2.x, b0.y + b2.y);
+            s[i0 + 2 * h] = make_float2(b0.x - b2.x, b0.y - b2.y);
+            s[i0 + h] = make_float2(b1.x + b3.x, b1.y + b3.y);
+            s[i0 + 3 * h] = make_float2(b1.x - b3.x, b1.y - b3.y);
+        }
+        __syncthreads();
+    }
+    if (st == lg) {
         const int half = 1 << (st - 1), stride = L >> st;
         for (int b = threadIdx.x; b < (L >> 1); b += blockDim.x) {
             const int j = b & (half - 1);
             const int i0 = ((b >> (st - 1)) << st) + j, i1 = i0 + half;
             float2 w = tw[j * stride];
-            if (inverse) w.y = -w.y;
+            w.y *= sg;
             const float2 a = s[i0], c = cmul(s[i1], w);
             s[i0] = make_float2(a.x + c.x, a.y + c.y);
             s[i1] = make_float2(a.x - c.x, a.y - c.y);
@@ -69,16 +96,21 @@ __device__ void dct2_pair_to_lds(const float* in0, const float* in1, float2* s, 
     }
 }
 
-__global__ __launch_bounds__(FB) void fft_pass_a(const float* rho, int ld_in, float* T, int ld_out, const float2* tw,
-                                                 const float2* ph, int L, int lg) {
+__device__ __forceinline__ void pass_a_body(const float* rho, int ld_in, float* T, int ld_out, const float2* tw,
+                                            const float2* ph, int L, int lg, int wg) {
     extern __shared__ float2 lds[];
     float2* s = lds;
     float2* tw_s = lds + L;
     load_twiddles(tw_s, tw, L);
-    const int r0 = 2 * blockIdx.x;
+    const int r0 = 2 * wg;
     __syncthreads();
     dct2_pair_to_lds(rho + (long long)r0 * ld_in, rho + (long long)(r0 + 1) * ld_in, s, tw_s, ph, L, lg,
                      T + (long long)r0 * ld_out, T + (long long)(r0 + 1) * ld_out);
+}
+
+__global__ __launch_bounds__(FB) void fft_pass_a(const float* rho, int ld_in, float* T, int ld_out, const float2* tw,
+                                                 const float2* ph, int L, int lg) {
+    pass_a_body(rho, ld_in, T, ld_out, tw, ph, L, lg, blockIdx.x);
 }
 
 // cosine series of a_cos and sine series of a_sin (both real, length L, in LDS) from one inverse FFT:
@@ -117,7 +149,7 @@ struct PassB {
     const float2* ph;
 };
 
-__global__ __launch_bounds__(FB) void fft_pass_b(PassB a) {
+__device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
     extern __shared__ float2 lds[];
     const int L = a.N;
     float2* s = lds;
@@ -127,7 +159,7 @@ __global__ __launch_bounds__(FB) void fft_pass_b(PassB a) {
     float* ac = c1 + L;
     float* as = ac + L;
     load_twiddles(tw_s, a.tw, L);
-    const int l0 = 2 * blockIdx.x;
+    const int l0 = 2 * wg;
     __syncthreads();
     dct2_pair_to_lds(a.Tt + (long long)l0 * L, a.Tt + (long long)(l0 + 1) * L, s, tw_s, a.ph, L, a.lgN, c0, c1);
     __syncthreads();
@@ -147,15 +179,16 @@ __global__ __launch_bounds__(FB) void fft_pass_b(PassB a) {
     }
 }
 
-__global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V, float* fx, float* fy, int ld_out,
-                                                 const float2* tw, const float2* ph, int L, int lg) {
+__global__ __launch_bounds__(FB) void fft_pass_b(PassB a) { pass_b_body(a, blockIdx.x); }
+
+__device__ __forceinline__ void pass_c_body(const float* U, const float* V, float* fx, float* fy, int ld_out,
+                                            const float2* tw, const float2* ph, int L, int lg, int n) {
     extern __shared__ float2 lds[];
     float2* s = lds;
     float2* tw_s = lds + L;
     float* uc = (float*)(tw_s + L / 2);
     float* vs = uc + L;
     load_twiddles(tw_s, tw, L);
-    const int n = blockIdx.x;
     for (int j = threadIdx.x; j < L; j += blockDim.x) {
         uc[j] = U[(long long)n * L + j];
         vs[j] = V[(long long)n * L + j];
@@ -164,14 +197,19 @@ __global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V,
     cos_sin_series(uc, vs, s, tw_s, ph, L, lg, fx + (long long)n * ld_out, fy + (long long)n * ld_out);
 }
 
+__global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V, float* fx, float* fy, int ld_out,
+                                                 const float2* tw, const float2* ph, int L, int lg) {
+    pass_c_body(U, V, fx, fy, ld_out, tw, ph, L, lg, blockIdx.x);
+}
+
 // out[c][r] = in[r][c] for 2 planes (blockIdx.z), 32 x 32 tiles through LDS
 constexpr int TT = 32;
-__global__ __launch_bounds__(256) void transpose_kernel(const float* in0, const float* in1, float* out0, float* out1,
-                                                        int R, int C) {
+__device__ __forceinline__ void transpose_body(const float* in0, const float* in1, float* out0, float* out1, int R,
+                                               int C, int bx, int by, int bz) {
     __shared__ float tile[TT][TT + 1];
-    const float* in = blockIdx.z ? in1 : in0;
-    float* out = blockIdx.z ? out1 : out0;
-    const int c0 = blockIdx.x * TT, r0 = blockIdx.y * TT;
+    const float* in = bz ? in1 : in0;
+    float* out = bz ? out1 : out0;
+    const int c0 = bx * TT, r0 = by * TT;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
     for (int k = ty; k < TT; k += 8) {
         const int r = r0 + k, c = c0 + tx;
@@ -182,6 +220,128 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* in0, const 
         const int c = c0 + k, r = r0 + tx;
         if (r < R && c < C) out[(long long)c * R + r] = tile[tx][k];
     }
+}
+
+__global__ __launch_bounds__(256) void transpose_kernel(const float* in0, const float* in1, float* out0, float* out1,
+                                                        int R, int C) {
+    transpose_body(in0, in1, out0, out1, R, C, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// ------------------------------------------------------------------------------------------------ grouped launches
+// Many grids (of any power-of-two sizes) per launch: workgroup b does map[b] = (grid, x, y, z) of its grid's pass.
+__device__ __forceinline__ bool fft_skip(const FftTab& t) { return t.S && t.S[STOP] != 0; }
+
+__global__ __launch_bounds__(FB) void fft_a_grouped(const FftTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    const FftTab t = tab[m.x];
+    if (fft_skip(t)) return;
+    pass_a_body(t.rho, t.rho_ld, t.w0, t.M, t.twM, t.phM, t.M, t.lgM, m.y);
+}
+__global__ __launch_bounds__(256) void fft_t1_grouped(const FftTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    const FftTab t = tab[m.x];
+    if (fft_skip(t)) return;
+    transpose_body(t.w0, t.w0, t.w1, t.w1, t.N, t.M, m.y, m.z, 0);
+}
+__global__ __launch_bounds__(FB) void fft_b_grouped(const FftTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    const FftTab t = tab[m.x];
+    if (fft_skip(t)) return;
+    const PassB b{t.w1, t.w2, t.w3, t.omega, t.nu, t.scale, t.bx, t.N, t.lgN, t.twN, t.phN};
+    pass_b_body(b, m.y);
+}
+__global__ __launch_bounds__(256) void fft_t2_grouped(const FftTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    const FftTab t = tab[m.x];
+    if (fft_skip(t)) return;
+    transpose_body(t.w2, t.w3, t.w0, t.w1, t.M, t.N, m.y, m.z, m.w);
+}
+__global__ __launch_bounds__(FB) void fft_c_grouped(const FftTab* tab, const int4* map) {
+    const int4 m = map[blockIdx.x];
+    const FftTab t = tab[m.x];
+    if (fft_skip(t)) return;
+    pass_c_body(t.w0, t.w1, t.fx, t.fy, t.fld, t.twM, t.phM, t.M, t.lgM, m.y);
+}
+
+int FftGroup::create(const std::vector<FftTab>& tabs) {
+    std::vector<int4> m[5];
+    for (int g = 0; g < (int)tabs.size(); g++) {
+        const FftTab& t = tabs[g];
+        AP_REQUIRE(t.N >= 2 && t.M >= 2 && (t.N & (t.N - 1)) == 0 && (t.M & (t.M - 1)) == 0,
+                   "fft group: grid %d is not a power of two", g);
+        for (int y = 0; y < t.N / 2; y++) m[0].push_back(make_int4(g, y, 0, 0));
+        for (int y = 0; y < (int)grid_of(t.N, TT); y++)
+            for (int x = 0; x < (int)grid_of(t.M, TT); x++) m[1].push_back(make_int4(g, x, y, 0));
+        for (int y = 0; y < t.M / 2; y++) m[2].push_back(make_int4(g, y, 0, 0));
+        for (int z = 0; z < 2; z++)
+            for (int y = 0; y < (int)grid_of(t.M, TT); y++)
+                for (int x = 0; x < (int)grid_of(t.N, TT); x++) m[3].push_back(make_int4(g, x, y, z));
+        for (int y = 0; y < t.N; y++) m[4].push_back(make_int4(g, y, 0, 0));
+        lds_ac = std::max(lds_ac, (size_t)(t.M + t.M / 2) * sizeof(float2) + 2 * t.M * sizeof(float));
+        lds_b = std::max(lds_b, (size_t)(t.N + t.N / 2) * sizeof(float2) + 4 * t.N * sizeof(float));
+    }
+    for (int k = 0; k < 5; k++) {
+        count[k] = (int)m[k].size();
+        if (!count[k]) continue;
+        AP_HIP(hipMalloc(&map[k], m[k].size() * sizeof(int4)));
+        AP_HIP(hipMemcpy(map[k], m[k].data(), m[k].size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    return AP_OK;
+}
+
+void FftGroup::destroy() {
+    for (int k = 0; k < 5; k++) {
+        if (map[k]) (void)hipFree(map[k]);
+        map[k] = nullptr;
+        count[k] = 0;
+    }
+}
+
+int FftGroup::launch(const FftTab* d_tab, hipStream_t s) const {
+    if (!count[0]) return AP_OK;
+    hipLaunchKernelGGL(fft_a_grouped, dim3(count[0]), dim3(FB), lds_ac, s, d_tab, map[0]);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(fft_t1_grouped, dim3(count[1]), dim3(256), 0, s, d_tab, map[1]);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(fft_b_grouped, dim3(count[2]), dim3(FB), lds_b, s, d_tab, map[2]);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(fft_t2_grouped, dim3(count[3]), dim3(256), 0, s, d_tab, map[3]);
+    AP_LAUNCHED();
+    hipLaunchKernelGGL(fft_c_grouped, dim3(count[4]), dim3(FB), lds_ac, s, d_tab, map[4]);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+FftTab Poisson::fft_tab(const float* rho, int rho_ld, float* fx, float* fy, int fld, const double* S) const {
+    const size_t nm = (size_t)N * M;
+    FftTab t{};
+    t.S = S;
+    t.rho = rho;
+    t.rho_ld = rho_ld;
+    t.fx = fx;
+    t.fy = fy;
+    t.fld = fld;
+    t.w0 = d_work;
+    t.w1 = d_work + nm;
+    t.w2 = d_work + 2 * nm;
+    t.w3 = d_work + 3 * nm;
+    t.twN = d_tw;
+    t.phN = d_tw + N / 2;
+    t.twM = d_tw + N / 2 + N;
+    t.phM = t.twM + M / 2;
+    t.omega = d_omega;
+    t.nu = d_omega + N;
+    t.N = N;
+    t.M = M;
+    int l = 0;
+    while ((1 << l) < N) l++;
+    t.lgN = l;
+    l = 0;
+    while ((1 << l) < M) l++;
+    t.lgM = l;
+    t.scale = (float)(4.0 / ((double)N * M));
+    t.bx = (float)bx;
+    return t;
 }
 
 // ------------------------------------------------------------------------------------------------ the plan

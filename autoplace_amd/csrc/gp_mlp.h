@@ -2,7 +2,7 @@
 // the logs of factors on the density-weight multiplier and on the smoothing of each pass (ScheduleAdjust). Its
 // features are per-pass statistics of the run (mlp_features). It has no counterpart in the reference, whose rule it
 // adjusts (gp_cuda_kernel.cu:198-209); with zero outputs, or after `window` passes without an overflow improvement
-// (the fallback), the rule runs unchanged.
+// once the density share is 0.95 or more (the fallback), the rule runs unchanged.
 #pragma once
 #include "gp_common.h"
 #include "gp_control.h"
@@ -26,7 +26,8 @@ struct Ctl {
     unsigned long long seed;
     float* log;                 // null or [log_cap][MLP_LOG]
     int log_cap;
-    int window;                 // fallback after this many passes without a 0.1 % overflow improvement
+    int window;                 // fallback after this many passes without a 0.1 % overflow improvement once
+                                // the density share is 0.95 or more (the rule's own stall test starts there)
     int hold;                   // the exploration draw changes every `hold` learned passes
     float phase;                // 1: a coarse (cluster) level, 0: the cell level
 };

@@ -8,6 +8,31 @@ namespace ap {
 
 void poisson_weights(int N, int M, double bx, double by, std::vector<float>& wx, std::vector<float>& wy);
 
+// One grid's FFT solve as an entry of the argument table of grouped launches (many grids per launch): the buffers
+// of a Poisson plan of the FFT method and the map / field it is applied to.
+struct FftTab {
+    const double* S;             // control state of the grid's block: skipped once S[STOP] != 0 (null: never)
+    const float* rho;
+    int rho_ld;
+    float *fx, *fy;
+    int fld;
+    float *w0, *w1, *w2, *w3;
+    const float2 *twN, *phN, *twM, *phM;
+    const float *omega, *nu;
+    int N, M, lgN, lgM;
+    float scale, bx;
+};
+
+// work maps of the five grouped FFT launches: per workgroup (grid, x, y, z)
+struct FftGroup {
+    int4* map[5] = {};
+    int count[5] = {};
+    size_t lds_ac = 0, lds_b = 0;
+    int create(const std::vector<FftTab>& tabs);
+    void destroy();
+    int launch(const FftTab* d_tab, hipStream_t s) const;
+};
+
 struct Poisson {
     int N = 0, M = 0, method = 0;
     double bx = 1, by = 1;
@@ -31,6 +56,7 @@ struct Poisson {
     int fft_create();
     void fft_destroy();
     int fft_solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s);
+    FftTab fft_tab(const float* rho, int rho_ld, float* fx, float* fy, int fld, const double* S) const;
 };
 
 }  // namespace ap

@@ -6,6 +6,22 @@
 
 namespace ap {
 
+struct WaveArgs {
+    const float2* pos;
+    const float2* pos_other;
+    const int* flag;
+    const unsigned char* lg;   // per wavefront: log2 of its lanes per net
+    const int4* lanes;         // per lane: cell (-1: none), gradient slot, offset x, y
+    int num_waves;
+    int num_mov;
+    const float* gamma;
+    float2* pin_grad;
+    double* partial;           // per workgroup (WA, HPWL)
+};
+
+// grouped launch of many levels' WA (tables on the device; see wa_wave_grouped)
+int launch_wa_grouped(const WaveArgs* d_tab, const int4* d_map, int count, const double* const* d_S, hipStream_t s);
+
 struct WaPlan {
     int num_waves = 0, num_big = 0;
     unsigned char* d_lg = nullptr;   // per wavefront: log2 of its lanes per net
@@ -19,6 +35,7 @@ struct WaPlan {
     int wave_blocks() const { return (num_waves + 3) / 4; }
     int partial_count() const { return wave_blocks() + num_big; }   // (WA, HPWL) pairs written per launch
     int launch(const WaArgs& a, double* partial, hipStream_t s) const;
+    WaveArgs wave_args(const WaArgs& a, double* partial) const;
 };
 
 // Work item of workgroup b among nb when consecutive workgroups land on the 8 XCDs in turn: XCD x gets a contiguous

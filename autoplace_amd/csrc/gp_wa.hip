@@ -19,19 +19,6 @@
 
 namespace ap {
 
-struct WaveArgs {
-    const float2* pos;
-    const float2* pos_other;
-    const int* flag;
-    const unsigned char* lg;   // per wavefront: log2 of its lanes per net
-    const int4* lanes;         // per lane: cell (-1: none), gradient slot, offset x, y
-    int num_waves;
-    int num_mov;
-    const float* gamma;
-    float2* pin_grad;
-    double* partial;           // per workgroup (WA, HPWL)
-};
-
 // DPP lane permutations (gfx9 encodings of the dpp_ctrl operand)
 constexpr int DPP_QUAD_XOR1 = 0xB1;        // quad_perm [1, 0, 3, 2]
 constexpr int DPP_QUAD_XOR2 = 0x4E;        // quad_perm [2, 3, 0, 1]
@@ -86,14 +73,17 @@ __device__ __forceinline__ void wa_group(const WaveArgs& a, const float2* pos, c
     const float lx = group_reduce<LG>(live ? x.x : POS, MinOp()), ly = group_reduce<LG>(live ? x.y : POS, MinOp());
     const float epx = live ? expf((x.x - hx) * ig) : 0.f, epy = live ? expf((x.y - hy) * ig) : 0.f;
     const float enx = live ? expf((lx - x.x) * ig) : 0.f, eny = live ? expf((ly - x.y) * ig) : 0.f;
-    const float spx = group_reduce<LG>(epx, AddOp()), spy = group_reduce<LG>(epy, AddOp());
-    const float snx = group_reduce<LG>(enx, AddOp()), sny = group_reduce<LG>(eny, AddOp());
-    const float xpx = group_reduce<LG>(x.x * epx, AddOp()) / spx, xpy = group_reduce<LG>(x.y * epy, AddOp()) / spy;
-    const float xnx = group_reduce<LG>(x.x * enx, AddOp()) / snx, xny = group_reduce<LG>(x.y * eny, AddOp()) / sny;
+    // the sums are >= 1 (the extreme pin contributes e^0): hardware reciprocals (1 ulp) replace the 8 divisions
+    const float rpx = __builtin_amdgcn_rcpf(group_reduce<LG>(epx, AddOp()));
+    const float rpy = __builtin_amdgcn_rcpf(group_reduce<LG>(epy, AddOp()));
+    const float rnx = __builtin_amdgcn_rcpf(group_reduce<LG>(enx, AddOp()));
+    const float rny = __builtin_amdgcn_rcpf(group_reduce<LG>(eny, AddOp()));
+    const float xpx = group_reduce<LG>(x.x * epx, AddOp()) * rpx, xpy = group_reduce<LG>(x.y * epy, AddOp()) * rpy;
+    const float xnx = group_reduce<LG>(x.x * enx, AddOp()) * rnx, xny = group_reduce<LG>(x.y * eny, AddOp()) * rny;
     if (live) {
         float2 gr;
-        gr.x = epx * (1 + (x.x - xpx) * ig) / spx - enx * (1 - (x.x - xnx) * ig) / snx;
-        gr.y = epy * (1 + (x.y - xpy) * ig) / spy - eny * (1 - (x.y - xny) * ig) / sny;
+        gr.x = epx * (1 + (x.x - xpx) * ig) * rpx - enx * (1 - (x.x - xnx) * ig) * rnx;
+        gr.y = epy * (1 + (x.y - xpy) * ig) * rpy - eny * (1 - (x.y - xny) * ig) * rny;
         if (slot >= 0) a.pin_grad[slot] = gr;
         if (sub == 0) {
             acc[0] = (double)((xpx - xnx) + (xpy - xny));
@@ -102,10 +92,10 @@ __device__ __forceinline__ void wa_group(const WaveArgs& a, const float2* pos, c
     }
 }
 
-__global__ __launch_bounds__(256) void wa_wave_kernel(WaveArgs a) {
+__device__ __forceinline__ void wa_wave_body(const WaveArgs& a, int wg, int nwg) {
     __shared__ double red[16 * 2];
     const int lane = threadIdx.x & 63;
-    const int w = xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+    const int w = xcd_remap(wg, nwg) * 4 + (threadIdx.x >> 6);
     const float2* pos = a.flag ? (*a.flag == 0 ? a.pos_other : a.pos) : a.pos;
     double acc[2] = {0.0, 0.0};
     if (w < a.num_waves) {
@@ -121,9 +111,32 @@ __global__ __launch_bounds__(256) void wa_wave_kernel(WaveArgs a) {
     }
     block_sum<2>(acc, red);
     if (threadIdx.x == 0) {
-        a.partial[2 * blockIdx.x] = acc[0];
-        a.partial[2 * blockIdx.x + 1] = acc[1];
+        a.partial[2 * wg] = acc[0];
+        a.partial[2 * wg + 1] = acc[1];
     }
+}
+
+__global__ __launch_bounds__(256) void wa_wave_kernel(WaveArgs a) { wa_wave_body(a, blockIdx.x, gridDim.x); }
+
+// many levels per launch: workgroup b does map[b] = (level, workgroup, workgroups of the level, 0); a level whose
+// control state says stopped (S[STOP] != 0) is skipped
+__global__ __launch_bounds__(256) void wa_wave_grouped(const WaveArgs* tab, const int4* map,
+                                                       const double* const* S) {
+    const int4 m = map[blockIdx.x];
+    if (S[m.x][STOP] != 0) return;
+    const WaveArgs a = tab[m.x];
+    wa_wave_body(a, m.y, m.z);
+}
+
+int launch_wa_grouped(const WaveArgs* d_tab, const int4* d_map, int count, const double* const* d_S, hipStream_t s) {
+    if (!count) return AP_OK;
+    hipLaunchKernelGGL(wa_wave_grouped, dim3(count), dim3(256), 0, s, d_tab, d_map, d_S);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+WaveArgs WaPlan::wave_args(const WaArgs& a, double* partial) const {
+    return WaveArgs{a.pos, a.pos_other, a.flag, d_lg, d_lanes, num_waves, a.num_mov, a.gamma, a.pin_grad, partial};
 }
 
 // nets of more than 64 pins: one workgroup per net, the drop-in kernel's workgroup-wide path
@@ -225,7 +238,7 @@ void WaPlan::destroy() {
 
 int WaPlan::launch(const WaArgs& a, double* partial, hipStream_t s) const {
     if (num_waves) {
-        WaveArgs w{a.pos, a.pos_other, a.flag, d_lg, d_lanes, num_waves, a.num_mov, a.gamma, a.pin_grad, partial};
+        const WaveArgs w = wave_args(a, partial);
         hipLaunchKernelGGL(wa_wave_kernel, dim3(wave_blocks()), dim3(256), 0, s, w);
         AP_LAUNCHED();
     }

@@ -33,6 +33,54 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def setup_level(obj, wirelength, density, u, v, g, best, S, kp, c, poisson="auto", reorder=True):
+    """The ap_level_desc of a level (obj.desc), with obj holding the tensors it points to (obj._keep), the spatial
+    renumbering (obj.perm; None without reorder) and the engine's copies of u / v / g / best (obj.u ...; the
+    caller's in obj.user)."""
+    w, d, j = wirelength.opt, density, density.judge
+    obj.user = (u, v, g, best)
+    n, N = d.num_mov, d.num_nodes
+    if reorder:
+        perm = hilbert_order(v, d.nbx, d.nby, float(d.bin_np[0]), float(d.bin_np[1]), n)
+        net = renumber_netlist(w["cell32"], w["off"], w["net_start"], n, perm)
+    else:
+        perm = None
+        net = w
+    pm = (lambda t: t[perm].contiguous()) if reorder else (lambda t: t)           # noqa: E731
+    pc = (lambda t: t[perm[:n]].contiguous()) if reorder else (lambda t: t)       # noqa: E731
+    obj.perm = perm
+    obj.u, obj.v, obj.g, obj.best = pm(u), pm(v), pm(g), pm(best)
+    arrays = dict(size=pm(d.smooth["size"]), ratio=pm(d.smooth["ratio"]), pins=pm(d.pins), area=pm(d.area),
+                  lo=pm(d.pos_lo), hi=pm(d.pos_hi))
+    obj._keep = [net["cell32"], net["off"], net["net_start"], net["cell_start"], net["cell_pin"], obj.u, obj.v,
+                 obj.g, obj.best, S, kp, c, d.fixed_map] + list(arrays.values())
+    for t in obj._keep:
+        if not t.is_contiguous():
+            raise ValueError("engine tensors must be contiguous")
+    desc = _lib.LevelDesc()
+    desc.num_nodes, desc.num_mov = N, n
+    desc.num_pins, desc.num_nets = net["cell32"].numel(), net["num_nets"]
+    desc.pin_cell, desc.pin_off, desc.net_start = _p(net["cell32"]), _p(net["off"]), _p(net["net_start"])
+    desc.cell_start, desc.cell_pin = _p(net["cell_start"]), _p(net["cell_pin"])
+    desc.size, desc.ratio = _p(arrays["size"]), _p(arrays["ratio"])
+    desc.pins, desc.area, desc.lo, desc.hi = _p(arrays["pins"]), _p(arrays["area"]), _p(arrays["lo"]), \
+        _p(arrays["hi"])
+    desc.nbx, desc.nby = d.nbx, d.nby
+    desc.bx, desc.by = float(d.bin_np[0]), float(d.bin_np[1])
+    desc.fixed, desc.target = _p(d.fixed_map), float(d.target)
+    desc.bin_area, desc.mov_area = d.bin_area, d.mov_area
+    if j is not None:
+        js, jr = pc(j.smooth["size"]), pc(j.smooth["ratio"])
+        obj._keep += [js, jr, j.fixed_map]
+        desc.has_judge, desc.njx, desc.njy = 1, j.nbx, j.nby
+        desc.jbx, desc.jby, desc.judge_bin_area = float(j.bin_np[0]), float(j.bin_np[1]), j.bin_area
+        desc.judge_size, desc.judge_ratio, desc.judge_fixed = _p(js), _p(jr), _p(j.fixed_map)
+    desc.poisson_method = {"auto": 0, "gemm": 1, "fft": 2}[poisson]
+    desc.u, desc.v, desc.g, desc.best = _p(obj.u), _p(obj.v), _p(obj.g), _p(obj.best)
+    desc.S, desc.kp, desc.c = _p(S), _p(kp), _p(c)
+    obj.desc = desc
+
+
 class Engine(object):
     """A level's problem bound to the native GP iteration engine (ap_level).
 
@@ -41,55 +89,21 @@ class Engine(object):
     every tensor it hands to the library alive."""
 
     def __init__(self, wirelength, density, u, v, g, best, S, kp, c, poisson="auto", reorder=True):
-        w, d, j = wirelength.opt, density, density.judge
-        self.user = (u, v, g, best)
-        n, N = d.num_mov, d.num_nodes
-        if reorder:
-            perm = hilbert_order(v, d.nbx, d.nby, float(d.bin_np[0]), float(d.bin_np[1]), n)
-            net = renumber_netlist(w["cell32"], w["off"], w["net_start"], n, perm)
-        else:
-            perm = None
-            net = w
-        pm = (lambda t: t[perm].contiguous()) if reorder else (lambda t: t)           # noqa: E731
-        pc = (lambda t: t[perm[:n]].contiguous()) if reorder else (lambda t: t)       # noqa: E731
-        self.perm = perm
-        self.u, self.v, self.g, self.best = pm(u), pm(v), pm(g), pm(best)
-        arrays = dict(size=pm(d.smooth["size"]), ratio=pm(d.smooth["ratio"]), pins=pm(d.pins), area=pm(d.area),
-                      lo=pm(d.pos_lo), hi=pm(d.pos_hi))
-        self._keep = [net["cell32"], net["off"], net["net_start"], net["cell_start"], net["cell_pin"], self.u, self.v,
-                      self.g, self.best, S, kp, c, d.fixed_map] + list(arrays.values())
-        for t in self._keep:
-            if not t.is_contiguous():
-                raise ValueError("engine tensors must be contiguous")
-        desc = _lib.LevelDesc()
-        desc.num_nodes, desc.num_mov = N, n
-        desc.num_pins, desc.num_nets = net["cell32"].numel(), net["num_nets"]
-        desc.pin_cell, desc.pin_off, desc.net_start = _p(net["cell32"]), _p(net["off"]), _p(net["net_start"])
-        desc.cell_start, desc.cell_pin = _p(net["cell_start"]), _p(net["cell_pin"])
-        desc.size, desc.ratio = _p(arrays["size"]), _p(arrays["ratio"])
-        desc.pins, desc.area, desc.lo, desc.hi = _p(arrays["pins"]), _p(arrays["area"]), _p(arrays["lo"]), \
-            _p(arrays["hi"])
-        desc.nbx, desc.nby = d.nbx, d.nby
-        desc.bx, desc.by = float(d.bin_np[0]), float(d.bin_np[1])
-        desc.fixed, desc.target = _p(d.fixed_map), float(d.target)
-        desc.bin_area, desc.mov_area = d.bin_area, d.mov_area
-        if j is not None:
-            js, jr = pc(j.smooth["size"]), pc(j.smooth["ratio"])
-            self._keep += [js, jr, j.fixed_map]
-            desc.has_judge, desc.njx, desc.njy = 1, j.nbx, j.nby
-            desc.jbx, desc.jby, desc.judge_bin_area = float(j.bin_np[0]), float(j.bin_np[1]), j.bin_area
-            desc.judge_size, desc.judge_ratio, desc.judge_fixed = _p(js), _p(jr), _p(j.fixed_map)
-        desc.poisson_method = {"auto": 0, "gemm": 1, "fft": 2}[poisson]
-        desc.u, desc.v, desc.g, desc.best = _p(self.u), _p(self.v), _p(self.g), _p(self.best)
-        desc.S, desc.kp, desc.c = _p(S), _p(kp), _p(c)
-        self.desc = desc
+        setup_level(self, wirelength, density, u, v, g, best, S, kp, c, poisson, reorder)
         h = ctypes.c_void_p()
-        _lib.check(_lib.lib().ap_level_create(ctypes.byref(desc), ctypes.byref(h)), "level_create")
+        _lib.check(_lib.lib().ap_level_create(ctypes.byref(self.desc), ctypes.byref(h)), "level_create")
         self._h = h
 
     def run(self, passes, use_graph=True):
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         _lib.check(_lib.lib().ap_level_run(self._h, passes, int(use_graph), stream), "level_run")
+
+    def autotune(self, reps=5):
+        """Time every admissible density tile side at the current point and keep the fastest (results unchanged).
+        -> (force-grid side, overflow-grid side or 0)."""
+        sides = (ctypes.c_int * 2)()
+        _lib.check(_lib.lib().ap_level_autotune(self._h, int(reps), sides), "level_autotune")
+        return sides[0], sides[1]
 
     def finish(self):
         """Write u / v / g / best back into the caller's tensors (caller's node order)."""
@@ -186,12 +200,14 @@ def control_state(ps, optimizer, iteration, args):
 
 
 def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every=16, use_graph=True,
-                     poisson="auto", reorder=True, hook=None, controller=None, phase=0.0):
+                     poisson="auto", reorder=True, hook=None, controller=None, phase=0.0, autotune=None):
     """Continue the level from `iteration` (density weight on) until it stops; updates optimizer, ps and
     objective.evals as the host loop would. -> (reason, iterations, hpwl, overflow) of the kept placement.
     hook(objective, u, v, g, S, kp, c), if given, sees the state the device loop starts from (instrumentation).
     controller: None (the plain rule) or a dict for the learned controller (controller.py: `attach`), whose "levels"
-    list receives this level's record (passes under the controller, fallback, the logged passes)."""
+    list receives this level's record (passes under the controller, fallback, the logged passes).
+    autotune: time the density tile sides at the start and keep the fastest (None: for levels of 100k nodes and
+    more); the results do not depend on it."""
     device, pos = optimizer.v.device, optimizer.v
     S, kp, c = control_state(ps, optimizer, iteration, args)
     S = torch.tensor(S, dtype=torch.float64, device=device)
@@ -203,6 +219,8 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
         hook(objective, u, pos, g, S, kp, c)
     best = pos.clone()
     eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
+    if autotune or (autotune is None and objective.density.num_nodes >= 100000):
+        eng.autotune(3)
     ctl_log = None
     if controller is not None:
         ctl_log = eng.set_controller(controller, phase)

@@ -179,6 +179,29 @@ AP_API int ap_level_objective(ap_level* level, const float* pos, float* grad, do
  * ap_level_objective): the top-left quadrant of the reference's density_map result. */
 AP_API int ap_level_density(ap_level* level, float* rho, void* stream);
 
+/* Autotuner of the density tile sides: every admissible side (8 .. 64 bins) of the force grid, then of the overflow
+ * grid, timed on the density stage at the caller's current point (reps runs each); the fastest is kept.
+ * sides i32[2] (optional): the chosen sides (0: no overflow grid). The density map does not depend on the side, so
+ * the results of later passes are the same whatever the choice. Drops the level's captured graphs. */
+AP_API int ap_level_autotune(ap_level* level, int reps, int* sides);
+
+/* ---------------------------------------------------------------- batch: many independent levels per launch
+ * No counterpart in the reference. count levels (blocks) described as for ap_level_create (each with its own
+ * buffers and control state S / kp / c; the FFT Poisson method and nets of at most 64 pins) run their passes
+ * together: every stage of a pass is one launch over all blocks' workgroups, and a run of passes replays one HIP
+ * graph. A block that has stopped is skipped, so each block's result is the one ap_level_run gives it. */
+typedef struct ap_batch ap_batch;
+AP_API int ap_batch_create(const ap_level_desc* blocks, int count, ap_batch** batch);
+AP_API int ap_batch_destroy(ap_batch* batch);
+AP_API int ap_batch_prepare(ap_batch* batch, int passes);
+AP_API int ap_batch_run(ap_batch* batch, int passes, int use_graph, void* stream);
+/* ms between the first and last kernel of the last ap_batch_run */
+AP_API int ap_batch_last_ms(ap_batch* batch, float* ms);
+/* ap_level_autotune for every block (force-grid side of block b into sides[b], optional) */
+AP_API int ap_batch_autotune(ap_batch* batch, int reps, int* sides);
+/* the force-grid tile side of every block: sides i32[count] */
+AP_API int ap_batch_tile_sides(ap_batch* batch, int* sides);
+
 /* ---------------------------------------------------------------- part 4: learned schedule controller
  * No counterpart in the reference: a 12 -> 64 -> 64 -> 2 ReLU MLP (FP32, parameters f32[ap_mlp_param_count()] laid
  * out W1[64][12], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]) whose outputs, clipped to +-0.2, are the logs of
@@ -205,7 +228,8 @@ AP_API int ap_mlp_adam(float* params, float* m, float* v, const float* grad, int
  * to the outputs before the clip (counter-based draws from seed, a new draw every `hold` learned passes); phase: 1
  * for a coarse level; log: null or
  * f32[log_cap][16] receiving, per learned pass, the 12 features, the 2 applied outputs, overflow and HPWL; window:
- * the plain rule takes over for the rest of the level after `window` passes without a 0.1 % overflow improvement.
+ * the plain rule takes over for the rest of the level after `window` passes without a 0.1 % overflow improvement,
+ * counted once the density share lambda A / (pins + lambda A) is 0.95 or more (where the rule's stall test counts).
  * Resets the controller's state and drops the level's captured graphs. */
 AP_API int ap_level_set_controller(ap_level* level, const float* params, float sigma, unsigned long long seed,
                                    int hold, float phase, float* log, int log_cap, int window);

@@ -128,8 +128,11 @@ def controller_pass(S, kp, c, X, p, moved, turned, hpwl, ovf, wa, wl2, dn2, phas
     octl.control_pass(S, kp, c, moved, turned, hpwl, ovf, wa, lam_f, gam_f)
     if S[K["accept"]] != 0:
         X[X_PREV_OVF] = ovf
-        if ovf < X[X_BEST_OVF] * 0.999:
-            X[X_BEST_OVF], X[X_LAST_IMPROVE] = ovf, S[K["iter"]]
+        better = ovf < X[X_BEST_OVF] * 0.999
+        if better:
+            X[X_BEST_OVF] = ovf
+        if better or S[K["ww"]] < 0.95:
+            X[X_LAST_IMPROVE] = S[K["iter"]]
         elif S[K["iter"]] - X[X_LAST_IMPROVE] >= window:
             X[X_FALLBACK] = 1
     return x, o

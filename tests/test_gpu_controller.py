@@ -108,13 +108,17 @@ def test_logged_passes_match_the_oracle_network(small_design):
 
 
 def test_fallback_hands_over_to_the_rule(small_design):
-    """A window of 3 passes without an overflow improvement: the controller stops after it and the rule finishes."""
+    """A network that adds 0.2 to the log density-weight growth every pass (its bias) drives the density share past
+    0.95, where stalls count; then the first pass without a 0.1 % overflow improvement (window 1) hands the level
+    over to the rule, which finishes it."""
     from autoplace_amd import controller
     data = small_design[2]
-    rng = np.random.default_rng(7)
-    p = rng.normal(0, 0.05, omlp.PARAMS).astype(np.float32)
-    ctl = controller.options(_t(p), window=3, log_cap=4096)
+    p = np.zeros(omlp.PARAMS, np.float32)
+    p[omlp.B3] = 0.2
+    ctl = controller.options(_t(p), window=1, log_cap=4096)
     _, m = _place(data, ctl)
+    print("fallback test: %d iterations, %s; levels %s" % (m["iters"], m["stop_reason"],
+                                                           [(lv["passes"], lv["fallback"]) for lv in ctl["levels"]]))
     assert any(lv["fallback"] for lv in ctl["levels"])
     assert m["stop_reason"] == "converged"
     total = sum(lv["passes"] for lv in ctl["levels"])

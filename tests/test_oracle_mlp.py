@@ -45,7 +45,8 @@ def test_zero_network_is_the_plain_rule():
 
 
 def test_fallback_after_a_stalled_overflow():
-    """With the overflow stuck the controller hands over to the rule after `window` passes."""
+    """With the overflow stuck the controller hands over to the rule after `window` passes -- counted once the
+    density share is 0.95 or more, as the rule's stall test."""
     K = octl.K
     S = [0.0] * len(octl.FIELDS)
     S[K["lam"]], S[K["gam"]], S[K["alpha"]], S[K["a"]], S[K["iter"]] = 1e-3, 20.0, 0.5, 3.0, 10
@@ -56,10 +57,15 @@ def test_fallback_after_a_stalled_overflow():
     rng = np.random.default_rng(1)
     p = rng.normal(0, 0.05, omlp.PARAMS)
     learned = 0
+    for i in range(80):                            # density share 0.0025: no stall is counted
+        x, o = omlp.controller_pass(S, kp, c, X, p, 3.0, 2.0, 1e6, 0.5, 9e5, 4.0, 1.0, window=50)
+        learned += x is not None
+    assert X[omlp.X_FALLBACK] == 0 and learned == 80
+    S[K["lam"]] = 10.0                              # share 0.96: the clock runs
     for i in range(80):
         x, o = omlp.controller_pass(S, kp, c, X, p, 3.0, 2.0, 1e6, 0.5, 9e5, 4.0, 1.0, window=50)
         learned += x is not None
-    assert X[omlp.X_FALLBACK] == 1 and learned == 50 and X[omlp.X_COUNT] == 50
+    assert X[omlp.X_FALLBACK] == 1 and learned == 80 + 50 and X[omlp.X_COUNT] == 130
 
 
 def test_normal_draws_are_standard_normal():
