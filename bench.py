@@ -147,6 +147,8 @@ def gpu_passes(snap, steps, warmup):
     c[9] = 1e12                                      # inner_iter: no max_iter stop inside the timed window
     u, v, g = snap["u"].clone(), snap["v"].clone(), snap["g"].clone()
     eng = gpu_loop.Engine(wl, den, u, v, g, v.clone(), S, kp, c)
+    sides = eng.autotune(5)                          # as the flow does for levels of 100k nodes and more
+    print("density tile sides (autotuned): force grid %d, overflow grid %d" % sides, file=sys.stderr, flush=True)
     prof = eng.profile(max(warmup, 1))               # the warm-up passes, per kernel class
     dom = max(SINGLE_KERNEL, key=lambda k: prof[k])
     eng.time_class(dom)
