@@ -16,7 +16,7 @@ D = ctypes.c_double
 
 AP_OK, AP_ERR_ARG, AP_ERR_HIP, AP_ERR_UNSUPPORTED = 0, 1, 2, 3
 AP_POISSON_AUTO, AP_POISSON_GEMM, AP_POISSON_FFT = 0, 1, 2
-PROF_CLASSES = ("nesterov", "wa", "density_bin", "density_tiles", "poisson", "gather", "control")
+PROF_CLASSES = ("nesterov", "wa", "density_bin", "density_tiles", "overflow_tiles", "poisson", "gather", "control")
 
 _SIGNATURES = {
     "ap_last_error": ([], ctypes.c_char_p),
@@ -51,13 +51,14 @@ _SIGNATURES = {
     "ap_level_set_controller": ([P, P, F, ctypes.c_ulonglong, I, F, P, I, I], I),
     "ap_level_controller_state": ([P, ctypes.POINTER(D)], I),
     "ap_level_autotune": ([P, I, P], I),
+    "ap_level_set_tile_layout": ([P, I, I, I, I], I),
     "ap_batch_create": ([P, I, ctypes.POINTER(P)], I),
     "ap_batch_destroy": ([P], I),
     "ap_batch_prepare": ([P, I], I),
     "ap_batch_run": ([P, I, I, P], I),
     "ap_batch_last_ms": ([P, ctypes.POINTER(F)], I),
     "ap_batch_autotune": ([P, I, P], I),
-    "ap_batch_tile_sides": ([P, P], I),
+    "ap_batch_tile_layout": ([P, P], I),
 }
 
 

@@ -122,17 +122,18 @@ def run_sequential(states, check_every=16, max_passes=20000):
 
 
 def run_batched(states, check_every=16, max_passes=20000, autotune=False, reps=5):
-    """All blocks' device loops together (ap_batch). -> dict(blocks, seconds, positions, tile_sides)."""
+    """All blocks' device loops together (ap_batch). -> dict(blocks, seconds, positions, tile_layout: per block
+    [side, tiles per workgroup])."""
     blocks = [_Block(st) for st in states]
     descs = (_lib.LevelDesc * len(blocks))(*[b.desc for b in blocks])
     h = ctypes.c_void_p()
     _lib.check(_lib.lib().ap_batch_create(descs, len(blocks), ctypes.byref(h)), "batch_create")
     lib = _lib.lib()
-    sides = (ctypes.c_int * len(blocks))()
+    lay = (ctypes.c_int * (2 * len(blocks)))()
     try:
         if autotune:
-            _lib.check(lib.ap_batch_autotune(h, int(reps), sides), "batch_autotune")
-        _lib.check(lib.ap_batch_tile_sides(h, sides), "batch_tile_sides")
+            _lib.check(lib.ap_batch_autotune(h, int(reps), lay), "batch_autotune")
+        _lib.check(lib.ap_batch_tile_layout(h, lay), "batch_tile_layout")
         _lib.check(lib.ap_batch_prepare(h, check_every), "batch_prepare")
         stop_idx = gpu_loop.K["stop"]
         S_all = [b.S for b in blocks]
@@ -162,4 +163,5 @@ def run_batched(states, check_every=16, max_passes=20000, autotune=False, reps=5
         r["completion_s"] = t
         out.append(r)
         pos.append(b.user[1].cpu().numpy())          # v in the caller's node order
-    return {"blocks": out, "seconds": seconds, "positions": pos, "tile_sides": list(sides)}
+    return {"blocks": out, "seconds": seconds, "positions": pos,
+            "tile_layout": [[lay[2 * k], lay[2 * k + 1]] for k in range(len(blocks))]}

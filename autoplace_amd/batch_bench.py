@@ -6,7 +6,7 @@ of the same device loops on the same GPU.
 Both start from every block's state at the start of its device loop (batch.start_state: host preparation and the
 wirelength-only start run first, untimed). Reported: aggregate iterations per second (all blocks' passes / wall
 time), per-block completion time p50 / p99 (from the start of the timed region; the batch sees a block's stop at
-its next 16-pass check), the autotuned tile sides, and whether every block's final state and placement is the same
+its next 16-pass check), the autotuned tile layouts, and whether every block's final state and placement is the same
 bit for bit in both runs (but for the last pass's accept / is_best flags, which every pass after a stop resets).
 """
 import argparse
@@ -52,7 +52,7 @@ def main(argv=None):
     out = {"metric": "aggregate iterations/s, %d blocks" % len(blocks), "blocks": len(blocks),
            "cells": [b.num_mov for b in blocks], "grids": [batch.block_grid(b.num_mov) for b in blocks],
            "iterations": iters, "stops": sorted({r["stop"] for r in seq["blocks"]}),
-           "sequential": summary(seq), "batched": summary(bat), "tile_sides": bat["tile_sides"],
+           "sequential": summary(seq), "batched": summary(bat), "tile_layout": bat["tile_layout"],
            "identical_results": bool(same)}
     out["speedup"] = out["batched"]["iterations_per_s"] / out["sequential"]["iterations_per_s"]
     line = json.dumps(out)

@@ -99,8 +99,10 @@ __global__ __launch_bounds__(EB) void eng_nesterov(NestArgs a) { nesterov_body(a
 //                    goes to the grid's list of big nodes
 //   eng_bin_scan     bucket starts (exclusive scan of the counts); the counts are cleared for the next pass
 //   eng_bin_scatter  each node's centre, size and ratio into its buckets
-//   eng_bin_tiles    one workgroup per tile: the contributions of the bucket's nodes and of the big nodes, clipped to
-//                    the tile and summed with LDS integer atomics; then the tile's density and overflow partial sum
+//   eng_bin_tiles    per tile (G tiles per workgroup): the contributions of the bucket's nodes and of the big nodes,
+//                    clipped to the tile and summed with LDS integer atomics; then the tile's density and overflow
+//                    partial sum
+// (eng_over_* : the same for the overflow grid, in launches of their own)
 // A bucket holds the nodes of one tile and one plane (cells, fillers). The contributions are the reference's 2^32
 // fixed-point integers (contribution() below), so the sums, and the map, do not depend on the order in which the
 // buckets were filled.
@@ -116,6 +118,8 @@ struct BinJob {
     float inv_bx, inv_by;
     int nbx, nby;
     int T, tiles_y, tiles;     // tile side in bins, tiles per column, tiles
+    int G, wgs;                // tiles per workgroup of the tile kernel (1, 2, 4, 8: TB / G lanes each), its
+                               // workgroups
     int planes;                // 2: cells + fillers, 1: cells only
     int blocks;                // workgroups of the count and scatter kernels for this grid
     int* count;                // [planes tiles] bucket sizes of the pass; zero between passes
@@ -195,8 +199,14 @@ __device__ __forceinline__ void bin_count_body(const Pick& pos_p, const BinJob& 
         J.off[i] = o;
     }
 }
+// The single-level kernels of the binning run one grid per launch: eng_bin_* the force grid (f.blocks / 1 / f.wgs
+// workgroups), eng_over_* the overflow grid (j.blocks / 1 / j.wgs), so that the two can run side by side and each
+// launch reserves the LDS of its own grid only.
 __global__ __launch_bounds__(EB) void eng_bin_count(Pick pos_p, BinJob f, BinJob j) {
     bin_count_body(pos_p, f, j, xcd_remap(blockIdx.x, gridDim.x));
+}
+__global__ __launch_bounds__(EB) void eng_over_count(Pick pos_p, BinJob f, BinJob j) {
+    bin_count_body(pos_p, f, j, f.blocks + xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // one workgroup per grid
@@ -230,7 +240,8 @@ __device__ __forceinline__ void bin_scan_body(const BinJob& f, const BinJob& j, 
         *J.big_count = 0;
     }
 }
-__global__ __launch_bounds__(SCAN_T) void eng_bin_scan(BinJob f, BinJob j) { bin_scan_body(f, j, blockIdx.x); }
+__global__ __launch_bounds__(SCAN_T) void eng_bin_scan(BinJob f, BinJob j) { bin_scan_body(f, j, 0); }
+__global__ __launch_bounds__(SCAN_T) void eng_over_scan(BinJob f, BinJob j) { bin_scan_body(f, j, 1); }
 
 __device__ __forceinline__ void bin_scatter_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk) {
     const bool judge = blk >= f.blocks;
@@ -273,6 +284,9 @@ __device__ __forceinline__ void bin_scatter_body(const Pick& pos_p, const BinJob
 __global__ __launch_bounds__(EB) void eng_bin_scatter(Pick pos_p, BinJob f, BinJob j) {
     bin_scatter_body(pos_p, f, j, xcd_remap(blockIdx.x, gridDim.x));
 }
+__global__ __launch_bounds__(EB) void eng_over_scatter(Pick pos_p, BinJob f, BinJob j) {
+    bin_scatter_body(pos_p, f, j, f.blocks + xcd_remap(blockIdx.x, gridDim.x));
+}
 
 // a node's contributions to the bins of the tile [X0, X1] x [Y0, Y1], into the tile's LDS accumulator a (row stride T)
 __device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0, int Y0, int X1, int Y1,
@@ -290,32 +304,40 @@ __device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0
 
 constexpr int TB = 512;   // lanes per workgroup of the tile kernel: a crowded tile's bucket spreads over 8 waves
 
+// One workgroup sums J.G tiles (G = 1, 2, 4, 8), each with its own TB / G lanes (whole waves) and its own LDS
+// accumulator, so that small tiles do not each hold a workgroup (and the launch's LDS reservation) to themselves.
+// The sub-groups pass the same barriers; a sub-group past the last tile only takes part in them.
 __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& f, const BinJob& j, float target,
                                                double* part, int wg) {
-    extern __shared__ unsigned long long acc[];   // planes x T x T
-    __shared__ double red[16];
-    const bool judge = wg >= f.tiles;
+    extern __shared__ unsigned long long acc_all[];   // G x planes x T x T
+    __shared__ double red[TB / 64];
+    const bool judge = wg >= f.wgs;
     const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
-    const int t = wg - (judge ? f.tiles : 0);
+    const int S = TB / J.G;            // lanes per tile
+    const int sub = threadIdx.x / S, l = threadIdx.x - sub * S;
+    const int t = (wg - (judge ? f.wgs : 0)) * J.G + sub;
+    const bool live = t < J.tiles;
     const int T = J.T, TT = T * T;
     const int X0 = (t / J.tiles_y) * T, Y0 = (t % J.tiles_y) * T;
     const int X1 = min(X0 + T, J.nbx) - 1, Y1 = min(Y0 + T, J.nby) - 1;
-    for (int e = threadIdx.x; e < J.planes * TT; e += TB) acc[e] = 0ull;
+    unsigned long long* acc = acc_all + sub * J.planes * TT;
+    if (live)
+        for (int e = l; e < J.planes * TT; e += S) acc[e] = 0ull;
     __syncthreads();
-    for (int pl = 0; pl < J.planes; pl++) {
+    for (int pl = 0; live && pl < J.planes; pl++) {
         const int b = pl * J.tiles + t;
         const int e1 = J.start[b + 1];
         unsigned long long* a = acc + pl * TT;
         // two entries per lane per round, both loaded before either is used
-        for (int e = J.start[b] + threadIdx.x; e < e1; e += 2 * TB) {
-            const bool two = e + TB < e1;
+        for (int e = J.start[b] + l; e < e1; e += 2 * S) {
+            const bool two = e + S < e1;
             const float4 g0 = J.geo[e];
             const float r0 = J.gr[e];
             float4 g1 = g0;
             float r1 = 0.f;
             if (two) {
-                g1 = J.geo[e + TB];
-                r1 = J.gr[e + TB];
+                g1 = J.geo[e + S];
+                r1 = J.gr[e + S];
             }
             add_clipped(a, T, X0, Y0, X1, Y1, make_span(g0.x, g0.z, J.inv_bx, J.nbx),
                         make_span(g0.y, g0.w, J.inv_by, J.nby), r0);
@@ -324,8 +346,8 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
                             make_span(g1.y, g1.w, J.inv_by, J.nby), r1);
         }
     }
-    // big nodes: one at a time, their bins in this tile spread over the lanes
-    const int nbig = *J.big_n;
+    // big nodes: one at a time, their bins in this tile spread over the tile's lanes
+    const int nbig = live ? *J.big_n : 0;
     if (nbig) {
         const float2* pos = pos_p.get(true);
         for (int k = 0; k < nbig; k++) {
@@ -333,11 +355,11 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
             const float2 p = pos[i], sz = J.size[i];
             const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
             const int x0 = max(sx.b0, X0), x1 = min(sx.b1, X1), y0 = max(sy.b0, Y0), y1 = min(sy.b1, Y1);
-            if (x0 > x1 || y0 > y1) continue;                   // uniform over the workgroup
+            if (x0 > x1 || y0 > y1) continue;                   // uniform over the tile's lanes
             const float r = J.ratio[i];
             unsigned long long* a = acc + (i >= J.num_mov ? TT : 0);
             const int h = y1 - y0 + 1, n = (x1 - x0 + 1) * h;
-            for (int e = threadIdx.x; e < n; e += TB) {
+            for (int e = l; e < n; e += S) {
                 const int bx = x0 + e / h, by = y0 + e % h;
                 const unsigned long long c = contribution(r * span_overlap(sx, bx), sy, by);
                 if (c) atomicAdd(a + (bx - X0) * T + (by - Y0), c);
@@ -345,21 +367,31 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
         }
     }
     __syncthreads();
-    double o[1] = {0.0};
-    for (int e = threadIdx.x; e < TT; e += TB) {
+    double o = 0.0;
+    for (int e = l; live && e < TT; e += S) {
         const int bx = X0 + e / T, by = Y0 + e % T;
         if (bx > X1 || by > Y1) continue;
         const long long bin = (long long)bx * J.nby + by;
         const float mov = (float)((double)acc[e] / kDensityScale) + J.fixed[bin];
         const float r = J.planes == 2 ? mov + (float)((double)acc[TT + e] / kDensityScale) : mov;
         if (J.rho) J.rho[bin] = r;
-        if (J.count_over) o[0] += (double)fmaxf(mov - target, 0.f);
+        if (J.count_over) o += (double)fmaxf(mov - target, 0.f);
     }
-    block_sum<1>(o, red);
-    if (threadIdx.x == 0) part[wg] = o[0];
+    // the tile's overflow: wave sums, then the tile's waves in order (G = 1: the order of block_sum)
+    o = wave_sum(o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = o;
+    __syncthreads();
+    if (live && l == 0) {
+        double s = 0;
+        for (int w = 0; w < S / 64; w++) s += red[sub * (S / 64) + w];
+        part[(judge ? f.tiles : 0) + t] = s;
+    }
 }
 __global__ __launch_bounds__(TB) void eng_bin_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
     bin_tiles_body(pos_p, f, j, target, part, blockIdx.x);
+}
+__global__ __launch_bounds__(TB) void eng_over_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
+    bin_tiles_body(pos_p, f, j, target, part, f.wgs + blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------ gather (+ cell WA sum, |g' - g|)
@@ -680,7 +712,8 @@ struct Level {
     int* block_net = nullptr;  // WA workgroup -> first owned net
     int* pin_slot = nullptr;   // pin -> position in the cell CSR (-1: fixed pin)
     BinJob fj{}, jj{};         // density binning of the force grid and of the overflow grid
-    size_t hist_lds = 0, acc_lds = 0;
+    size_t hist_f = 0, hist_j = 0, acc_f = 0, acc_j = 0;   // dynamic LDS of the count / tile kernels per grid
+    size_t hist_lds = 0, acc_lds = 0;                       // the larger of the two (the batch's grouped launches)
     std::vector<void*> bin_mem;
     float *rho = nullptr, *fx = nullptr, *fy = nullptr;
     double* part = nullptr;
@@ -693,7 +726,8 @@ struct Level {
     double *p_moved = nullptr, *p_wa = nullptr, *p_over = nullptr, *p_turned = nullptr, *p_wl2 = nullptr,
            *p_dn2 = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr;
+    hipStream_t side = nullptr;                             // WA and the overflow grid, beside the force grid
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr, ev_fork = nullptr, ev_join = nullptr;
     GraphEntry graphs[GRAPH_CACHE];
     int graph_next = 0;
     // event timing of one kernel class: events before and after it in every pass of a run (graph or direct)
@@ -709,13 +743,17 @@ struct Level {
     bool prof = false;
     hipEvent_t pev[AP_PROF_CLASSES + 1] = {};
     double prof_ms[AP_PROF_CLASSES] = {};
-    void mark(int k, hipStream_t s) {
+    void mark(int k, hipStream_t s) {   // profiled passes: the start of class k (and the end of the one before)
         if (prof) (void)hipEventRecord(pev[k], s);
-        if (rec_ev && (k == timing_cls || k == timing_cls + 1)) {
-            hipEvent_t ev = rec_ev[2 * rec_pass + (k == timing_cls ? 0 : 1)];
-            const hipError_t e = rec_capture ? record_in_capture(ev, s) : hipEventRecord(ev, s);
-            if (e != hipSuccess && rec_err == hipSuccess) rec_err = e;
-        }
+    }
+    // the timed class's events, on the stream the class runs on
+    void tbegin(int k, hipStream_t s) { tevent(k, 0, s); }
+    void tend(int k, hipStream_t s) { tevent(k, 1, s); }
+    void tevent(int k, int end, hipStream_t s) {
+        if (!rec_ev || k != timing_cls) return;
+        hipEvent_t ev = rec_ev[2 * rec_pass + end];
+        const hipError_t e = rec_capture ? record_in_capture(ev, s) : hipEventRecord(ev, s);
+        if (e != hipSuccess && rec_err == hipSuccess) rec_err = e;
     }
     // an event-record node after the nodes captured so far, which the next captured work depends on: the event of a
     // stream-ordered hipEventRecord, inside the graph
@@ -752,8 +790,11 @@ struct Level {
     int bin_job(BinJob& J, const float* size, const float* ratio, int num_nodes, int num_mov, double bx, double by,
                 int nbx, int nby, int planes, const float* fixed, float* out, int count_over);
     bool side_ok(const BinJob& J, int T) const;
-    void set_side(BinJob& J, int T);
+    void set_side(BinJob& J, int T, int G = 0);
     void update_lds();
+    int set_lds() const;
+    int bin_grid(int which, const Pick& pos, hipStream_t s);
+    int tiles_grid(int which, const Pick& pos, hipStream_t s);
     int time_density(int reps, hipStream_t s, float& ms);
     int autotune(int reps, hipStream_t s, int* sides);
     int create(const ap_level_desc& desc);
@@ -817,41 +858,98 @@ bool Level::side_ok(const BinJob& J, int T) const {
     return T >= 8 && T <= 64 && (long long)J.planes * grid_of(J.nbx, T) * grid_of(J.nby, T) <= BIN_MAX_BUCKETS;
 }
 
-void Level::set_side(BinJob& J, int T) {
+// tiles per workgroup of the tile kernel, by default: as many as keep the workgroup's accumulators within 4096 bins
+// (32 KB of LDS), 8 at most (a tile keeps one wave at least)
+static int default_group(int planes, int T) {
+    int G = 1;
+    while (G < 8 && 2 * G * planes * T * T <= 4096) G *= 2;
+    return G;
+}
+
+// a group the tile kernel can use for side T: 1, 2, 4 or 8 tiles per workgroup, within 64 KB of LDS
+static bool group_ok(int planes, int T, int G) {
+    return (G == 1 || G == 2 || G == 4 || G == 8) && (size_t)G * planes * T * T * sizeof(unsigned long long) <= 65536;
+}
+
+void Level::set_side(BinJob& J, int T, int G) {
     J.T = T;
     J.tiles_y = (int)grid_of(J.nby, T);
     J.tiles = (int)grid_of(J.nbx, T) * J.tiles_y;
+    J.G = G ? G : default_group(J.planes, T);
+    J.wgs = (J.tiles + J.G - 1) / J.G;
 }
 
 void Level::update_lds() {
-    hist_lds = acc_lds = 0;
-    for (const BinJob* J : {&fj, &jj}) {
-        if (!J->count) continue;
-        hist_lds = std::max(hist_lds, (size_t)J->planes * J->tiles * sizeof(int));
-        acc_lds = std::max(acc_lds, (size_t)J->planes * J->T * J->T * sizeof(unsigned long long));
-    }
+    auto hist = [](const BinJob& J) { return J.count ? (size_t)J.planes * J.tiles * sizeof(int) : (size_t)0; };
+    auto acc = [](const BinJob& J) {
+        return J.count ? (size_t)J.G * J.planes * J.T * J.T * sizeof(unsigned long long) : (size_t)0;
+    };
+    hist_f = hist(fj);
+    hist_j = hist(jj);
+    acc_f = acc(fj);
+    acc_j = acc(jj);
+    hist_lds = std::max(hist_f, hist_j);
+    acc_lds = std::max(acc_f, acc_j);
 }
 
-// The density stage (count, scan, scatter, tiles) at the caller's current point v (between runs the current point is
-// always set 0), `reps` times: ms per run. It changes nothing a pass reads before writing it again (bucket counts end
-// at zero, rho is rewritten).
+// the dynamic LDS limits of the single-level count and tile kernels, for the current layout
+int Level::set_lds() const {
+    const void* fn[4] = {(const void*)eng_bin_count, (const void*)eng_over_count, (const void*)eng_bin_tiles,
+                         (const void*)eng_over_tiles};
+    const size_t lds[4] = {hist_f, hist_j, acc_f, acc_j};
+    for (int k = 0; k < 4; k++)
+        if (lds[k]) AP_HIP(hipFuncSetAttribute(fn[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds[k]));
+    return AP_OK;
+}
+
+// count, scan and scatter of one grid (0: force, 1: overflow)
+int Level::bin_grid(int which, const Pick& pos, hipStream_t s) {
+    if (!which) {
+        hipLaunchKernelGGL(eng_bin_count, dim3(fj.blocks), dim3(EB), hist_f, s, pos, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_bin_scan, dim3(1), dim3(SCAN_T), 0, s, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_bin_scatter, dim3(fj.blocks), dim3(EB), 0, s, pos, fj, jj);
+        AP_LAUNCHED();
+    } else {
+        hipLaunchKernelGGL(eng_over_count, dim3(jj.blocks), dim3(EB), hist_j, s, pos, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_over_scan, dim3(1), dim3(SCAN_T), 0, s, fj, jj);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(eng_over_scatter, dim3(jj.blocks), dim3(EB), 0, s, pos, fj, jj);
+        AP_LAUNCHED();
+    }
+    return AP_OK;
+}
+
+// the tile sums of one grid: density and overflow partials
+int Level::tiles_grid(int which, const Pick& pos, hipStream_t s) {
+    if (!which)
+        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.wgs), dim3(TB), acc_f, s, pos, fj, jj, d.target, p_over);
+    else
+        hipLaunchKernelGGL(eng_over_tiles, dim3(jj.wgs), dim3(TB), acc_j, s, pos, fj, jj, d.target, p_over);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+// The front of a pass as the passes run it -- WA and the overflow grid's binning and tiles on the side stream, the
+// force grid's binning, tiles and Poisson solve on s -- at the caller's current point v (between runs the current
+// point is always set 0), `reps` times: ms per run. It changes nothing a pass reads before writing it again (bucket
+// counts end at zero; pin gradients, partials, rho and the field are rewritten by every pass).
 int Level::time_density(int reps, hipStream_t s, float& ms) {
     const Pick pos{(const float2*)d.v, nullptr, nullptr};
-    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)hist_lds));
-    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)acc_lds));
+    int st;
+    if ((st = set_lds())) return st;
     AP_HIP(hipEventRecord(t0, s));
     for (int r = 0; r < reps; r++) {
-        hipLaunchKernelGGL(eng_bin_count, dim3(fj.blocks + jj.blocks), dim3(EB), hist_lds, s, pos, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_bin_scan, dim3(d.has_judge ? 2 : 1), dim3(SCAN_T), 0, s, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_bin_scatter, dim3(fj.blocks + jj.blocks), dim3(EB), 0, s, pos, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.tiles + jj.tiles), dim3(TB), acc_lds, s, pos, fj, jj, d.target,
-                           p_over);
-        AP_LAUNCHED();
+        AP_HIP(hipEventRecord(ev_fork, s));
+        AP_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+        if ((st = wa_plan.launch(wa_args(pos, d.kp), p_wa, side))) return st;
+        if (jj.count && ((st = bin_grid(1, pos, side)) || (st = tiles_grid(1, pos, side)))) return st;
+        if ((st = bin_grid(0, pos, s)) || (st = tiles_grid(0, pos, s))) return st;
+        if ((st = poisson.solve(rho, d.nby, fx, fy, d.nby, s))) return st;
+        AP_HIP(hipEventRecord(ev_join, side));
+        AP_HIP(hipStreamWaitEvent(s, ev_join, 0));
     }
     AP_HIP(hipEventRecord(t1, s));
     AP_HIP(hipEventSynchronize(t1));
@@ -860,42 +958,51 @@ int Level::time_density(int reps, hipStream_t s, float& ms) {
     return AP_OK;
 }
 
-// Autotuner of the tile sides: for each grid (force, then overflow), every admissible side from 8 to 64 is timed on
-// the density stage at the current positions and the fastest kept. The map does not depend on the side (integer
-// sums), so the choice changes the time only, never the results.
+// Autotuner of the tile layout: for each grid (force, then overflow), every admissible side from 8 to 64 with every
+// admissible group (tiles per workgroup) is timed on the front of a pass (time_density) at the current positions and
+// the fastest kept. The map does not depend on either (integer sums), so the choice changes the time only, never
+// the results.
 int Level::autotune(int reps, hipStream_t s, int* sides) {
     AP_HIP(hipDeviceSynchronize());
+    int st;
     for (BinJob* J : {&fj, &jj}) {
         if (!J->count) continue;
-        int best = J->T;
+        int best = J->T, best_g = J->G;
         float best_ms = 1e30f;
         for (int T = 8; T <= 64; T *= 2) {
             if (!side_ok(*J, T)) continue;
-            set_side(*J, T);
-            update_lds();
-            float ms = 0;
-            int st = time_density(1, s, ms);                  // warm-up at this side
-            if (!st) st = time_density(reps, s, ms);
-            if (st) return st;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = T;
+            for (int G = 1; G <= 8; G *= 2) {
+                if (!group_ok(J->planes, T, G)) break;
+                set_side(*J, T, G);
+                update_lds();
+                float warm = 0, ms = 0, again = 0;
+                if ((st = time_density(1, s, warm))) return st;   // warm-up at this layout
+                if (warm > 1.5f * best_ms) continue;              // clearly slower: not timed further
+                if ((st = time_density(reps, s, ms))) return st;
+                if (ms < 1.1f * best_ms) {                        // a contender: the better of two timings
+                    if ((st = time_density(reps, s, again))) return st;
+                    ms = std::min(ms, again);
+                }
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = T;
+                    best_g = G;
+                }
             }
         }
-        set_side(*J, best);
+        set_side(*J, best, best_g);
         update_lds();
     }
-    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)hist_lds));
-    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)acc_lds));
+    if ((st = set_lds())) return st;
     for (GraphEntry& G : graphs) {                    // the kernels' arguments change: capture again
         if (last_ev == &G.ev) last_ev = nullptr;
         G.clear();
     }
     if (sides) {
         sides[0] = fj.T;
-        sides[1] = jj.count ? jj.T : 0;
+        sides[1] = fj.G;
+        sides[2] = jj.count ? jj.T : 0;
+        sides[3] = jj.count ? jj.G : 0;
     }
     AP_HIP(hipDeviceSynchronize());
     return AP_OK;
@@ -941,10 +1048,7 @@ int Level::create(const ap_level_desc& desc) {
         (st = bin_job(jj, d.judge_size, d.judge_ratio, d.num_mov, d.num_mov, d.jbx, d.jby, d.njx, d.njy, 1,
                       d.judge_fixed, nullptr, 1)))
         return st;
-    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_count, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)hist_lds));
-    AP_HIP(hipFuncSetAttribute((const void*)eng_bin_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)acc_lds));
+    if ((st = set_lds())) return st;
     // one overflow partial per tile: room for the most tiles any side the autotuner may pick gives (a grid has at
     // most BIN_MAX_BUCKETS tiles)
     const long long n_over = 2LL * BIN_MAX_BUCKETS;
@@ -965,8 +1069,11 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipMalloc(&flags, 4 * sizeof(int)));
     AP_HIP(hipMemset(flags, 0, 4 * sizeof(int)));
     AP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    AP_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     AP_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
     AP_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+    AP_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    AP_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     AP_HIP(hipEventCreate(&t0));
     AP_HIP(hipEventCreate(&t1));
     AP_HIP(hipDeviceSynchronize());
@@ -988,7 +1095,8 @@ void Level::destroy() {
     poisson.destroy();
     wa_plan.destroy();
     if (stream) (void)hipStreamDestroy(stream);
-    hipEvent_t evs[] = {ev_in, ev_out, t0, t1};
+    if (side) (void)hipStreamDestroy(side);
+    hipEvent_t evs[] = {ev_in, ev_out, t0, t1, ev_fork, ev_join};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
 }
@@ -1043,32 +1151,57 @@ NestArgs Level::nest_args() const {
 Pick Level::pass_pos() const { return Pick{(const float2*)d.v, (const float2*)v1, flags}; }
 
 // objective at pos with the parameters kp; g_prev (optional) gives |g' - g|. With S the control pass follows.
+// The objective's kernels. Passes that are not profiled run two streams: the caller's stream s takes the force grid
+// (binning, tiles, Poisson solve), and a side stream forked from it takes WA and the overflow grid, which read the
+// same positions and write buffers of their own; the gather, which needs both, waits for the side stream. Profiled
+// passes run every kernel class in order on s, so that the events between the classes time each of them alone.
 int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev,
                      const float2* g_prev_b, float2* g_out_a, float2* g_out_b, hipStream_t s, double* S, float* kp_out,
                      int use_density) {
     int st;
+    const bool two = !prof;
+    const bool over = use_density && jj.count;
+    hipStream_t w = s;                                       // the stream of WA and the overflow grid
+    if (two) {
+        w = side;
+        AP_HIP(hipEventRecord(ev_fork, s));
+        AP_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+    }
     mark(AP_PROF_WA, s);
-    if ((st = wa_plan.launch(wa_args(pos, kp_ptr), p_wa, s))) return st;
+    tbegin(AP_PROF_WA, w);
+    if ((st = wa_plan.launch(wa_args(pos, kp_ptr), p_wa, w))) return st;
+    tend(AP_PROF_WA, w);
     if (use_density) {
         mark(AP_PROF_MAP, s);
-        hipLaunchKernelGGL(eng_bin_count, dim3(fj.blocks + jj.blocks), dim3(EB), hist_lds, s, pos, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_bin_scan, dim3(d.has_judge ? 2 : 1), dim3(SCAN_T), 0, s, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_bin_scatter, dim3(fj.blocks + jj.blocks), dim3(EB), 0, s, pos, fj, jj);
-        AP_LAUNCHED();
+        tbegin(AP_PROF_MAP, s);
+        if ((st = bin_grid(0, pos, s))) return st;
+        if (over && (st = bin_grid(1, pos, w))) return st;
+        tend(AP_PROF_MAP, s);
         mark(AP_PROF_POST, s);
-        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.tiles + jj.tiles), dim3(TB), acc_lds, s, pos, fj, jj, d.target,
-                           p_over);
-        AP_LAUNCHED();
+        tbegin(AP_PROF_POST, s);
+        if ((st = tiles_grid(0, pos, s))) return st;
+        tend(AP_PROF_POST, s);
+        mark(AP_PROF_OVER_TILES, s);
+        tbegin(AP_PROF_OVER_TILES, w);
+        if (over && (st = tiles_grid(1, pos, w))) return st;
+        tend(AP_PROF_OVER_TILES, w);
         mark(AP_PROF_POISSON, s);
+        tbegin(AP_PROF_POISSON, s);
         if ((st = poisson.solve(rho, d.nby, fx, fy, d.nby, s))) return st;
+        tend(AP_PROF_POISSON, s);
+    }
+    if (two) {
+        AP_HIP(hipEventRecord(ev_join, side));
+        AP_HIP(hipStreamWaitEvent(s, ev_join, 0));
     }
     const GatherArgs g = gather_args(pos, kp_ptr, g_prev, g_prev_b, g_out_a, g_out_b, use_density);
     mark(AP_PROF_GATHER, s);
+    tbegin(AP_PROF_GATHER, s);
     hipLaunchKernelGGL(eng_gather, dim3(nblk_n), dim3(EB), 0, s, g, p_turned, p_wl2, p_dn2);
     AP_LAUNCHED();
+    tend(AP_PROF_GATHER, s);
     mark(AP_PROF_CONTROL, s);
+    tbegin(AP_PROF_CONTROL, s);
     hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(EB), 0, s, segs(with_state, g_prev != nullptr, use_density),
                        red);
     AP_LAUNCHED();
@@ -1077,14 +1210,17 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red, use_density ? over_scale() : 0.0, S, kp_out, d.c,
                        flags, stats, ctl_pass);
     AP_LAUNCHED();
+    tend(AP_PROF_CONTROL, s);
     mark(AP_PROF_CLASSES, s);
     return AP_OK;
 }
 
 int Level::pass(hipStream_t s) {
     mark(AP_PROF_NESTEROV, s);
+    tbegin(AP_PROF_NESTEROV, s);
     hipLaunchKernelGGL(eng_nesterov, dim3(nblk_n), dim3(EB), 0, s, nest_args());
     AP_LAUNCHED();
+    tend(AP_PROF_NESTEROV, s);
     return objective(pass_pos(), d.kp, true, (const float2*)d.g, g1, (float2*)d.g, g1, s, d.S, d.kp, 1);
 }
 
@@ -1333,7 +1469,7 @@ int Batch::build() {
         add(ST_COUNT, L.fj.blocks + L.jj.blocks);
         add(ST_SCAN, L.d.has_judge ? 2 : 1);
         add(ST_SCATTER, L.fj.blocks + L.jj.blocks);
-        add(ST_TILES, L.fj.tiles + L.jj.tiles);
+        add(ST_TILES, L.fj.wgs + L.jj.wgs);
         add(ST_GATHER, L.nblk_n);
         add(ST_REDUCE, RED_BLOCKS);
         add(ST_CONTROL, 1);
@@ -1715,21 +1851,51 @@ int ap_level_autotune(ap_level* level, int reps, int* sides) {
     return L.autotune(reps, L.stream, sides);
 }
 
-int ap_batch_autotune(ap_batch* batch, int reps, int* sides) {
+int ap_level_set_tile_layout(ap_level* level, int force_side, int force_group, int overflow_side,
+                             int overflow_group) {
+    AP_REQUIRE(level, "level_set_tile_layout: null argument");
+    Level& L = level->L;
+    AP_REQUIRE(L.side_ok(L.fj, force_side) &&
+                   (!force_group || group_ok(L.fj.planes, force_side, force_group)),
+               "level_set_tile_layout: side %d, group %d do not fit the force grid", force_side, force_group);
+    AP_REQUIRE(!L.jj.count || (L.side_ok(L.jj, overflow_side) &&
+                               (!overflow_group || group_ok(L.jj.planes, overflow_side, overflow_group))),
+               "level_set_tile_layout: side %d, group %d do not fit the overflow grid", overflow_side,
+               overflow_group);
+    AP_HIP(hipDeviceSynchronize());
+    L.set_side(L.fj, force_side, force_group);
+    if (L.jj.count) L.set_side(L.jj, overflow_side, overflow_group);
+    L.update_lds();
+    const int st = L.set_lds();
+    if (st) return st;
+    for (GraphEntry& G : L.graphs) {
+        if (L.last_ev == &G.ev) L.last_ev = nullptr;
+        G.clear();
+    }
+    return AP_OK;
+}
+
+int ap_batch_autotune(ap_batch* batch, int reps, int* layout) {
     AP_REQUIRE(batch && reps > 0, "batch_autotune: bad arguments");
     Batch& B = batch->B;
     for (size_t b = 0; b < B.levels.size(); b++) {
-        int two[2];
-        const int st = B.levels[b]->autotune(reps, B.stream, two);
+        int four[4];
+        const int st = B.levels[b]->autotune(reps, B.stream, four);
         if (st) return st;
-        if (sides) sides[b] = two[0];
+        if (layout) {
+            layout[2 * b] = four[0];
+            layout[2 * b + 1] = four[1];
+        }
     }
     return B.build();
 }
 
-int ap_batch_tile_sides(ap_batch* batch, int* sides) {
-    AP_REQUIRE(batch && sides, "batch_tile_sides: null argument");
-    for (size_t b = 0; b < batch->B.levels.size(); b++) sides[b] = batch->B.levels[b]->fj.T;
+int ap_batch_tile_layout(ap_batch* batch, int* layout) {
+    AP_REQUIRE(batch && layout, "batch_tile_layout: null argument");
+    for (size_t b = 0; b < batch->B.levels.size(); b++) {
+        layout[2 * b] = batch->B.levels[b]->fj.T;
+        layout[2 * b + 1] = batch->B.levels[b]->fj.G;
+    }
     return AP_OK;
 }
 

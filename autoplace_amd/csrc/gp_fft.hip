@@ -79,21 +79,22 @@ __device__ __forceinline__ void load_twiddles(float2* tw_s, const float2* tw, in
     for (int j = threadIdx.x; j < (L >> 1); j += blockDim.x) tw_s[j] = tw[j];
 }
 
+// DCT-II values X0[k], X1[k] of the row pair packed in one forward FFT Z (Zr = Z[(L - k) mod L], p = ph[k] =
+// (cos, sin)(pi k / 2L)): V0 = (Z + conj Zr) / 2, V1 = (Z - conj Zr) / 2i, X = Re(conj(p) V)
+__device__ __forceinline__ void dct_pair_at(float2 Z, float2 Zr, float2 p, float& X0, float& X1) {
+    const float v0r = 0.5f * (Z.x + Zr.x), v0i = 0.5f * (Z.y - Zr.y);
+    const float v1r = 0.5f * (Z.y + Zr.y), v1i = -0.5f * (Z.x - Zr.x);
+    X0 = p.x * v0r + p.y * v0i;
+    X1 = p.x * v1r + p.y * v1i;
+}
+
 // DCT-II of the rows r0 = 2 blockIdx.x, r0 + 1 (real, length L) -> out rows; spectrum of the pair from one FFT
 __device__ void dct2_pair_to_lds(const float* in0, const float* in1, float2* s, const float2* tw_s, const float2* ph,
                                  int L, int lg, float* X0, float* X1) {
     for (int m = threadIdx.x; m < L; m += blockDim.x) s[bitrev(makhoul(m, L), lg)] = make_float2(in0[m], in1[m]);
     __syncthreads();
     fft_lds(s, tw_s, L, lg, false);
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        const float2 Z = s[k], Zr = s[(L - k) & (L - 1)];
-        // V0 = (Z + conj Zr) / 2,  V1 = (Z - conj Zr) / 2i
-        const float v0r = 0.5f * (Z.x + Zr.x), v0i = 0.5f * (Z.y - Zr.y);
-        const float v1r = 0.5f * (Z.y + Zr.y), v1i = -0.5f * (Z.x - Zr.x);
-        const float2 p = ph[k];   // (cos, sin)(pi k / 2L)
-        X0[k] = p.x * v0r + p.y * v0i;
-        X1[k] = p.x * v1r + p.y * v1i;
-    }
+    for (int k = threadIdx.x; k < L; k += blockDim.x) dct_pair_at(s[k], s[(L - k) & (L - 1)], ph[k], X0[k], X1[k]);
 }
 
 __device__ __forceinline__ void pass_a_body(const float* rho, int ld_in, float* T, int ld_out, const float2* tw,
@@ -113,22 +114,34 @@ __global__ __launch_bounds__(FB) void fft_pass_a(const float* rho, int ld_in, fl
     pass_a_body(rho, ld_in, T, ld_out, tw, ph, L, lg, blockIdx.x);
 }
 
-// cosine series of a_cos and sine series of a_sin (both real, length L, in LDS) from one inverse FFT:
-// h[k] = Vc[k] + i Vs[k], Vc / Vs Makhoul's inverse spectra of a_cos and of the reversed a_sin
-__device__ void cos_sin_series(const float* a_cos, const float* a_sin, float2* s, const float2* tw_s, const float2* ph,
-                               int L, int lg, float* y_cos, float* y_sin) {
-    for (int k = threadIdx.x; k < L; k += blockDim.x) {
-        const int kr = (L - k) & (L - 1);
-        // X'[k] - i X'[L-k] for the cosine input (X'[0] = 2 a[0], X'[L] = 0)
-        const float xc = k ? a_cos[k] : 2.f * a_cos[0], xcr = k ? a_cos[kr] : 0.f;
-        // reversed sine input: at[k] = a_sin[L - k] (k >= 1), at[0] = 0; X'[k] = at[k] (X'[0] = 0), X'[L-k] = at[L-k]
-        const float xs = k ? a_sin[kr] : 0.f, xsr = k ? a_sin[k] : 0.f;
-        const float2 p = ph[k];   // e^{i pi k / 2L}
-        const float2 vc = cmul(p, make_float2(xc, -xcr)), vs = cmul(p, make_float2(xs, -xsr));
-        s[bitrev(k, lg)] = make_float2(vc.x - vs.y, vc.y + vs.x);   // vc + i vs
+// The cosine series of a_cos and the sine series of a_sin (both real, length L) from one inverse FFT: its input at k
+// is h[k] = Vc[k] + i Vs[k], Vc / Vs Makhoul's inverse spectra of a_cos and of the reversed a_sin, which need the
+// coefficients at k and at L - k. A lane therefore owns pair units u = (k, L - k), k = lane + u FB <= L / 2, and
+// keeps the coefficients of its units in registers: no LDS beyond the FFT's own.
+constexpr int FFT_UNITS = (4096 / 2 + FB) / FB;   // pair units per lane for L <= 4096
+
+// h[k] from X'[k] - i X'[L-k] of the cosine input (xc, xcr) and of the reversed sine input (xs, xsr)
+__device__ __forceinline__ void cs_put(float2* s, const float2* ph, int lg, int k, float xc, float xcr, float xs,
+                                       float xsr) {
+    const float2 p = ph[k];   // e^{i pi k / 2L}
+    const float2 vc = cmul(p, make_float2(xc, -xcr)), vs = cmul(p, make_float2(xs, -xsr));
+    s[bitrev(k, lg)] = make_float2(vc.x - vs.y, vc.y + vs.x);   // vc + i vs
+}
+// the unit (k, L - k): c / cr = a_cos[k] / a_cos[L-k], sv / sr = a_sin[k] / a_sin[L-k] (X'[0] = 2 a_cos[0] and 0 for
+// the reversed sine input; at k = L / 2 both indices are the same)
+__device__ __forceinline__ void cs_put_unit(float2* s, const float2* ph, int L, int lg, int k, float c, float cr,
+                                            float sv, float sr) {
+    if (k == 0) {
+        cs_put(s, ph, lg, 0, 2.f * c, 0.f, 0.f, 0.f);
+    } else if (2 * k == L) {
+        cs_put(s, ph, lg, k, c, c, sv, sv);
+    } else {
+        cs_put(s, ph, lg, k, c, cr, sr, sv);
+        cs_put(s, ph, lg, L - k, cr, c, sv, sr);
     }
-    __syncthreads();
-    fft_lds(s, tw_s, L, lg, true);
+}
+// after the inverse FFT of h: y_cos[j], y_sin[j] (Makhoul's output order)
+__device__ __forceinline__ void cs_store(const float2* s, int L, float* y_cos, float* y_sin) {
     for (int j = threadIdx.x; j < L; j += blockDim.x) {
         const float2 y = s[makhoul(j, L)];
         y_cos[j] = 0.5f * y.x;
@@ -149,32 +162,54 @@ struct PassB {
     const float2* ph;
 };
 
+// the spectral weights of row l at x-frequency k: Ay = nu / den C, Ax = omega / den C
+__device__ __forceinline__ void weights_at(const PassB& a, int k, int l, float nu, float C, float& ay, float& ax) {
+    const float w = a.omega[k], den = (w * w + nu * nu) * a.bx;
+    const float inv = den > 0.f ? a.scale / den : 0.f;
+    ay = nu * inv * (k == 0 ? 0.5f : 1.f) * C;
+    ax = w * inv * (l == 0 ? 0.5f : 1.f) * C;
+}
+
 __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
     extern __shared__ float2 lds[];
-    const int L = a.N;
+    const int L = a.N, lg = a.lgN, half = L >> 1;
     float2* s = lds;
     float2* tw_s = lds + L;
-    float* c0 = (float*)(tw_s + L / 2);
-    float* c1 = c0 + L;
-    float* ac = c1 + L;
-    float* as = ac + L;
     load_twiddles(tw_s, a.tw, L);
     const int l0 = 2 * wg;
+    const float* in0 = a.Tt + (long long)l0 * L;
+    const float* in1 = in0 + L;
+    for (int m = threadIdx.x; m < L; m += blockDim.x) s[bitrev(makhoul(m, L), lg)] = make_float2(in0[m], in1[m]);
     __syncthreads();
-    dct2_pair_to_lds(a.Tt + (long long)l0 * L, a.Tt + (long long)(l0 + 1) * L, s, tw_s, a.ph, L, a.lgN, c0, c1);
-    __syncthreads();
-    for (int q = 0; q < 2; q++) {
+    fft_lds(s, tw_s, L, lg, false);
+    // DCT-II of both rows at this lane's units: c[u][row][0 at k, 1 at L - k]
+    float c[FFT_UNITS][2][2];
+#pragma unroll
+    for (int u = 0; u < FFT_UNITS; u++) {
+        const int k = threadIdx.x + u * FB;
+        c[u][0][0] = c[u][0][1] = c[u][1][0] = c[u][1][1] = 0.f;
+        if (k > half) continue;
+        const float2 Zk = s[k], Zr = s[(L - k) & (L - 1)];
+        dct_pair_at(Zk, Zr, a.ph[k], c[u][0][0], c[u][1][0]);
+        if (k && 2 * k != L) dct_pair_at(Zr, Zk, a.ph[L - k], c[u][0][1], c[u][1][1]);
+    }
+    __syncthreads();   // s is the inverse FFT's from here on
+#pragma unroll
+    for (int q = 0; q < 2; q++) {   // unrolled: c stays in registers
         const int l = l0 + q;
-        const float* C = q ? c1 : c0;
         const float nu = a.nu[l];
-        for (int k = threadIdx.x; k < L; k += blockDim.x) {
-            const float w = a.omega[k], den = (w * w + nu * nu) * a.bx;
-            const float inv = den > 0.f ? a.scale / den : 0.f;
-            ac[k] = nu * inv * (k == 0 ? 0.5f : 1.f) * C[k];   // Ay
-            as[k] = w * inv * (l == 0 ? 0.5f : 1.f) * C[k];    // Ax
+#pragma unroll
+        for (int u = 0; u < FFT_UNITS; u++) {
+            const int k = threadIdx.x + u * FB;
+            if (k > half) continue;
+            float ay, ax, ayr = 0.f, axr = 0.f;
+            weights_at(a, k, l, nu, c[u][q][0], ay, ax);
+            if (k && 2 * k != L) weights_at(a, L - k, l, nu, c[u][q][1], ayr, axr);
+            cs_put_unit(s, a.ph, L, lg, k, ay, ayr, ax, axr);   // cosine series of Ay, sine series of Ax
         }
         __syncthreads();
-        cos_sin_series(ac, as, s, tw_s, a.ph, L, a.lgN, a.Vt + (long long)l * L, a.Ut + (long long)l * L);
+        fft_lds(s, tw_s, L, lg, true);
+        cs_store(s, L, a.Vt + (long long)l * L, a.Ut + (long long)l * L);
         __syncthreads();
     }
 }
@@ -184,17 +219,22 @@ __global__ __launch_bounds__(FB) void fft_pass_b(PassB a) { pass_b_body(a, block
 __device__ __forceinline__ void pass_c_body(const float* U, const float* V, float* fx, float* fy, int ld_out,
                                             const float2* tw, const float2* ph, int L, int lg, int n) {
     extern __shared__ float2 lds[];
+    const int half = L >> 1;
     float2* s = lds;
     float2* tw_s = lds + L;
-    float* uc = (float*)(tw_s + L / 2);
-    float* vs = uc + L;
     load_twiddles(tw_s, tw, L);
-    for (int j = threadIdx.x; j < L; j += blockDim.x) {
-        uc[j] = U[(long long)n * L + j];
-        vs[j] = V[(long long)n * L + j];
+    const float* u_row = U + (long long)n * L;
+    const float* v_row = V + (long long)n * L;
+#pragma unroll
+    for (int u = 0; u < FFT_UNITS; u++) {
+        const int k = threadIdx.x + u * FB;
+        if (k > half) continue;
+        const bool two = k && 2 * k != L;
+        cs_put_unit(s, ph, L, lg, k, u_row[k], two ? u_row[L - k] : 0.f, v_row[k], two ? v_row[L - k] : 0.f);
     }
     __syncthreads();
-    cos_sin_series(uc, vs, s, tw_s, ph, L, lg, fx + (long long)n * ld_out, fy + (long long)n * ld_out);
+    fft_lds(s, tw_s, L, lg, true);
+    cs_store(s, L, fx + (long long)n * ld_out, fy + (long long)n * ld_out);
 }
 
 __global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V, float* fx, float* fy, int ld_out,
@@ -277,8 +317,8 @@ int FftGroup::create(const std::vector<FftTab>& tabs) {
             for (int y = 0; y < (int)grid_of(t.M, TT); y++)
                 for (int x = 0; x < (int)grid_of(t.N, TT); x++) m[3].push_back(make_int4(g, x, y, z));
         for (int y = 0; y < t.N; y++) m[4].push_back(make_int4(g, y, 0, 0));
-        lds_ac = std::max(lds_ac, (size_t)(t.M + t.M / 2) * sizeof(float2) + 2 * t.M * sizeof(float));
-        lds_b = std::max(lds_b, (size_t)(t.N + t.N / 2) * sizeof(float2) + 4 * t.N * sizeof(float));
+        lds_ac = std::max(lds_ac, (size_t)(t.M + t.M / 2) * sizeof(float2));
+        lds_b = std::max(lds_b, (size_t)(t.N + t.N / 2) * sizeof(float2));
     }
     for (int k = 0; k < 5; k++) {
         count[k] = (int)m[k].size();
@@ -399,15 +439,14 @@ int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, int f
     AP_LAUNCHED();
     // pass B: rows of Tt (length N) -> Ut (w2), Vt (w3)
     PassB b{w1, w2, w3, d_omega, d_omega + N, (float)(4.0 / ((double)N * M)), (float)bx, N, lgN, twN, phN};
-    hipLaunchKernelGGL(fft_pass_b, dim3(M / 2), dim3(FB), (size_t)(N + N / 2) * sizeof(float2) + 4 * N * sizeof(float),
-                       s, b);
+    hipLaunchKernelGGL(fft_pass_b, dim3(M / 2), dim3(FB), (size_t)(N + N / 2) * sizeof(float2), s, b);
     AP_LAUNCHED();
     hipLaunchKernelGGL(transpose_kernel, dim3(grid_of(N, TT), grid_of(M, TT), 2), dim3(256), 0, s, w2, w3, w0, w1, M,
                        N);
     AP_LAUNCHED();
     // pass C: rows of U (w0), V (w1) -> field
-    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), (size_t)(M + M / 2) * sizeof(float2) + 2 * M * sizeof(float), s,
-                       w0, w1, fx, fy, fld, twM, phM, M, lgM);
+    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), (size_t)(M + M / 2) * sizeof(float2), s, w0, w1, fx, fy, fld,
+                       twM, phM, M, lgM);
     AP_LAUNCHED();
     return AP_OK;
 }

@@ -99,11 +99,16 @@ class Engine(object):
         _lib.check(_lib.lib().ap_level_run(self._h, passes, int(use_graph), stream), "level_run")
 
     def autotune(self, reps=5):
-        """Time every admissible density tile side at the current point and keep the fastest (results unchanged).
-        -> (force-grid side, overflow-grid side or 0)."""
-        sides = (ctypes.c_int * 2)()
-        _lib.check(_lib.lib().ap_level_autotune(self._h, int(reps), sides), "level_autotune")
-        return sides[0], sides[1]
+        """Time every admissible density tile layout (side, tiles per workgroup) at the current point and keep the
+        fastest (results unchanged). -> ((force side, group), (overflow side, group) or (0, 0))."""
+        lay = (ctypes.c_int * 4)()
+        _lib.check(_lib.lib().ap_level_autotune(self._h, int(reps), lay), "level_autotune")
+        return (lay[0], lay[1]), (lay[2], lay[3])
+
+    def set_tile_layout(self, force, overflow=(0, 0)):
+        """force / overflow: (side, group); group 0 = the default for the side."""
+        _lib.check(_lib.lib().ap_level_set_tile_layout(self._h, int(force[0]), int(force[1]), int(overflow[0]),
+                                                       int(overflow[1])), "level_set_tile_layout")
 
     def finish(self):
         """Write u / v / g / best back into the caller's tensors (caller's node order)."""

@@ -1,7 +1,7 @@
 """Benchmark of the GP iteration on the 2M-cell synthetic design, one GPU.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C] [--no-cpu]
-                    [--save-snapshot F] [--snapshot F] [--copy-calibration]
+                    [--save-snapshot F] [--snapshot F] [--copy-calibration] [--tile-layout F,G,J,H]
 
 Workload: the 2,000,000-cell design (2,100,000 nets, ~8.2M pins, 16 fixed macros on 15 % of the die, generator
 seed 0, placer seed 0): force grid 2048 x 2048, overflow grid 512 x 512, default options.
@@ -9,18 +9,21 @@ seed 0, placer seed 0): force grid 2048 x 2048, overflow grid 512 x 512, default
 1. A whole two-level global placement to overflow <= 0.07 (untimed by this script's step clock; its own GP clock
    is reported): GP seconds, iterations, gradient evaluations, HPWL, overflow -- and it captures the state the
    cell level's device loop starts from (cells just expanded from their clusters).
-2. W warm-up passes from that state with direct launches and HIP events around every kernel class (the per-class
-   breakdown `kernel_ms`); the class with the largest single-kernel time is the roofline kernel.
+2. W warm-up passes from that state with direct launches, every kernel class in order on one stream with HIP
+   events between them (the per-class breakdown `kernel_ms`); the class with the largest single-kernel time is the
+   roofline kernel.
 3. The timed region: exactly K passes of the GP iteration on the device (Nesterov step, WA, density binning of both
    grids, Poisson solve, gather, control), one replay of a HIP graph captured beforehand, bracketed by
-   synchronisations. value = K / time = gradient evaluations per second. The graph carries a HIP event before and
-   after the roofline kernel in every pass (on the engine's stream): its average duration over the K passes gives
+   synchronisations. value = K / time = gradient evaluations per second. In the graph WA and the overflow grid run
+   on a second stream beside the force grid and the Poisson solve. The graph carries a HIP event before and after the
+   roofline kernel in every pass (on the stream that kernel runs on): its average duration over the K passes gives
    `roofline.achieved` = its algorithmic bytes per launch / that duration.
 4. cpu_baseline: the CPU oracle (numpy, one thread) running passes from the same captured state, bounded to ~25 s.
 
 --snapshot F starts from a state written by --save-snapshot (no full run, no CPU baseline): the form used under
 rocprofv3, together with --copy-calibration (three 256 MiB device copies first, the scale of the FETCH_SIZE /
-WRITE_SIZE counters).
+WRITE_SIZE counters) and --tile-layout F,G,J,H (the autotuner's choice of tile side and tiles per workgroup for
+the force and the overflow grid, replayed, so that its trial launches stay out of the profile).
 """
 import argparse
 import json
@@ -34,10 +37,10 @@ sys.path.insert(0, ROOT)
 PEAK_HBM = 8.0e12            # B/s
 PEAK_FP32_MFMA = 157.3e12    # flop/s
 
-# kernel classes made of one kernel (the others launch several: binning = count + scan + scatter, Poisson = 5,
-# control = reduce + control)
+# kernel classes made of one kernel (the others launch several: binning = count + scan + scatter per grid,
+# Poisson = 5, control = reduce + control)
 SINGLE_KERNEL = {"nesterov": "eng_nesterov", "wa": "wa_wave_kernel", "density_tiles": "eng_bin_tiles",
-                 "gather": "eng_gather"}
+                 "overflow_tiles": "eng_over_tiles", "gather": "eng_gather"}
 
 
 def algorithmic_bytes(cls, dims):
@@ -53,8 +56,10 @@ def algorithmic_bytes(cls, dims):
         "wa": 28 * P + 4 * nets,
         # per node and grid: position 8 + size 8 + ratio 4 read (force grid: all nodes; overflow grid: cells)
         "density_bin": 20 * N + 20 * M,
-        # the maps: fixed 4 read and density 4 written per force bin, fixed 4 read per overflow bin
-        "density_tiles": 8 * nb2 + 4 * nj2,
+        # the force map: fixed 4 read and density 4 written per bin
+        "density_tiles": 8 * nb2,
+        # the overflow map: fixed 4 read per bin
+        "overflow_tiles": 4 * nj2,
         # per node: position 8, size 8, ratio 4, pins 4, area 4, previous gradient 8 read, gradient 8 written;
         # per cell: cell_start 4; pin gradients 8 per pin; the field 8 per bin
         "gather": 44 * N + 4 * M + 8 * P + 8 * nb2,
@@ -137,7 +142,7 @@ def copy_calibration():
     torch.cuda.synchronize()
 
 
-def gpu_passes(snap, steps, warmup):
+def gpu_passes(snap, steps, warmup, layout=None):
     import torch
 
     from autoplace_amd import gpu_loop
@@ -147,8 +152,13 @@ def gpu_passes(snap, steps, warmup):
     c[9] = 1e12                                      # inner_iter: no max_iter stop inside the timed window
     u, v, g = snap["u"].clone(), snap["v"].clone(), snap["g"].clone()
     eng = gpu_loop.Engine(wl, den, u, v, g, v.clone(), S, kp, c)
-    sides = eng.autotune(5)                          # as the flow does for levels of 100k nodes and more
-    print("density tile sides (autotuned): force grid %d, overflow grid %d" % sides, file=sys.stderr, flush=True)
+    if layout:
+        eng.set_tile_layout(layout[:2], layout[2:])  # a choice of the autotuner, replayed (profiling runs)
+        lay = (tuple(layout[:2]), tuple(layout[2:]))
+    else:
+        lay = eng.autotune(5)                        # as the flow does for levels of 100k nodes and more
+    print("density tile layout (%s): force grid side %d x %d per workgroup, overflow grid side %d x %d"
+          % ((("given" if layout else "autotuned"),) + lay[0] + lay[1]), file=sys.stderr, flush=True)
     prof = eng.profile(max(warmup, 1))               # the warm-up passes, per kernel class
     dom = max(SINGLE_KERNEL, key=lambda k: prof[k])
     eng.time_class(dom)
@@ -223,6 +233,8 @@ def main():
     p.add_argument("--save-snapshot", default="", help="write the captured state (npz)")
     p.add_argument("--snapshot", default="", help="start from a saved state: no full run, no CPU baseline")
     p.add_argument("--copy-calibration", action="store_true", help="three 256 MiB device copies first")
+    p.add_argument("--tile-layout", default="", help="F,G,J,H: force side, group, overflow side, group "
+                                                       "(skip the autotuner)")
     a = p.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the GP iteration of one design runs on one device here "
@@ -243,7 +255,8 @@ def main():
     if a.save_snapshot:
         import numpy as np
         np.savez(a.save_snapshot, **{k: snap[k].cpu().numpy() for k in ("u", "v", "g", "S", "kp", "c")})
-    elapsed, prof, dom, dom_ms, dims, method, stop, in_graph = gpu_passes(snap, a.steps, a.warmup)
+    layout = tuple(int(x) for x in a.tile_layout.split(",")) if a.tile_layout else None
+    elapsed, prof, dom, dom_ms, dims, method, stop, in_graph = gpu_passes(snap, a.steps, a.warmup, layout)
     log("per-pass kernel ms (warm-up, direct launches): " + ", ".join("%s %.3f" % kv for kv in prof.items()))
     log("roofline kernel %s (%s): %.4f ms per launch over the timed passes" % (dom, SINGLE_KERNEL[dom], dom_ms))
     by = algorithmic_bytes(dom, dims)

@@ -156,12 +156,13 @@ AP_API int ap_level_prepare(ap_level* level, int passes);
 /* Kernel classes of a pass, in launch order, for ap_level_profile and ap_level_time_class. */
 #define AP_PROF_NESTEROV 0
 #define AP_PROF_WA 1
-#define AP_PROF_MAP 2      /* density: tile binning (count, scan, scatter) */
-#define AP_PROF_POST 3     /* density: per-tile sums, density and overflow */
-#define AP_PROF_POISSON 4
-#define AP_PROF_GATHER 5
-#define AP_PROF_CONTROL 6
-#define AP_PROF_CLASSES 7
+#define AP_PROF_MAP 2         /* density: tile binning of both grids (count, scan, scatter) */
+#define AP_PROF_POST 3        /* density: per-tile sums of the force grid, density and overflow */
+#define AP_PROF_OVER_TILES 4  /* density: per-tile sums of the overflow grid (when it has one) */
+#define AP_PROF_POISSON 5
+#define AP_PROF_GATHER 6
+#define AP_PROF_CONTROL 7
+#define AP_PROF_CLASSES 8
 /* Run `passes` passes with direct launches and HIP events around every kernel class; ms[AP_PROF_CLASSES] receives
  * the average milliseconds per pass of each class. Blocks the calling thread until the passes are done. */
 AP_API int ap_level_profile(ap_level* level, int passes, double* ms, void* stream);
@@ -179,11 +180,18 @@ AP_API int ap_level_objective(ap_level* level, const float* pos, float* grad, do
  * ap_level_objective): the top-left quadrant of the reference's density_map result. */
 AP_API int ap_level_density(ap_level* level, float* rho, void* stream);
 
-/* Autotuner of the density tile sides: every admissible side (8 .. 64 bins) of the force grid, then of the overflow
- * grid, timed on the density stage at the caller's current point (reps runs each); the fastest is kept.
- * sides i32[2] (optional): the chosen sides (0: no overflow grid). The density map does not depend on the side, so
- * the results of later passes are the same whatever the choice. Drops the level's captured graphs. */
-AP_API int ap_level_autotune(ap_level* level, int reps, int* sides);
+/* Autotuner of the density tile layout: every admissible side (8 .. 64 bins) with every admissible group (1, 2, 4, 8
+ * tiles per workgroup of the tile kernel, within 64 KB of LDS) of the force grid, then of the overflow grid, timed on
+ * the front of a pass (WA, binning, tiles, Poisson solve, as the passes overlap them) at the caller's current point
+ * (reps runs each); the fastest is kept. layout i32[4] (optional):
+ * force side, force group, overflow side, overflow group (0, 0: no overflow grid). The density map depends on
+ * neither, so the results of later passes are the same whatever the choice. Drops the level's captured graphs. */
+AP_API int ap_level_autotune(ap_level* level, int reps, int* layout);
+/* Set the density tile layout directly (an admissible side and group of each grid, group 0: the default for the
+ * side; the overflow pair is ignored without an overflow grid): the autotuner's choice, replayed. Drops the level's
+ * captured graphs. */
+AP_API int ap_level_set_tile_layout(ap_level* level, int force_side, int force_group, int overflow_side,
+                                    int overflow_group);
 
 /* ---------------------------------------------------------------- batch: many independent levels per launch
  * No counterpart in the reference. count levels (blocks) described as for ap_level_create (each with its own
@@ -197,10 +205,10 @@ AP_API int ap_batch_prepare(ap_batch* batch, int passes);
 AP_API int ap_batch_run(ap_batch* batch, int passes, int use_graph, void* stream);
 /* ms between the first and last kernel of the last ap_batch_run */
 AP_API int ap_batch_last_ms(ap_batch* batch, float* ms);
-/* ap_level_autotune for every block (force-grid side of block b into sides[b], optional) */
-AP_API int ap_batch_autotune(ap_batch* batch, int reps, int* sides);
-/* the force-grid tile side of every block: sides i32[count] */
-AP_API int ap_batch_tile_sides(ap_batch* batch, int* sides);
+/* ap_level_autotune for every block (optional layout i32[2 count]: side and group of block b's grid at 2b, 2b + 1) */
+AP_API int ap_batch_autotune(ap_batch* batch, int reps, int* layout);
+/* the tile layout of every block's grid: layout i32[2 count], side and group of block b at 2b, 2b + 1 */
+AP_API int ap_batch_tile_layout(ap_batch* batch, int* layout);
 
 /* ---------------------------------------------------------------- part 4: learned schedule controller
  * No counterpart in the reference: a 12 -> 64 -> 64 -> 2 ReLU MLP (FP32, parameters f32[ap_mlp_param_count()] laid

@@ -1,5 +1,5 @@
 """Batched placement on the GPU: blocks run together (grouped launches, one graph per run of passes) end exactly
-where their own device loops end (bit for bit), whatever tile sides the autotuner picks."""
+where their own device loops end (bit for bit), whatever tile layouts the autotuner picks."""
 import numpy as np
 import pytest
 
@@ -31,9 +31,9 @@ def test_batch_equals_sequential_runs(states):
     from autoplace_amd import batch
     seq = batch.run_sequential(states)
     bat = batch.run_batched(states)
-    print("sequential: %.3f s, batched: %.3f s; iterations %s; stops %s; tile sides %s"
+    print("sequential: %.3f s, batched: %.3f s; iterations %s; stops %s; tile layouts %s"
           % (seq["seconds"], bat["seconds"], [r["iters"] for r in seq["blocks"]],
-             [r["stop"] for r in seq["blocks"]], bat["tile_sides"]))
+             [r["stop"] for r in seq["blocks"]], bat["tile_layout"]))
     assert all(r["stop"] == "converged" for r in seq["blocks"])
     _same(seq, bat)
 
@@ -42,5 +42,5 @@ def test_autotuned_batch_gives_the_same_results(states):
     from autoplace_amd import batch
     plain = batch.run_batched(states)
     tuned = batch.run_batched(states, autotune=True, reps=2)
-    print("tile sides: default %s, autotuned %s" % (plain["tile_sides"], tuned["tile_sides"]))
+    print("tile layouts: default %s, autotuned %s" % (plain["tile_layout"], tuned["tile_layout"]))
     _same(plain, tuned)

@@ -15,11 +15,11 @@ pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
 
 
-def _setup(data, weight=2e-3, gamma=30.0, seed=1):
+def _setup(data, weight=2e-3, gamma=30.0, seed=1, grids=None):
     from autoplace_amd import gpu_loop
     from autoplace_amd.problem import build_problem, default_args
     from autoplace_amd.run_placement import _scheduler, build_operators
-    args = default_args()
+    args = default_args() if grids is None else default_args(num_bin=grids[0], force_num_bin=grids[1])
     prob = build_problem(data, args)
     wl, den = build_operators(prob, args, DEV, "gemm")
     rng = np.random.default_rng(seed)
@@ -74,7 +74,7 @@ def test_engine_objective_vs_oracle(small_design):
 def test_engine_density_bit_exact(small_design, case):
     """The engine's tile-binned density map equals the oracle's fixed-point map bit for bit, and its overflow the
     oracle's: on the small design (16 x 16-bin tiles) and on a 1024 x 1024 force grid (32 x 32-bin tiles) where
-    nodes span several tiles and the biggest go through the big-node path."""
+    nodes span several tiles and the biggest go through the big-node path; the same for every tile layout."""
     from autoplace_amd import synth
     from autoplace_amd.database import PlaceData
     from autoplace_amd.problem import build_problem, default_args
@@ -102,10 +102,29 @@ def test_engine_density_bit_exact(small_design, case):
     eng, _ = _engine(wl, den, pos, S, kp, c, np.zeros_like(pos), reorder=True, poisson="fft")
     grad = torch.empty_like(eng.v)
     stats = torch.zeros(8, dtype=torch.float64, device=DEV)
-    eng.objective(eng.v, grad, stats)
-    rho_dev = eng.density().cpu().numpy()
-    ovf_dev = float(stats[2])
+
+    def evaluate():
+        stats.zero_()
+        eng.objective(eng.v, grad, stats)
+        return eng.density().cpu().numpy(), float(stats[2])
+
+    rho_dev, ovf_dev = evaluate()
+    # every tile layout (side 8 .. 64, 1 .. 8 tiles per workgroup) the grids admit gives the same map and overflow
+    layouts = [(t, g) for t in (8, 16, 32, 64) for g in (1, 2, 4, 8)]
+    tried = []
+    for k, fl in enumerate(layouts):
+        jl = layouts[(7 * k + 3) % len(layouts)] if den.judge is not None else (0, 0)
+        try:
+            eng.set_tile_layout(fl, jl)
+        except ValueError:                           # a side or group the grid does not admit
+            continue
+        r, o = evaluate()
+        assert np.array_equal(r, rho_dev), (fl, jl)
+        assert rel_err(o, ovf_dev) < 1e-12, (fl, jl, o, ovf_dev)   # FP64 sums of per-tile partials
+        tried.append((fl, jl))
     eng.close()
+    print("tile layouts with the same map and overflow: %s" % tried)
+    assert len(tried) >= 6
     smooth, ratio = den.smooth["size"].cpu().numpy(), den.smooth["ratio"].cpu().numpy()
     bx, by = float(den.bin_np[0]), float(den.bin_np[1])
     m0, m1 = oden.density_maps(pos, smooth, ratio, data.num_mov, bx, by, den.nbx, den.nby)
@@ -131,8 +150,8 @@ def test_engine_density_bit_exact(small_design, case):
     assert rel_err(ovf_dev, ovf) < 1e-12
 
 
-def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cls=None):
-    args, prob, wl, den, pos, S, kp, c = _setup(data)
+def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cls=None, profile=False, grids=None):
+    args, prob, wl, den, pos, S, kp, c = _setup(data, grids=grids)
     g0 = np.zeros_like(pos)
     eng, bufs = _engine(wl, den, pos, S, kp, c, g0, reorder=False, poisson=poisson)
     grad = torch.empty_like(bufs[1])
@@ -145,7 +164,10 @@ def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cl
     if time_cls:
         eng.time_class(time_cls)
     for _ in range(calls):
-        eng.run(passes, use_graph)
+        if profile:
+            eng.profile(passes)
+        else:
+            eng.run(passes, use_graph)
     timing = eng.class_ms() if time_cls else None
     eng.finish()
     torch.cuda.synchronize()
@@ -171,13 +193,29 @@ def test_engine_graph_equals_direct_and_is_deterministic(small_design):
     assert a[4][K["loop_it"]] == 40 + 32 and a[4][K["evals"]] == 32
 
 
+@pytest.mark.parametrize("grids", [None, (32, 64)])
+def test_engine_two_streams_equal_one_stream(small_design, grids):
+    """Graph and direct passes, which run WA and the overflow grid on a second stream beside the force grid, end bit
+    for bit where profiled passes end, which run every kernel in order on one stream (with an overflow grid of its
+    own: force grid 64, overflow grid 32)."""
+    _, _, data = small_design
+    one, (_, prob, _, _, _, _, _, _) = _run(data, 16, False, profile=True, grids=grids)
+    print("force grid %d, overflow grid %d" % (prob.force_bin, prob.num_bin))
+    if grids:
+        assert prob.force_bin != prob.num_bin
+    for use_graph in (True, False):
+        two, _ = _run(data, 16, use_graph, grids=grids)
+        for x, y in zip(one, two):
+            assert np.array_equal(x, y)
+
+
 def test_engine_timing_events_change_nothing(small_design):
     """Event-record nodes around a kernel class (captured graph) and event records (direct launches) leave the results
     bit for bit as they are, and time the class."""
     _, _, data = small_design
     ref, _ = _run(data, 16, True)
     for use_graph in (True, False):
-        for cls in ("density_tiles", "wa"):
+        for cls in ("density_tiles", "wa", "gather"):
             out, (_, _, _, _, _, _, _, timing) = _run(data, 16, use_graph, time_cls=cls)
             ms, n = timing
             print("timed %s (%s): %.4f ms per pass over %d passes" % (cls, "graph" if use_graph else "direct", ms, n))
