@@ -16,7 +16,8 @@ D = ctypes.c_double
 
 AP_OK, AP_ERR_ARG, AP_ERR_HIP, AP_ERR_UNSUPPORTED = 0, 1, 2, 3
 AP_POISSON_AUTO, AP_POISSON_GEMM, AP_POISSON_FFT = 0, 1, 2
-PROF_CLASSES = ("nesterov", "wa", "density_bin", "density_tiles", "overflow_tiles", "poisson", "gather", "control")
+PROF_CLASSES = ("nesterov", "wa", "wa_cells", "density_bin", "density_tiles", "overflow_tiles", "poisson", "gather",
+                "control")
 
 _SIGNATURES = {
     "ap_last_error": ([], ctypes.c_char_p),

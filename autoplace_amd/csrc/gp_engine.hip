@@ -97,7 +97,8 @@ __global__ __launch_bounds__(EB) void eng_nesterov(NestArgs a) { nesterov_body(a
 //   eng_bin_count    a node whose bins fall in at most 2 x 2 tiles takes a place in the bucket of each of those tiles
 //                    (LDS histogram per workgroup, then one global atomic per workgroup and bucket); any other node
 //                    goes to the grid's list of big nodes
-//   eng_bin_scan     bucket starts (exclusive scan of the counts); the counts are cleared for the next pass
+//                    then bucket starts (exclusive scan of the counts) by the last workgroup to finish; the counts
+//                    are cleared for the next pass (the batch's grouped launches scan in a launch of their own)
 //   eng_bin_scatter  each node's centre, size and ratio into its buckets
 //   eng_bin_tiles    per tile (G tiles per workgroup): the contributions of the bucket's nodes and of the big nodes,
 //                    clipped to the tile and summed with LDS integer atomics; then the tile's density and overflow
@@ -131,6 +132,7 @@ struct BinJob {
     int* big_count;            // [1] big nodes listed in the pass (cleared by the scan)
     int* big_n;                // [1] big nodes of the pass (written by the scan)
     int* big;                  // [num_nodes] big node ids
+    int* done;                 // [1] count workgroups finished in the pass (the last one scans; back to zero after)
     const float* fixed;        // [nbx][nby]
     float* rho;                // density out (null: overflow only)
     int count_over;            // 1: this grid's overflow feeds the control
@@ -150,7 +152,12 @@ __device__ __forceinline__ bool small_node(const Span& sx, const Span& sy, int T
     return wx <= 1 && wy <= 1;
 }
 
-__device__ __forceinline__ void bin_count_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk) {
+template <int NT, bool SEEN_BY_ATOMICS>
+__device__ __forceinline__ void scan_buckets(const BinJob& J);
+
+// fused_scan: the last workgroup to finish scans the buckets (the scan kernel's work, without its launch)
+__device__ __forceinline__ void bin_count_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk,
+                                               bool fused_scan) {
     extern __shared__ int hist[];   // planes x tiles of the grid
     const bool judge = blk >= f.blocks;
     const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
@@ -198,50 +205,83 @@ __device__ __forceinline__ void bin_count_body(const Pick& pos_p, const BinJob& 
         o.w = rank[q][3] >= 0 ? hist[b0[q] + J.tiles_y + 1] + rank[q][3] : -1;
         J.off[i] = o;
     }
+    if (!fused_scan) return;
+    // The workgroup's count atomics have all returned (their values were used above) before it counts itself as
+    // done, so the last workgroup to do so reads the final counts with device-scope atomic loads.
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(J.done, 1) == J.blocks - 1;
+    __syncthreads();
+    if (!last) return;
+    scan_buckets<EB, true>(J);
+    if (threadIdx.x == 0) *J.done = 0;
 }
 // The single-level kernels of the binning run one grid per launch: eng_bin_* the force grid (f.blocks / 1 / f.wgs
 // workgroups), eng_over_* the overflow grid (j.blocks / 1 / j.wgs), so that the two can run side by side and each
 // launch reserves the LDS of its own grid only.
 __global__ __launch_bounds__(EB) void eng_bin_count(Pick pos_p, BinJob f, BinJob j) {
-    bin_count_body(pos_p, f, j, xcd_remap(blockIdx.x, gridDim.x));
+    bin_count_body(pos_p, f, j, xcd_remap(blockIdx.x, gridDim.x), true);
 }
 __global__ __launch_bounds__(EB) void eng_over_count(Pick pos_p, BinJob f, BinJob j) {
-    bin_count_body(pos_p, f, j, f.blocks + xcd_remap(blockIdx.x, gridDim.x));
+    bin_count_body(pos_p, f, j, f.blocks + xcd_remap(blockIdx.x, gridDim.x), true);
 }
 
-// one workgroup per grid
-__device__ __forceinline__ void bin_scan_body(const BinJob& f, const BinJob& j, int which) {
-    __shared__ int wsum[SCAN_T / 64];
-    const BinJob J = which ? j : f;
+// the bucket starts from the counts of the pass (exclusive scan by the NT lanes of one workgroup, 4 NT buckets a
+// round, coalesced), the counts back to zero for the next pass. SEEN_BY_ATOMICS: the counts (and the big-node count)
+// are read with device-scope atomic loads, inside the kernel whose workgroups added them.
+template <int NT, bool SEEN_BY_ATOMICS>
+__device__ __forceinline__ int bucket_load(const int* p) {
+    return SEEN_BY_ATOMICS ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+}
+template <int NT, bool SEEN_BY_ATOMICS>
+__device__ __forceinline__ void scan_buckets(const BinJob& J) {
+    __shared__ int wsum[NT / 64];
     const int n = J.planes * J.tiles;
-    const int per = (n + SCAN_T - 1) / SCAN_T;
-    const int lo = min(n, (int)threadIdx.x * per), hi = min(n, lo + per);
-    int s = 0;
-    for (int k = lo; k < hi; k++) s += J.count[k];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int x = s;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
+    int carry = 0;                                  // the same in every lane
+    for (int base = 0; base < n; base += 4 * NT) {
+        const int k0 = base + 4 * threadIdx.x;
+        int c[4], sum = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            c[i] = k0 + i < n ? bucket_load<NT, SEEN_BY_ATOMICS>(J.count + k0 + i) : 0;
+            sum += c[i];
+        }
+        int x = sum;                                // inclusive scan of the lanes' sums
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        int run = carry + x - sum, total = 0;
+        for (int w = 0; w < NT / 64; w++) {
+            const int v = wsum[w];
+            if (w < wave) run += v;
+            total += v;
+        }
+        __syncthreads();                            // wsum is rewritten by the next round
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (k0 + i >= n) break;
+            J.start[k0 + i] = run;
+            run += c[i];
+            J.count[k0 + i] = 0;
+        }
+        carry += total;
     }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    int run = x - s;
-    for (int w = 0; w < wave; w++) run += wsum[w];
-    for (int k = lo; k < hi; k++) {
-        const int c = J.count[k];
-        J.start[k] = run;
-        run += c;
-        J.count[k] = 0;
-    }
-    if (threadIdx.x == SCAN_T - 1) J.start[n] = run;   // the last lane's running sum is the total
     if (threadIdx.x == 0) {
-        *J.big_n = *J.big_count;
+        J.start[n] = carry;
+        *J.big_n = bucket_load<NT, SEEN_BY_ATOMICS>(J.big_count);
         *J.big_count = 0;
     }
 }
-__global__ __launch_bounds__(SCAN_T) void eng_bin_scan(BinJob f, BinJob j) { bin_scan_body(f, j, 0); }
-__global__ __launch_bounds__(SCAN_T) void eng_over_scan(BinJob f, BinJob j) { bin_scan_body(f, j, 1); }
+
+// the batch's scan launch (its count launches do not scan): one workgroup per grid
+__device__ __forceinline__ void bin_scan_body(const BinJob& f, const BinJob& j, int which) {
+    const BinJob J = which ? j : f;
+    scan_buckets<SCAN_T, false>(J);
+}
 
 __device__ __forceinline__ void bin_scatter_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk) {
     const bool judge = blk >= f.blocks;
@@ -302,7 +342,8 @@ __device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0
     }
 }
 
-constexpr int TB = 512;   // lanes per workgroup of the tile kernel: a crowded tile's bucket spreads over 8 waves
+constexpr int TB = 512;     // lanes per workgroup of the tile kernel: a crowded tile's bucket spreads over 8 waves
+constexpr int TILE_PF = 4;  // fixed-map values per lane loaded ahead (the whole tile up to T x T = 4 x lanes)
 
 // One workgroup sums J.G tiles (G = 1, 2, 4, 8), each with its own TB / G lanes (whole waves) and its own LDS
 // accumulator, so that small tiles do not each hold a workgroup (and the launch's LDS reservation) to themselves.
@@ -321,30 +362,50 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
     const int X0 = (t / J.tiles_y) * T, Y0 = (t % J.tiles_y) * T;
     const int X1 = min(X0 + T, J.nbx) - 1, Y1 = min(Y0 + T, J.nby) - 1;
     unsigned long long* acc = acc_all + sub * J.planes * TT;
+    // issued before anything waits on memory: the bucket bounds of both planes and this lane's share of the fixed
+    // map (the first TILE_PF bins of its stride), so that a tile pays two dependent memory latencies, not four
+    int lo0 = 0, n0 = 0, lo1 = 0, n1 = 0;
+    if (live) {
+        lo0 = J.start[t];
+        n0 = J.start[t + 1];
+        if (J.planes == 2) {
+            lo1 = J.start[J.tiles + t];
+            n1 = J.start[J.tiles + t + 1];
+        }
+    }
+    float fix[TILE_PF];
+#pragma unroll
+    for (int i = 0; i < TILE_PF; i++) {
+        const int e = l + i * S, bx = X0 + e / T, by = Y0 + e % T;
+        fix[i] = live && e < TT && bx <= X1 && by <= Y1 ? J.fixed[(long long)bx * J.nby + by] : 0.f;
+    }
     if (live)
         for (int e = l; e < J.planes * TT; e += S) acc[e] = 0ull;
+    n0 -= lo0;
+    n1 -= lo1;
     __syncthreads();
-    for (int pl = 0; live && pl < J.planes; pl++) {
-        const int b = pl * J.tiles + t;
-        const int e1 = J.start[b + 1];
-        unsigned long long* a = acc + pl * TT;
-        // two entries per lane per round, both loaded before either is used
-        for (int e = J.start[b] + l; e < e1; e += 2 * S) {
-            const bool two = e + S < e1;
-            const float4 g0 = J.geo[e];
-            const float r0 = J.gr[e];
-            float4 g1 = g0;
-            float r1 = 0.f;
-            if (two) {
-                g1 = J.geo[e + S];
-                r1 = J.gr[e + S];
+    // the entries of both planes as one range (plane 0 first), two per lane per round, both loaded before either is
+    // used
+    const int n = live ? n0 + n1 : 0;
+    for (int q = l; q < n; q += 2 * S) {
+        float4 g[2];
+        float r[2];
+        int pl[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int x = q + k * S;
+            pl[k] = x >= n0;
+            const int e = x < n0 ? lo0 + x : lo1 + x - n0;
+            if (x < n) {
+                g[k] = J.geo[e];
+                r[k] = J.gr[e];
             }
-            add_clipped(a, T, X0, Y0, X1, Y1, make_span(g0.x, g0.z, J.inv_bx, J.nbx),
-                        make_span(g0.y, g0.w, J.inv_by, J.nby), r0);
-            if (two)
-                add_clipped(a, T, X0, Y0, X1, Y1, make_span(g1.x, g1.z, J.inv_bx, J.nbx),
-                            make_span(g1.y, g1.w, J.inv_by, J.nby), r1);
         }
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            if (q + k * S < n)
+                add_clipped(acc + pl[k] * TT, T, X0, Y0, X1, Y1, make_span(g[k].x, g[k].z, J.inv_bx, J.nbx),
+                            make_span(g[k].y, g[k].w, J.inv_by, J.nby), r[k]);
     }
     // big nodes: one at a time, their bins in this tile spread over the tile's lanes
     const int nbig = live ? *J.big_n : 0;
@@ -368,14 +429,23 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
     }
     __syncthreads();
     double o = 0.0;
-    for (int e = l; live && e < TT; e += S) {
+    auto post = [&](int e, float fixed) {
         const int bx = X0 + e / T, by = Y0 + e % T;
-        if (bx > X1 || by > Y1) continue;
+        if (bx > X1 || by > Y1) return;
         const long long bin = (long long)bx * J.nby + by;
-        const float mov = (float)((double)acc[e] / kDensityScale) + J.fixed[bin];
+        const float mov = (float)((double)acc[e] / kDensityScale) + fixed;
         const float r = J.planes == 2 ? mov + (float)((double)acc[TT + e] / kDensityScale) : mov;
         if (J.rho) J.rho[bin] = r;
         if (J.count_over) o += (double)fmaxf(mov - target, 0.f);
+    };
+#pragma unroll
+    for (int i = 0; i < TILE_PF; i++) {
+        const int e = l + i * S;
+        if (live && e < TT) post(e, fix[i]);
+    }
+    for (int e = l + TILE_PF * S; live && e < TT; e += S) {
+        const int bx = X0 + e / T, by = Y0 + e % T;
+        post(e, bx <= X1 && by <= Y1 ? J.fixed[(long long)bx * J.nby + by] : 0.f);
     }
     // the tile's overflow: wave sums, then the tile's waves in order (G = 1: the order of block_sum)
     o = wave_sum(o);
@@ -387,10 +457,10 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
         part[(judge ? f.tiles : 0) + t] = s;
     }
 }
-__global__ __launch_bounds__(TB) void eng_bin_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
+__global__ __launch_bounds__(TB, 8) void eng_bin_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
     bin_tiles_body(pos_p, f, j, target, part, blockIdx.x);
 }
-__global__ __launch_bounds__(TB) void eng_over_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
+__global__ __launch_bounds__(TB, 8) void eng_over_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
     bin_tiles_body(pos_p, f, j, target, part, f.wgs + blockIdx.x);
 }
 
@@ -408,6 +478,7 @@ struct GatherArgs {
     const float* fy;
     int ld;
     const float2* pin_grad;
+    const float2* cell_wl;    // the cells' WA gradients summed by eng_cell_wl (null: the gather sums the pins)
     const int* cell_start;
     const int* cell_pin;
     const float* pins;
@@ -419,8 +490,35 @@ struct GatherArgs {
     int use_density;          // 0: grad = wl / max(pins, 1) (the weight-0 objective)
 };
 
+// a cell's WA gradient: the sum of its pins' gradients, which the WA kernel wrote in cell order (contiguous)
+__device__ __forceinline__ float2 cell_pin_sum(const GatherArgs& a, int i) {
+    float wx = 0, wy = 0;
+    for (int k = a.cell_start[i], e = a.cell_start[i + 1]; k < e; k++) {
+        const float2 q = a.pin_grad[k];
+        wx += q.x;
+        wy += q.y;
+    }
+    return make_float2(wx, wy);
+}
+
+// The cells' WA gradients and part_wl[b] (their squared norms summed over workgroup b, as the gather would sum
+// them): the part of the gather that needs only WA, run beside the density stage. Same workgroups as the gather.
+__global__ __launch_bounds__(EB) void eng_cell_wl(GatherArgs a, float2* cell_wl, double* part_wl) {
+    __shared__ double red[16];
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * EB + threadIdx.x;
+    double t2[1] = {0.0};
+    if (i < a.num_mov) {
+        const float2 w = cell_pin_sum(a, i);
+        cell_wl[i] = w;
+        t2[0] = (double)w.x * w.x + (double)w.y * w.y;
+    }
+    block_sum<1>(t2, red);
+    if (threadIdx.x == 0) part_wl[blockIdx.x] = t2[0];
+}
+
 // part[b]: |g' - g|^2 of workgroup b; part_wl[b], part_dn[b]: squared norms of the wirelength part and of the
 // weighted density part of the (unpreconditioned) gradient, the learned controller's gradient-ratio feature
+// (part_wl is eng_cell_wl's when a.cell_wl is set)
 __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, double* part_wl, double* part_dn, int wg,
                                             int nwg) {
     __shared__ double red[16 * 3];
@@ -450,16 +548,10 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
             t2[2] = (double)gx * gx + (double)gy * gy;
         }
         if (i < a.num_mov) {
-            float wx = 0, wy = 0;
-            // the WA kernel wrote the pin gradients in cell order: a cell's pins are contiguous
-            for (int k = a.cell_start[i], e = a.cell_start[i + 1]; k < e; k++) {
-                const float2 q = a.pin_grad[k];
-                wx += q.x;
-                wy += q.y;
-            }
-            gx += wx;
-            gy += wy;
-            t2[1] = (double)wx * wx + (double)wy * wy;
+            const float2 w = a.cell_wl ? a.cell_wl[i] : cell_pin_sum(a, i);
+            gx += w.x;
+            gy += w.y;
+            t2[1] = (double)w.x * w.x + (double)w.y * w.y;
         }
         float scale;
         if (a.use_density) {
@@ -480,7 +572,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
     block_sum<3>(t2, red);
     if (threadIdx.x == 0) {
         part[wg] = t2[0];
-        part_wl[wg] = t2[1];
+        if (!a.cell_wl) part_wl[wg] = t2[1];
         part_dn[wg] = t2[2];
     }
 }
@@ -709,6 +801,7 @@ struct Level {
     Poisson poisson;
     WaPlan wa_plan;
     float2 *u1 = nullptr, *v1 = nullptr, *g1 = nullptr, *pin_grad = nullptr;
+    float2* cell_wl = nullptr;   // the cells' WA gradients (eng_cell_wl, two-stream passes)
     int* block_net = nullptr;  // WA workgroup -> first owned net
     int* pin_slot = nullptr;   // pin -> position in the cell CSR (-1: fixed pin)
     BinJob fj{}, jj{};         // density binning of the force grid and of the overflow grid
@@ -844,11 +937,12 @@ int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nod
     if ((st = alloc(&J.count, BIN_MAX_BUCKETS)) || (st = alloc(&J.start, BIN_MAX_BUCKETS + 1)) ||
         (st = alloc(&J.off, num_nodes)) || (st = alloc(&J.geo, 4LL * num_nodes)) ||
         (st = alloc(&J.gr, 4LL * num_nodes)) || (st = alloc(&J.big_count, 1)) || (st = alloc(&J.big_n, 1)) ||
-        (st = alloc(&J.big, num_nodes)))
+        (st = alloc(&J.big, num_nodes)) || (st = alloc(&J.done, 1)))
         return st;
     AP_HIP(hipMemset(J.count, 0, BIN_MAX_BUCKETS * sizeof(int)));
     AP_HIP(hipMemset(J.big_count, 0, sizeof(int)));
     AP_HIP(hipMemset(J.big_n, 0, sizeof(int)));
+    AP_HIP(hipMemset(J.done, 0, sizeof(int)));
     update_lds();
     return AP_OK;
 }
@@ -902,19 +996,15 @@ int Level::set_lds() const {
     return AP_OK;
 }
 
-// count, scan and scatter of one grid (0: force, 1: overflow)
+// count (with the scan in its last workgroup) and scatter of one grid (0: force, 1: overflow)
 int Level::bin_grid(int which, const Pick& pos, hipStream_t s) {
     if (!which) {
         hipLaunchKernelGGL(eng_bin_count, dim3(fj.blocks), dim3(EB), hist_f, s, pos, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_bin_scan, dim3(1), dim3(SCAN_T), 0, s, fj, jj);
         AP_LAUNCHED();
         hipLaunchKernelGGL(eng_bin_scatter, dim3(fj.blocks), dim3(EB), 0, s, pos, fj, jj);
         AP_LAUNCHED();
     } else {
         hipLaunchKernelGGL(eng_over_count, dim3(jj.blocks), dim3(EB), hist_j, s, pos, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_over_scan, dim3(1), dim3(SCAN_T), 0, s, fj, jj);
         AP_LAUNCHED();
         hipLaunchKernelGGL(eng_over_scatter, dim3(jj.blocks), dim3(EB), 0, s, pos, fj, jj);
         AP_LAUNCHED();
@@ -932,10 +1022,11 @@ int Level::tiles_grid(int which, const Pick& pos, hipStream_t s) {
     return AP_OK;
 }
 
-// The front of a pass as the passes run it -- WA and the overflow grid's binning and tiles on the side stream, the
-// force grid's binning, tiles and Poisson solve on s -- at the caller's current point v (between runs the current
-// point is always set 0), `reps` times: ms per run. It changes nothing a pass reads before writing it again (bucket
-// counts end at zero; pin gradients, partials, rho and the field are rewritten by every pass).
+// The front of a pass as the passes run it -- the overflow grid's binning and tiles, WA and the cells' WA gradients
+// on the side stream, the force grid's binning, tiles and Poisson solve on s -- at the caller's current point v
+// (between runs the current point is always set 0), `reps` times: ms per run. It changes nothing a pass reads before
+// writing it again (bucket counts end at zero; pin and cell gradients, partials, rho and the field are rewritten by
+// every pass).
 int Level::time_density(int reps, hipStream_t s, float& ms) {
     const Pick pos{(const float2*)d.v, nullptr, nullptr};
     int st;
@@ -944,8 +1035,11 @@ int Level::time_density(int reps, hipStream_t s, float& ms) {
     for (int r = 0; r < reps; r++) {
         AP_HIP(hipEventRecord(ev_fork, s));
         AP_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-        if ((st = wa_plan.launch(wa_args(pos, d.kp), p_wa, side))) return st;
         if (jj.count && ((st = bin_grid(1, pos, side)) || (st = tiles_grid(1, pos, side)))) return st;
+        if ((st = wa_plan.launch(wa_args(pos, d.kp), p_wa, side))) return st;
+        hipLaunchKernelGGL(eng_cell_wl, dim3(nblk_n), dim3(EB), 0, side,
+                           gather_args(pos, d.kp, nullptr, nullptr, nullptr, nullptr, 1), cell_wl, p_wl2);
+        AP_LAUNCHED();
         if ((st = bin_grid(0, pos, s)) || (st = tiles_grid(0, pos, s))) return st;
         if ((st = poisson.solve(rho, d.nby, fx, fy, d.nby, s))) return st;
         AP_HIP(hipEventRecord(ev_join, side));
@@ -965,8 +1059,11 @@ int Level::time_density(int reps, hipStream_t s, float& ms) {
 int Level::autotune(int reps, hipStream_t s, int* sides) {
     AP_HIP(hipDeviceSynchronize());
     int st;
-    for (BinJob* J : {&fj, &jj}) {
-        if (!J->count) continue;
+    // coordinate descent: the force grid, the overflow grid, then the force grid again with the overflow grid's
+    // choice in place (the two share the GPU in a pass, so each one's best depends on the other's)
+    BinJob* order[3] = {&fj, &jj, &fj};
+    for (int sweep = 0; sweep < (jj.count ? 3 : 1); sweep++) {
+        BinJob* J = order[sweep];
         int best = J->T, best_g = J->G;
         float best_ms = 1e30f;
         for (int T = 8; T <= 64; T *= 2) {
@@ -1026,6 +1123,7 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipMalloc(&v1, N * sizeof(float2)));
     AP_HIP(hipMalloc(&g1, N * sizeof(float2)));
     AP_HIP(hipMalloc(&pin_grad, (d.num_pins + 1) * sizeof(float2)));
+    AP_HIP(hipMalloc(&cell_wl, ((size_t)d.num_mov + 1) * sizeof(float2)));
     AP_HIP(hipMalloc(&pin_slot, (d.num_pins + 1) * sizeof(int)));
     if (d.num_pins > 0) {
         hipLaunchKernelGGL(eng_pin_slot_clear, dim3(grid_of(d.num_pins, EB)), dim3(EB), 0, nullptr, pin_slot,
@@ -1087,7 +1185,7 @@ void Level::destroy() {
     last_ev = nullptr;
     for (hipEvent_t& ev : pev)
         if (ev) (void)hipEventDestroy(ev);
-    void* ptrs[] = {u1, v1, g1, pin_grad, rho, fx, fy, part, stats, flags, red, ctl_x, block_net, pin_slot};
+    void* ptrs[] = {u1, v1, g1, pin_grad, cell_wl, rho, fx, fy, part, stats, flags, red, ctl_x, block_net, pin_slot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : bin_mem) (void)hipFree(p);
@@ -1167,15 +1265,30 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
         AP_HIP(hipEventRecord(ev_fork, s));
         AP_HIP(hipStreamWaitEvent(side, ev_fork, 0));
     }
-    mark(AP_PROF_WA, s);
-    tbegin(AP_PROF_WA, w);
-    if ((st = wa_plan.launch(wa_args(pos, kp_ptr), p_wa, w))) return st;
-    tend(AP_PROF_WA, w);
+    // the side stream: the overflow grid, then WA (which overlaps the force grid's tiles and Poisson solve, the
+    // latency-bound part of the front); profiled passes keep the class order on s
+    GatherArgs g = gather_args(pos, kp_ptr, g_prev, g_prev_b, g_out_a, g_out_b, use_density);
+    auto wa = [&]() -> int {
+        mark(AP_PROF_WA, s);
+        tbegin(AP_PROF_WA, w);
+        const int r = wa_plan.launch(wa_args(pos, kp_ptr), p_wa, w);
+        tend(AP_PROF_WA, w);
+        if (r) return r;
+        // the cells' WA gradients, off the gather's critical path
+        mark(AP_PROF_WA_CELLS, s);
+        tbegin(AP_PROF_WA_CELLS, w);
+        hipLaunchKernelGGL(eng_cell_wl, dim3(nblk_n), dim3(EB), 0, w, g, cell_wl, p_wl2);
+        AP_LAUNCHED();
+        tend(AP_PROF_WA_CELLS, w);
+        g.cell_wl = cell_wl;
+        return AP_OK;
+    };
+    if (!two && (st = wa())) return st;
     if (use_density) {
         mark(AP_PROF_MAP, s);
         tbegin(AP_PROF_MAP, s);
-        if ((st = bin_grid(0, pos, s))) return st;
         if (over && (st = bin_grid(1, pos, w))) return st;
+        if ((st = bin_grid(0, pos, s))) return st;
         tend(AP_PROF_MAP, s);
         mark(AP_PROF_POST, s);
         tbegin(AP_PROF_POST, s);
@@ -1190,11 +1303,11 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
         if ((st = poisson.solve(rho, d.nby, fx, fy, d.nby, s))) return st;
         tend(AP_PROF_POISSON, s);
     }
+    if (two && (st = wa())) return st;
     if (two) {
         AP_HIP(hipEventRecord(ev_join, side));
         AP_HIP(hipStreamWaitEvent(s, ev_join, 0));
     }
-    const GatherArgs g = gather_args(pos, kp_ptr, g_prev, g_prev_b, g_out_a, g_out_b, use_density);
     mark(AP_PROF_GATHER, s);
     tbegin(AP_PROF_GATHER, s);
     hipLaunchKernelGGL(eng_gather, dim3(nblk_n), dim3(EB), 0, s, g, p_turned, p_wl2, p_dn2);
@@ -1318,7 +1431,7 @@ __global__ __launch_bounds__(EB) void eng_bin_count_grouped(const PassTab* tab, 
     const int4 m = map[blockIdx.x];
     if (stopped(tab, m.x)) return;
     const PassTab& T = tab[m.x];
-    bin_count_body(T.pos, T.fj, T.jj, m.y);
+    bin_count_body(T.pos, T.fj, T.jj, m.y, false);
 }
 __global__ __launch_bounds__(SCAN_T) void eng_bin_scan_grouped(const PassTab* tab, const int4* map) {
     const int4 m = map[blockIdx.x];

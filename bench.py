@@ -39,8 +39,8 @@ PEAK_FP32_MFMA = 157.3e12    # flop/s
 
 # kernel classes made of one kernel (the others launch several: binning = count + scan + scatter per grid,
 # Poisson = 5, control = reduce + control)
-SINGLE_KERNEL = {"nesterov": "eng_nesterov", "wa": "wa_wave_kernel", "density_tiles": "eng_bin_tiles",
-                 "overflow_tiles": "eng_over_tiles", "gather": "eng_gather"}
+SINGLE_KERNEL = {"nesterov": "eng_nesterov", "wa": "wa_wave_kernel", "wa_cells": "eng_cell_wl",
+                 "density_tiles": "eng_bin_tiles", "overflow_tiles": "eng_over_tiles", "gather": "eng_gather"}
 
 
 def algorithmic_bytes(cls, dims):
@@ -51,9 +51,11 @@ def algorithmic_bytes(cls, dims):
     return {
         # u, v, g, lo, hi read; u', v' written (8 B each per node)
         "nesterov": 56 * N,
-        # per pin: cell 4 + offset 8 + position 8 read, gradient 8 written (the cell sum is the gather's);
+        # per pin: cell 4 + offset 8 + position 8 read, gradient 8 written (the cell sums are wa_cells');
         # per net: its start 4
         "wa": 28 * P + 4 * nets,
+        # per pin: its gradient 8 read; per cell: cell_start 4 read, its gradient 8 written
+        "wa_cells": 8 * P + 12 * M,
         # per node and grid: position 8 + size 8 + ratio 4 read (force grid: all nodes; overflow grid: cells)
         "density_bin": 20 * N + 20 * M,
         # the force map: fixed 4 read and density 4 written per bin
@@ -61,8 +63,8 @@ def algorithmic_bytes(cls, dims):
         # the overflow map: fixed 4 read per bin
         "overflow_tiles": 4 * nj2,
         # per node: position 8, size 8, ratio 4, pins 4, area 4, previous gradient 8 read, gradient 8 written;
-        # per cell: cell_start 4; pin gradients 8 per pin; the field 8 per bin
-        "gather": 44 * N + 4 * M + 8 * P + 8 * nb2,
+        # per cell: its WA gradient 8; the field 8 per bin
+        "gather": 44 * N + 8 * M + 8 * nb2,
     }.get(cls)
 
 

@@ -156,13 +156,14 @@ AP_API int ap_level_prepare(ap_level* level, int passes);
 /* Kernel classes of a pass, in launch order, for ap_level_profile and ap_level_time_class. */
 #define AP_PROF_NESTEROV 0
 #define AP_PROF_WA 1
-#define AP_PROF_MAP 2         /* density: tile binning of both grids (count, scan, scatter) */
-#define AP_PROF_POST 3        /* density: per-tile sums of the force grid, density and overflow */
-#define AP_PROF_OVER_TILES 4  /* density: per-tile sums of the overflow grid (when it has one) */
-#define AP_PROF_POISSON 5
-#define AP_PROF_GATHER 6
-#define AP_PROF_CONTROL 7
-#define AP_PROF_CLASSES 8
+#define AP_PROF_WA_CELLS 2    /* the cells' WA gradients (sums of their pins') */
+#define AP_PROF_MAP 3         /* density: tile binning of both grids (count, scan, scatter) */
+#define AP_PROF_POST 4        /* density: per-tile sums of the force grid, density and overflow */
+#define AP_PROF_OVER_TILES 5  /* density: per-tile sums of the overflow grid (when it has one) */
+#define AP_PROF_POISSON 6
+#define AP_PROF_GATHER 7
+#define AP_PROF_CONTROL 8
+#define AP_PROF_CLASSES 9
 /* Run `passes` passes with direct launches and HIP events around every kernel class; ms[AP_PROF_CLASSES] receives
  * the average milliseconds per pass of each class. Blocks the calling thread until the passes are done. */
 AP_API int ap_level_profile(ap_level* level, int passes, double* ms, void* stream);
