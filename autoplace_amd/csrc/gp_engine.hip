@@ -332,6 +332,25 @@ __global__ __launch_bounds__(EB) void eng_over_scatter(Pick pos_p, BinJob f, Bin
 __device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0, int Y0, int X1, int Y1,
                                             const Span& sx, const Span& sy, float r) {
     const int x0 = max(sx.b0, X0), x1 = min(sx.b1, X1), y0 = max(sy.b0, Y0), y1 = min(sy.b1, Y1);
+    if (x1 - x0 < 3 && y1 - y0 < 3) {
+        // at most 3 x 3 bins (every node whose smoothed size is under 2 bins): straight-line code over a fixed
+        // 3 x 3 window, the weights of each axis computed once; the same float products as the loop below
+        float wx[3], wy[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            wx[k] = x0 + k <= x1 ? r * span_overlap(sx, x0 + k) : 0.f;
+            wy[k] = y0 + k <= y1 ? span_overlap(sy, y0 + k) : 0.f;
+        }
+        unsigned long long* p = a + (x0 - X0) * T + (y0 - Y0);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const unsigned long long c = (unsigned long long)((double)(wx[i] * wy[j]) * kDensityScale);
+                if (c) atomicAdd(p + i * T + j, c);
+            }
+        return;
+    }
     for (int bx = x0; bx <= x1; bx++) {
         const float wx = r * span_overlap(sx, bx);
         unsigned long long* row = a + (bx - X0) * T - Y0;
@@ -533,14 +552,42 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
             const float2 p = pos[i], sz = a.size[i];
             const float r = a.ratio[i];
             Span sx = make_span(p.x, sz.x, a.inv_bx, a.nbx), sy = make_span(p.y, sz.y, a.inv_by, a.nby);
-            for (int bx = sx.b0; bx <= sx.b1; bx++) {
-                const float wx = r * span_overlap(sx, bx);
-                const float* fxr = a.fx + (long long)bx * a.ld;
-                const float* fyr = a.fy + (long long)bx * a.ld;
-                for (int by = sy.b0; by <= sy.b1; by++) {
-                    const float w = wx * span_overlap(sy, by);
-                    gx -= w * fxr[by];
-                    gy -= w * fyr[by];
+            if (sx.b1 - sx.b0 < 3 && sy.b1 - sy.b0 < 3) {
+                // at most 3 x 3 bins: a fixed window, each axis's weights computed once, the field loads issued
+                // together; the terms in the loop's order (outside bins have weight 0 and add nothing)
+                float wx[3], wy[3], ex[3][3], ey[3][3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    wx[k] = sx.b0 + k <= sx.b1 ? r * span_overlap(sx, sx.b0 + k) : 0.f;
+                    wy[k] = sy.b0 + k <= sy.b1 ? span_overlap(sy, sy.b0 + k) : 0.f;
+                }
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const bool in = sx.b0 + i <= sx.b1 && sy.b0 + j <= sy.b1;
+                        const long long o = (long long)(sx.b0 + i) * a.ld + sy.b0 + j;
+                        ex[i][j] = in ? a.fx[o] : 0.f;
+                        ey[i][j] = in ? a.fy[o] : 0.f;
+                    }
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const float w = wx[i] * wy[j];
+                        gx -= w * ex[i][j];
+                        gy -= w * ey[i][j];
+                    }
+            } else {
+                for (int bx = sx.b0; bx <= sx.b1; bx++) {
+                    const float wx = r * span_overlap(sx, bx);
+                    const float* fxr = a.fx + (long long)bx * a.ld;
+                    const float* fyr = a.fy + (long long)bx * a.ld;
+                    for (int by = sy.b0; by <= sy.b1; by++) {
+                        const float w = wx * span_overlap(sy, by);
+                        gx -= w * fxr[by];
+                        gy -= w * fyr[by];
+                    }
                 }
             }
             gx *= weight;
