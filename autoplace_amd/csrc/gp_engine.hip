@@ -406,7 +406,10 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
     // the entries of both planes as one range (plane 0 first), two per lane per round, both loaded before either is
     // used
     const int n = live ? n0 + n1 : 0;
-    for (int q = l; q < n; q += 2 * S) {
+    // A bucket lists its nodes about in curve order, so consecutive entries hit the same bins. The lanes of one wave
+    // take entries S / 64 apart (lane i of wave w: entry i S / 64 + w), which spreads a wave's atomics over the tile.
+    const int lp = (l & 63) * (S >> 6) + (l >> 6);
+    for (int q = lp; q < n; q += 2 * S) {
         float4 g[2];
         float r[2];
         int pl[2];
