@@ -229,7 +229,7 @@ __global__ __launch_bounds__(EB) void eng_over_count(Pick pos_p, BinJob f, BinJo
 // the bucket starts from the counts of the pass (exclusive scan by the NT lanes of one workgroup, 4 NT buckets a
 // round, coalesced), the counts back to zero for the next pass. SEEN_BY_ATOMICS: the counts (and the big-node count)
 // are read with device-scope atomic loads, inside the kernel whose workgroups added them.
-template <int NT, bool SEEN_BY_ATOMICS>
+template <bool SEEN_BY_ATOMICS>
 __device__ __forceinline__ int bucket_load(const int* p) {
     return SEEN_BY_ATOMICS ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
 }
@@ -244,7 +244,7 @@ __device__ __forceinline__ void scan_buckets(const BinJob& J) {
         int c[4], sum = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            c[i] = k0 + i < n ? bucket_load<NT, SEEN_BY_ATOMICS>(J.count + k0 + i) : 0;
+            c[i] = k0 + i < n ? bucket_load<SEEN_BY_ATOMICS>(J.count + k0 + i) : 0;
             sum += c[i];
         }
         int x = sum;                                // inclusive scan of the lanes' sums
@@ -272,7 +272,7 @@ __device__ __forceinline__ void scan_buckets(const BinJob& J) {
     }
     if (threadIdx.x == 0) {
         J.start[n] = carry;
-        *J.big_n = bucket_load<NT, SEEN_BY_ATOMICS>(J.big_count);
+        *J.big_n = bucket_load<SEEN_BY_ATOMICS>(J.big_count);
         *J.big_count = 0;
     }
 }

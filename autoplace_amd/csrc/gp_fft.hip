@@ -28,6 +28,10 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ int bitrev(int n, int lg) { return (int)(__brev((unsigned)n) >> (32 - lg)); }
+// LDS place of FFT element i: one pad slot per 8 elements, which breaks the power-of-two strides of the butterflies'
+// accesses across the LDS banks (their conflict degree about halves); an FFT of L elements takes lds_len(L) slots
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
+__host__ __device__ __forceinline__ int lds_len(int L) { return L + (L >> 3); }
 // position of input element m in Makhoul's order: even m -> m / 2, odd m -> L - 1 - (m - 1) / 2
 __device__ __forceinline__ int makhoul(int m, int L) { return (m & 1) ? L - 1 - (m >> 1) : (m >> 1); }
 
@@ -49,14 +53,15 @@ __device__ void fft_lds(float2* s, const float2* tw, int L, int lg, bool inverse
             w2.y *= sg;
             // W_{4h}^{j+h} = W_{4h}^j * W_4^1, and W_4^1 = -i (forward) or +i (inverse)
             const float2 w3 = make_float2(sg * w2.y, -sg * w2.x);
-            const float2 a0 = s[i0], a1 = cmul(s[i0 + h], w1), a2 = s[i0 + 2 * h], a3 = cmul(s[i0 + 3 * h], w1);
+            const float2 a0 = s[lpad(i0)], a1 = cmul(s[lpad(i0 + h)], w1);
+            const float2 a2 = s[lpad(i0 + 2 * h)], a3 = cmul(s[lpad(i0 + 3 * h)], w1);
             const float2 b0 = make_float2(a0.x + a1.x, a0.y + a1.y), b1 = make_float2(a0.x - a1.x, a0.y - a1.y);
             const float2 b2 = cmul(make_float2(a2.x + a3.x, a2.y + a3.y), w2);
             const float2 b3 = cmul(make_float2(a2.x - a3.x, a2.y - a3.y), w3);
-            s[i0] = make_float2(b0.x + b2.x, b0.y + b2.y);
-            s[i0 + 2 * h] = make_float2(b0.x - b2.x, b0.y - b2.y);
-            s[i0 + h] = make_float2(b1.x + b3.x, b1.y + b3.y);
-            s[i0 + 3 * h] = make_float2(b1.x - b3.x, b1.y - b3.y);
+            s[lpad(i0)] = make_float2(b0.x + b2.x, b0.y + b2.y);
+            s[lpad(i0 + 2 * h)] = make_float2(b0.x - b2.x, b0.y - b2.y);
+            s[lpad(i0 + h)] = make_float2(b1.x + b3.x, b1.y + b3.y);
+            s[lpad(i0 + 3 * h)] = make_float2(b1.x - b3.x, b1.y - b3.y);
         }
         __syncthreads();
     }
@@ -67,9 +72,9 @@ __device__ void fft_lds(float2* s, const float2* tw, int L, int lg, bool inverse
             const int i0 = ((b >> (st - 1)) << st) + j, i1 = i0 + half;
             float2 w = tw[j * stride];
             w.y *= sg;
-            const float2 a = s[i0], c = cmul(s[i1], w);
-            s[i0] = make_float2(a.x + c.x, a.y + c.y);
-            s[i1] = make_float2(a.x - c.x, a.y - c.y);
+            const float2 a = s[lpad(i0)], c = cmul(s[lpad(i1)], w);
+            s[lpad(i0)] = make_float2(a.x + c.x, a.y + c.y);
+            s[lpad(i1)] = make_float2(a.x - c.x, a.y - c.y);
         }
         __syncthreads();
     }
@@ -91,17 +96,19 @@ __device__ __forceinline__ void dct_pair_at(float2 Z, float2 Zr, float2 p, float
 // DCT-II of the rows r0 = 2 blockIdx.x, r0 + 1 (real, length L) -> out rows; spectrum of the pair from one FFT
 __device__ void dct2_pair_to_lds(const float* in0, const float* in1, float2* s, const float2* tw_s, const float2* ph,
                                  int L, int lg, float* X0, float* X1) {
-    for (int m = threadIdx.x; m < L; m += blockDim.x) s[bitrev(makhoul(m, L), lg)] = make_float2(in0[m], in1[m]);
+    for (int m = threadIdx.x; m < L; m += blockDim.x)
+        s[lpad(bitrev(makhoul(m, L), lg))] = make_float2(in0[m], in1[m]);
     __syncthreads();
     fft_lds(s, tw_s, L, lg, false);
-    for (int k = threadIdx.x; k < L; k += blockDim.x) dct_pair_at(s[k], s[(L - k) & (L - 1)], ph[k], X0[k], X1[k]);
+    for (int k = threadIdx.x; k < L; k += blockDim.x)
+        dct_pair_at(s[lpad(k)], s[lpad((L - k) & (L - 1))], ph[k], X0[k], X1[k]);
 }
 
 __device__ __forceinline__ void pass_a_body(const float* rho, int ld_in, float* T, int ld_out, const float2* tw,
                                             const float2* ph, int L, int lg, int wg) {
     extern __shared__ float2 lds[];
     float2* s = lds;
-    float2* tw_s = lds + L;
+    float2* tw_s = lds + lds_len(L);
     load_twiddles(tw_s, tw, L);
     const int r0 = 2 * wg;
     __syncthreads();
@@ -125,7 +132,7 @@ __device__ __forceinline__ void cs_put(float2* s, const float2* ph, int lg, int 
                                        float xsr) {
     const float2 p = ph[k];   // e^{i pi k / 2L}
     const float2 vc = cmul(p, make_float2(xc, -xcr)), vs = cmul(p, make_float2(xs, -xsr));
-    s[bitrev(k, lg)] = make_float2(vc.x - vs.y, vc.y + vs.x);   // vc + i vs
+    s[lpad(bitrev(k, lg))] = make_float2(vc.x - vs.y, vc.y + vs.x);   // vc + i vs
 }
 // the unit (k, L - k): c / cr = a_cos[k] / a_cos[L-k], sv / sr = a_sin[k] / a_sin[L-k] (X'[0] = 2 a_cos[0] and 0 for
 // the reversed sine input; at k = L / 2 both indices are the same)
@@ -143,7 +150,7 @@ __device__ __forceinline__ void cs_put_unit(float2* s, const float2* ph, int L, 
 // after the inverse FFT of h: y_cos[j], y_sin[j] (Makhoul's output order)
 __device__ __forceinline__ void cs_store(const float2* s, int L, float* y_cos, float* y_sin) {
     for (int j = threadIdx.x; j < L; j += blockDim.x) {
-        const float2 y = s[makhoul(j, L)];
+        const float2 y = s[lpad(makhoul(j, L))];
         y_cos[j] = 0.5f * y.x;
         y_sin[j] = (j & 1) ? -0.5f * y.y : 0.5f * y.y;
     }
@@ -174,12 +181,13 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
     extern __shared__ float2 lds[];
     const int L = a.N, lg = a.lgN, half = L >> 1;
     float2* s = lds;
-    float2* tw_s = lds + L;
+    float2* tw_s = lds + lds_len(L);
     load_twiddles(tw_s, a.tw, L);
     const int l0 = 2 * wg;
     const float* in0 = a.Tt + (long long)l0 * L;
     const float* in1 = in0 + L;
-    for (int m = threadIdx.x; m < L; m += blockDim.x) s[bitrev(makhoul(m, L), lg)] = make_float2(in0[m], in1[m]);
+    for (int m = threadIdx.x; m < L; m += blockDim.x)
+        s[lpad(bitrev(makhoul(m, L), lg))] = make_float2(in0[m], in1[m]);
     __syncthreads();
     fft_lds(s, tw_s, L, lg, false);
     // DCT-II of both rows at this lane's units: c[u][row][0 at k, 1 at L - k]
@@ -189,7 +197,7 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
         const int k = threadIdx.x + u * FB;
         c[u][0][0] = c[u][0][1] = c[u][1][0] = c[u][1][1] = 0.f;
         if (k > half) continue;
-        const float2 Zk = s[k], Zr = s[(L - k) & (L - 1)];
+        const float2 Zk = s[lpad(k)], Zr = s[lpad((L - k) & (L - 1))];
         dct_pair_at(Zk, Zr, a.ph[k], c[u][0][0], c[u][1][0]);
         if (k && 2 * k != L) dct_pair_at(Zr, Zk, a.ph[L - k], c[u][0][1], c[u][1][1]);
     }
@@ -221,7 +229,7 @@ __device__ __forceinline__ void pass_c_body(const float* U, const float* V, floa
     extern __shared__ float2 lds[];
     const int half = L >> 1;
     float2* s = lds;
-    float2* tw_s = lds + L;
+    float2* tw_s = lds + lds_len(L);
     load_twiddles(tw_s, tw, L);
     const float* u_row = U + (long long)n * L;
     const float* v_row = V + (long long)n * L;
@@ -317,8 +325,8 @@ int FftGroup::create(const std::vector<FftTab>& tabs) {
             for (int y = 0; y < (int)grid_of(t.M, TT); y++)
                 for (int x = 0; x < (int)grid_of(t.N, TT); x++) m[3].push_back(make_int4(g, x, y, z));
         for (int y = 0; y < t.N; y++) m[4].push_back(make_int4(g, y, 0, 0));
-        lds_ac = std::max(lds_ac, (size_t)(t.M + t.M / 2) * sizeof(float2));
-        lds_b = std::max(lds_b, (size_t)(t.N + t.N / 2) * sizeof(float2));
+        lds_ac = std::max(lds_ac, (size_t)(lds_len(t.M) + t.M / 2) * sizeof(float2));
+        lds_b = std::max(lds_b, (size_t)(lds_len(t.N) + t.N / 2) * sizeof(float2));
     }
     for (int k = 0; k < 5; k++) {
         count[k] = (int)m[k].size();
@@ -399,8 +407,10 @@ int Poisson::fft_create() {
     // per side: twiddles e^{-2 pi i j / L} (L / 2) then phases e^{i pi k / 2L} (L); omega / nu tables
     std::vector<float2> h;
     auto add = [&](int L) {
-        for (int j = 0; j < L / 2; j++) h.push_back(make_float2((float)cos(-2 * M_PI * j / L), (float)sin(-2 * M_PI * j / L)));
-        for (int k = 0; k < L; k++) h.push_back(make_float2((float)cos(M_PI * k / (2.0 * L)), (float)sin(M_PI * k / (2.0 * L))));
+        for (int j = 0; j < L / 2; j++)
+            h.push_back(make_float2((float)cos(-2 * M_PI * j / L), (float)sin(-2 * M_PI * j / L)));
+        for (int k = 0; k < L; k++)
+            h.push_back(make_float2((float)cos(M_PI * k / (2.0 * L)), (float)sin(M_PI * k / (2.0 * L))));
     };
     add(N);
     add(M);
@@ -431,21 +441,22 @@ int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, int f
     const float2* phM = twM + M / 2;
     float *w0 = d_work, *w1 = d_work + nm, *w2 = d_work + 2 * nm, *w3 = d_work + 3 * nm;
     // pass A: T = rho Cy^T (rows of length M), into w0
-    hipLaunchKernelGGL(fft_pass_a, dim3(N / 2), dim3(FB), (size_t)(M + M / 2) * sizeof(float2), s, rho, rho_ld, w0,
-                       M, twM, phM, M, lgM);
+    const size_t lds_m = (size_t)(lds_len(M) + M / 2) * sizeof(float2);
+    const size_t lds_n = (size_t)(lds_len(N) + N / 2) * sizeof(float2);
+    hipLaunchKernelGGL(fft_pass_a, dim3(N / 2), dim3(FB), lds_m, s, rho, rho_ld, w0, M, twM, phM, M, lgM);
     AP_LAUNCHED();
     hipLaunchKernelGGL(transpose_kernel, dim3(grid_of(M, TT), grid_of(N, TT), 1), dim3(256), 0, s, w0, w0, w1, w1, N,
                        M);
     AP_LAUNCHED();
     // pass B: rows of Tt (length N) -> Ut (w2), Vt (w3)
     PassB b{w1, w2, w3, d_omega, d_omega + N, (float)(4.0 / ((double)N * M)), (float)bx, N, lgN, twN, phN};
-    hipLaunchKernelGGL(fft_pass_b, dim3(M / 2), dim3(FB), (size_t)(N + N / 2) * sizeof(float2), s, b);
+    hipLaunchKernelGGL(fft_pass_b, dim3(M / 2), dim3(FB), lds_n, s, b);
     AP_LAUNCHED();
     hipLaunchKernelGGL(transpose_kernel, dim3(grid_of(N, TT), grid_of(M, TT), 2), dim3(256), 0, s, w2, w3, w0, w1, M,
                        N);
     AP_LAUNCHED();
     // pass C: rows of U (w0), V (w1) -> field
-    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), (size_t)(M + M / 2) * sizeof(float2), s, w0, w1, fx, fy, fld,
+    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), lds_m, s, w0, w1, fx, fy, fld,
                        twM, phM, M, lgM);
     AP_LAUNCHED();
     return AP_OK;
