@@ -112,8 +112,7 @@ def coarse_problem(data, node_size, wirelength, density, fixed_fraction, args, p
     sel = ~repeat
     off = torch.where(keep_off[sel].unsqueeze(1), pins["off"][sel], torch.zeros(1, 2, device=device))
     c_owner, c_net = owner[sel], pins["net"][sel]
-    cwl = Wirelength(c_owner.cpu().numpy(), off.cpu().numpy(), c_net.cpu().numpy(), pins["num_nets"], num,
-                     args.ignore_net_degree, device)
+    cwl = Wirelength(c_owner, off, c_net, pins["num_nets"], num, args.ignore_net_degree, device)   # CSR on the device
     cpins = torch.bincount(c_owner, minlength=num + 1)[:num].double().cpu().numpy()
     cnode = np.concatenate([csize.double().cpu().numpy(), node_size[n:]])
     judge_bin = density.judge.nbx if density.judge is not None else density.nbx

@@ -119,6 +119,7 @@ struct BinJob {
     float inv_bx, inv_by;
     int nbx, nby;
     int T, tiles_y, tiles;     // tile side in bins, tiles per column, tiles
+    int T0;                    // the side chosen from the node density at set-up (the autotuner searches around it)
     int G, wgs;                // tiles per workgroup of the tile kernel (1, 2, 4, 8: TB / G lanes each), its
                                // workgroups
     int planes;                // 2: cells + fillers, 1: cells only
@@ -971,7 +972,7 @@ int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nod
     J.planes = planes;
     std::vector<float2> host_size(num_nodes);
     AP_HIP(hipMemcpy(host_size.data(), size, (size_t)num_nodes * sizeof(float2), hipMemcpyDeviceToHost));
-    J.T = tile_side(host_size, bx, by, nbx, nby, planes);
+    J.T = J.T0 = tile_side(host_size, bx, by, nbx, nby, planes);
     if (!J.T) {
         set_error("level: density grid %d x %d is beyond the tile binning's %d buckets", nbx, nby, BIN_MAX_BUCKETS);
         return AP_ERR_UNSUPPORTED;
@@ -1002,13 +1003,9 @@ bool Level::side_ok(const BinJob& J, int T) const {
     return T >= 8 && T <= 64 && (long long)J.planes * grid_of(J.nbx, T) * grid_of(J.nby, T) <= BIN_MAX_BUCKETS;
 }
 
-// tiles per workgroup of the tile kernel, by default: as many as keep the workgroup's accumulators within 4096 bins
-// (32 KB of LDS), 8 at most (a tile keeps one wave at least)
-static int default_group(int planes, int T) {
-    int G = 1;
-    while (G < 8 && 2 * G * planes * T * T <= 4096) G *= 2;
-    return G;
-}
+// tiles per workgroup of the tile kernel, by default: one (a tile's bucket spread over all 512 lanes; on the 2M-cell
+// design this measured best or within 2 % of the best for every side of both grids)
+static int default_group(int, int) { return 1; }
 
 // a group the tile kernel can use for side T: 1, 2, 4 or 8 tiles per workgroup, within 64 KB of LDS
 static bool group_ok(int planes, int T, int G) {
@@ -1102,9 +1099,9 @@ int Level::time_density(int reps, hipStream_t s, float& ms) {
     return AP_OK;
 }
 
-// Autotuner of the tile layout: for each grid (force, then overflow), every admissible side from 8 to 64 with every
-// admissible group (tiles per workgroup) is timed on the front of a pass (time_density) at the current positions and
-// the fastest kept. The map does not depend on either (integer sums), so the choice changes the time only, never
+// Autotuner of the tile layout: for each grid (force, then overflow), the admissible sides from half to twice the
+// set-up side (8 .. 64) with 1 or 2 tiles per workgroup are timed on the front of a pass (time_density) at the
+// current positions and the fastest kept. The map does not depend on either (integer sums), so the choice changes the time only, never
 // the results.
 int Level::autotune(int reps, hipStream_t s, int* sides) {
     AP_HIP(hipDeviceSynchronize());
@@ -1116,9 +1113,9 @@ int Level::autotune(int reps, hipStream_t s, int* sides) {
         BinJob* J = order[sweep];
         int best = J->T, best_g = J->G;
         float best_ms = 1e30f;
-        for (int T = 8; T <= 64; T *= 2) {
+        for (int T = std::max(8, J->T0 / 2); T <= std::min(64, 2 * J->T0); T *= 2) {
             if (!side_ok(*J, T)) continue;
-            for (int G = 1; G <= 8; G *= 2) {
+            for (int G = 1; G <= 2; G *= 2) {
                 if (!group_ok(J->planes, T, G)) break;
                 set_side(*J, T, G);
                 update_lds();

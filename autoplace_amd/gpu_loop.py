@@ -99,8 +99,8 @@ class Engine(object):
         _lib.check(_lib.lib().ap_level_run(self._h, passes, int(use_graph), stream), "level_run")
 
     def autotune(self, reps=5):
-        """Time every admissible density tile layout (side, tiles per workgroup) at the current point and keep the
-        fastest (results unchanged). -> ((force side, group), (overflow side, group) or (0, 0))."""
+        """Time the density tile layouts around the set-up ones (side, tiles per workgroup) at the current point and
+        keep the fastest (results unchanged). -> ((force side, group), (overflow side, group) or (0, 0))."""
         lay = (ctypes.c_int * 4)()
         _lib.check(_lib.lib().ap_level_autotune(self._h, int(reps), lay), "level_autotune")
         return (lay[0], lay[1]), (lay[2], lay[3])
@@ -211,8 +211,8 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
     hook(objective, u, v, g, S, kp, c), if given, sees the state the device loop starts from (instrumentation).
     controller: None (the plain rule) or a dict for the learned controller (controller.py: `attach`), whose "levels"
     list receives this level's record (passes under the controller, fallback, the logged passes).
-    autotune: time the density tile sides at the start and keep the fastest (None: for levels of 100k nodes and
-    more); the results do not depend on it."""
+    autotune: time the density tile layouts at the start and keep the fastest (None: for the cell level when it has
+    100k nodes or more -- the long level, where the timing pays for itself); the results do not depend on it."""
     device, pos = optimizer.v.device, optimizer.v
     S, kp, c = control_state(ps, optimizer, iteration, args)
     S = torch.tensor(S, dtype=torch.float64, device=device)
@@ -224,7 +224,7 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
         hook(objective, u, pos, g, S, kp, c)
     best = pos.clone()
     eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
-    if autotune or (autotune is None and objective.density.num_nodes >= 100000):
+    if autotune or (autotune is None and phase == 0.0 and objective.density.num_nodes >= 100000):
         eng.autotune(3)
     ctl_log = None
     if controller is not None:

@@ -32,14 +32,37 @@ def net_csr(pin_cell, pin_off, pin_net, num_nets, num_mov, ignore_net_degree):
             "num_nets": n_opt}
 
 
+def net_csr_torch(pin_cell, pin_off, pin_net, num_nets, num_mov, ignore_net_degree):
+    """net_csr with torch ops on the tensors' device (the coarse netlist, built inside the GP clock): the same integer
+    steps (degree count, renumbering, a stable sort by cell), so the same arrays bit for bit. -> dict of tensors."""
+    pin_cell, pin_net = pin_cell.long(), pin_net.long()
+    dev = pin_cell.device
+    degree = torch.bincount(pin_net, minlength=num_nets)
+    use = (degree >= 2) & (degree <= ignore_net_degree)
+    new_id = torch.cumsum(use.long(), 0) - 1
+    sel = use[pin_net]
+    cell, net = pin_cell[sel], new_id[pin_net[sel]]
+    if len(net) > 1 and bool((net[1:] < net[:-1]).any()):
+        raise ValueError("the pins must be grouped by net")
+    n_opt = int(use.sum())
+    order = torch.argsort(cell, stable=True)
+    cell_start = torch.searchsorted(cell[order], torch.arange(num_mov + 1, device=dev))
+    return {"cell": cell.int(), "off": pin_off.float()[sel].contiguous(),
+            "net_start": torch.searchsorted(net, torch.arange(n_opt + 1, device=dev)).int(),
+            "cell_start": cell_start.int(), "cell_pin": order[:int(cell_start[-1])].int(), "num_nets": n_opt}
+
+
 class Wirelength(object):
     def __init__(self, pin_cell, pin_off, pin_net, num_nets, num_mov, ignore_net_degree, device):
         """pin_cell / pin_net int [P] grouped by net (num_mov = fixed pin, its absolute position in pin_off),
-        pin_off [P, 2]: host arrays. Two pin sets: `all` (every net: exact HPWL, clustering) and `opt` (the optimized
-        nets) as the kernels' CSR view."""
+        pin_off [P, 2]: host arrays, or tensors on the device (then the CSR is built there). Two pin sets: `all`
+        (every net: exact HPWL, clustering) and `opt` (the optimized nets) as the kernels' CSR view."""
         self.num_mov, self.device = num_mov, torch.device(device)
         if self.device.type != "cuda":
             raise RuntimeError("the wirelength kernels run on a HIP device (got %s)" % self.device)
+        if torch.is_tensor(pin_cell):
+            self._from_tensors(pin_cell, pin_off, pin_net, num_nets, num_mov, ignore_net_degree)
+            return
         t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.device)   # noqa: E731
         csr = net_csr(pin_cell, pin_off, pin_net, num_nets, num_mov, ignore_net_degree)
         self.num_opt_nets = csr["num_nets"]
@@ -51,6 +74,19 @@ class Wirelength(object):
                     "net": t(pin_net, torch.int32), "num_nets": num_nets,
                     "net_start": t(np.searchsorted(pin_net, np.arange(num_nets + 1)), torch.int32)}
         self._gamma = torch.empty(1, device=self.device)
+
+    def _from_tensors(self, pin_cell, pin_off, pin_net, num_nets, num_mov, ignore_net_degree):
+        dev = self.device
+        pin_cell, pin_off, pin_net = pin_cell.to(dev), pin_off.to(dev), pin_net.to(dev)
+        csr = net_csr_torch(pin_cell, pin_off, pin_net, num_nets, num_mov, ignore_net_degree)
+        self.num_opt_nets = csr["num_nets"]
+        self.opt = {"cell32": csr["cell"], "off": csr["off"], "net_start": csr["net_start"],
+                    "cell_start": csr["cell_start"], "cell_pin": csr["cell_pin"], "num_nets": csr["num_nets"]}
+        net = pin_net.long()
+        self.all = {"cell": pin_cell.int().contiguous(), "off": pin_off.float().contiguous(), "net": net.int(),
+                    "num_nets": num_nets,
+                    "net_start": torch.searchsorted(net, torch.arange(num_nets + 1, device=dev)).int()}
+        self._gamma = torch.empty(1, device=dev)
 
     def pins(self):
         """Every pin of every net: dict(cell, off, net, num_nets), long indices."""

@@ -181,12 +181,12 @@ AP_API int ap_level_objective(ap_level* level, const float* pos, float* grad, do
  * ap_level_objective): the top-left quadrant of the reference's density_map result. */
 AP_API int ap_level_density(ap_level* level, float* rho, void* stream);
 
-/* Autotuner of the density tile layout: every admissible side (8 .. 64 bins) with every admissible group (1, 2, 4, 8
- * tiles per workgroup of the tile kernel, within 64 KB of LDS) of the force grid, then of the overflow grid, timed on
- * the front of a pass (WA, binning, tiles, Poisson solve, as the passes overlap them) at the caller's current point
- * (reps runs each); the fastest is kept. layout i32[4] (optional):
- * force side, force group, overflow side, overflow group (0, 0: no overflow grid). The density map depends on
- * neither, so the results of later passes are the same whatever the choice. Drops the level's captured graphs. */
+/* Autotuner of the density tile layout: the admissible sides from half to twice the set-up side (8 .. 64 bins) with
+ * 1 or 2 tiles per workgroup of the tile kernel, for the force grid, then the overflow grid, then the force grid
+ * again, timed on the front of a pass (WA, binning, tiles, Poisson solve, as the passes overlap them) at the caller's
+ * current point (reps runs each); the fastest is kept. layout i32[4] (optional): force side, force group, overflow
+ * side, overflow group (0, 0: no overflow grid). The density map depends on neither, so the results of later passes
+ * are the same whatever the choice. Drops the level's captured graphs. */
 AP_API int ap_level_autotune(ap_level* level, int reps, int* layout);
 /* Set the density tile layout directly (an admissible side and group of each grid, group 0: the default for the
  * side; the overflow pair is ignored without an overflow grid): the autotuner's choice, replayed. Drops the level's

@@ -45,3 +45,22 @@ def test_renumbered_netlist_has_the_same_nets():
     assert sorted(cp.tolist()) == sorted(np.nonzero(new["cell32"].numpy() < n)[0].tolist())
     for c in range(0, n, 37):
         assert np.all(new["cell32"].numpy()[cp[cs[c]:cs[c + 1]]] == c)
+
+
+def test_device_net_csr_equals_the_host_one():
+    """The torch form of the optimized-net CSR (built on the device for the coarse netlist) gives the host form's
+    arrays bit for bit, here on CPU tensors; nets of degree 1 and above the limit are dropped by both."""
+    from autoplace_amd import synth
+    from autoplace_amd.database import PlaceData
+    from autoplace_amd.wirelength import net_csr, net_csr_torch
+    lef, d = synth.generate(3000, seed=7, io_pins=6)
+    data = PlaceData(lef, d)
+    n = data.num_mov
+    for limit in (100, 6):
+        ref = net_csr(data.pin_cell, data.pin_off, data.pin_net, data.num_nets, n, limit)
+        got = net_csr_torch(torch.as_tensor(np.asarray(data.pin_cell)), torch.as_tensor(np.asarray(data.pin_off)),
+                            torch.as_tensor(np.asarray(data.pin_net)), data.num_nets, n, limit)
+        assert got["num_nets"] == ref["num_nets"]
+        for k in ("cell", "off", "net_start", "cell_start", "cell_pin"):
+            a, b = np.asarray(ref[k]), got[k].numpy()
+            assert a.dtype == b.dtype and np.array_equal(a, b), k
