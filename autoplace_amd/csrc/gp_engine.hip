@@ -686,16 +686,31 @@ __global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) { reduce_b
 // Second stage and the control. With a learned controller (ctl.params) the 64 lanes evaluate the MLP on the pass's
 // features first; its clipped outputs (plus the exploration noise, when asked) scale the rule's density-weight
 // growth and smoothing, until the overflow stalls for ctl.window passes (then the rule alone, for the level).
-__device__ __forceinline__ void control_body(const double* red, double over_scale, double* S, float* kp,
-                                             const double* c, int* flags, double* stats, const Ctl& ctl) {
+__device__ __forceinline__ void control_body(const double* red, double over_scale, double* S_g, float* kp_g,
+                                             const double* c_g, int* flags, double* stats, const Ctl& ctl) {
     __shared__ double r[RED_VALS];
+    __shared__ double rb[RED_BLOCKS * RED_VALS];
     __shared__ float x[MLP_IN];
     __shared__ float h[MLP_H];
     __shared__ int learn;
+    // the first-stage sums, and the control state, into LDS by all lanes at once (one memory latency, not a chain of
+    // them in lane 0); the sums are added by one lane in block order (a fixed order), the state written back after
+    __shared__ double S_l[NUM_S], c_l[NUM_C];
+    __shared__ float kp_l[5];
+    for (int i = threadIdx.x; i < RED_BLOCKS * RED_VALS; i += blockDim.x) rb[i] = red[i];
+    if (S_g) {
+        for (int i = threadIdx.x; i < NUM_S; i += blockDim.x) S_l[i] = S_g[i];
+        for (int i = threadIdx.x; i < NUM_C; i += blockDim.x) c_l[i] = c_g[i];
+        if (threadIdx.x < 5) kp_l[threadIdx.x] = kp_g[threadIdx.x];
+    }
+    double* S = S_g ? S_l : nullptr;
+    float* kp = kp_l;
+    const double* c = c_l;
+    __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 0; k < RED_VALS; k++) r[k] = 0;
         for (int b = 0; b < RED_BLOCKS; b++)
-            for (int k = 0; k < RED_VALS; k++) r[k] += red[b * RED_VALS + k];
+            for (int k = 0; k < RED_VALS; k++) r[k] += rb[b * RED_VALS + k];
         learn = 0;
         if (S && ctl.params && S[STOP] == 0) {
             const double moved = sqrt(r[0]), turned = sqrt(r[1]), wa = r[2], hpwl = r[3], ovf = r[4] * over_scale;
@@ -760,6 +775,11 @@ __device__ __forceinline__ void control_body(const double* red, double over_scal
             if (S[ACCEPT] != 0) flags[0] = 1 - flags[0];
             flags[1] = S[IS_BEST] != 0 ? 1 : 0;
         }
+    }
+    if (S_g) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < NUM_S; i += blockDim.x) S_g[i] = S_l[i];
+        if (threadIdx.x < 5) kp_g[threadIdx.x] = kp_l[threadIdx.x];
     }
 }
 __global__ __launch_bounds__(64) void eng_control(const double* red, double over_scale, double* S, float* kp,

@@ -22,7 +22,7 @@
 
 namespace ap {
 
-constexpr int FB = 256;   // lanes per FFT workgroup
+constexpr int FB = 512;   // lanes per FFT workgroup
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
