@@ -5,14 +5,15 @@
 //   * u / v / g are double-buffered; the control kernel flips a device flag instead of select_kernel's 3 copies,
 //     and the best placement is copied by the next pass's Nesterov kernel (which reads v anyway);
 //   * the density map is built per tile of bins: a counting sort puts every node into the buckets of the tiles its
-//     bins fall in, and one workgroup per tile sums its bucket with LDS integer atomics and writes the tile's density
-//     and overflow once. No global atomics, no mirrored copies, no clearing pass (the reference adds every
+//     bins fall in, and the tile kernel sums each tile's buckets with LDS integer atomics and writes the tile's
+//     density and overflow once. No global atomics, no mirrored copies, no clearing pass (the reference adds every
 //     contribution to global memory with an atomic and writes the map 4x for the FFT), and the cost does not depend
 //     on the node order;
 //   * |v' - v| and |g' - g| are accumulated in FP64 by the kernels that write v' and g', the WA / HPWL totals by the
 //     WA kernel and the overflow by the tile kernel, as per-workgroup partials the control kernel sums in a fixed
 //     order (deterministic);
-//   * the per-cell WA gradient sum is fused into the gather.
+//   * the pass runs on two streams: the overflow grid, WA and the cells' WA sums beside the force grid's binning,
+//     tiles and Poisson solve (they read the same positions and write buffers of their own); the gather joins them.
 // Every kernel reads its parameters (gamma, weight, step) from device memory, so a run of passes has no host sync
 // and replays from a HIP graph.
 #include <stdio.h>
@@ -959,7 +960,7 @@ struct Level {
     int set_lds() const;
     int bin_grid(int which, const Pick& pos, hipStream_t s);
     int tiles_grid(int which, const Pick& pos, hipStream_t s);
-    int time_density(int reps, hipStream_t s, float& ms);
+    int time_front(int reps, hipStream_t s, float& ms);
     int autotune(int reps, hipStream_t s, int* sides);
     int create(const ap_level_desc& desc);
     void destroy();
@@ -1094,7 +1095,7 @@ int Level::tiles_grid(int which, const Pick& pos, hipStream_t s) {
 // (between runs the current point is always set 0), `reps` times: ms per run. It changes nothing a pass reads before
 // writing it again (bucket counts end at zero; pin and cell gradients, partials, rho and the field are rewritten by
 // every pass).
-int Level::time_density(int reps, hipStream_t s, float& ms) {
+int Level::time_front(int reps, hipStream_t s, float& ms) {
     const Pick pos{(const float2*)d.v, nullptr, nullptr};
     int st;
     if ((st = set_lds())) return st;
@@ -1120,7 +1121,7 @@ int Level::time_density(int reps, hipStream_t s, float& ms) {
 }
 
 // Autotuner of the tile layout: for each grid (force, then overflow), the admissible sides from half to twice the
-// set-up side (8 .. 64) with 1 or 2 tiles per workgroup are timed on the front of a pass (time_density) at the
+// set-up side (8 .. 64) with 1 or 2 tiles per workgroup are timed on the front of a pass (time_front) at the
 // current positions and the fastest kept. The map does not depend on either (integer sums), so the choice changes the time only, never
 // the results.
 int Level::autotune(int reps, hipStream_t s, int* sides) {
@@ -1140,11 +1141,11 @@ int Level::autotune(int reps, hipStream_t s, int* sides) {
                 set_side(*J, T, G);
                 update_lds();
                 float warm = 0, ms = 0, again = 0;
-                if ((st = time_density(1, s, warm))) return st;   // warm-up at this layout
+                if ((st = time_front(1, s, warm))) return st;   // warm-up at this layout
                 if (warm > 1.5f * best_ms) continue;              // clearly slower: not timed further
-                if ((st = time_density(reps, s, ms))) return st;
+                if ((st = time_front(reps, s, ms))) return st;
                 if (ms < 1.1f * best_ms) {                        // a contender: the better of two timings
-                    if ((st = time_density(reps, s, again))) return st;
+                    if ((st = time_front(reps, s, again))) return st;
                     ms = std::min(ms, again);
                 }
                 if (ms < best_ms) {
