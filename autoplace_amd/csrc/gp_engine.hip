@@ -500,6 +500,7 @@ struct GatherArgs {
     const float* ratio;
     const float* fx;
     const float* fy;
+    const float2* f2;         // non-null: the field as (x, y) pairs (fx / fy unused)
     int ld;
     const float2* pin_grad;
     const float2* cell_wl;    // the cells' WA gradients summed by eng_cell_wl (null: the gather sums the pins)
@@ -572,8 +573,10 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
                     for (int j = 0; j < 3; j++) {
                         const bool in = sx.b0 + i <= sx.b1 && sy.b0 + j <= sy.b1;
                         const long long o = (long long)(sx.b0 + i) * a.ld + sy.b0 + j;
-                        ex[i][j] = in ? a.fx[o] : 0.f;
-                        ey[i][j] = in ? a.fy[o] : 0.f;
+                        const float2 f = !in ? make_float2(0.f, 0.f)
+                                         : a.f2 ? a.f2[o] : make_float2(a.fx[o], a.fy[o]);
+                        ex[i][j] = f.x;
+                        ey[i][j] = f.y;
                     }
 #pragma unroll
                 for (int i = 0; i < 3; i++)
@@ -586,12 +589,12 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
             } else {
                 for (int bx = sx.b0; bx <= sx.b1; bx++) {
                     const float wx = r * span_overlap(sx, bx);
-                    const float* fxr = a.fx + (long long)bx * a.ld;
-                    const float* fyr = a.fy + (long long)bx * a.ld;
+                    const long long row = (long long)bx * a.ld;
                     for (int by = sy.b0; by <= sy.b1; by++) {
                         const float w = wx * span_overlap(sy, by);
-                        gx -= w * fxr[by];
-                        gy -= w * fyr[by];
+                        const float2 f = a.f2 ? a.f2[row + by] : make_float2(a.fx[row + by], a.fy[row + by]);
+                        gx -= w * f.x;
+                        gy -= w * f.y;
                     }
                 }
             }
@@ -881,6 +884,7 @@ struct Level {
     size_t hist_lds = 0, acc_lds = 0;                       // the larger of the two (the batch's grouped launches)
     std::vector<void*> bin_mem;
     float *rho = nullptr, *fx = nullptr, *fy = nullptr;
+    float2* f2 = nullptr;      // the field as (x, y) pairs (FFT method; fx / fy otherwise)
     double* part = nullptr;
     double* stats = nullptr;
     double* red = nullptr;     // first-stage sums, RED_BLOCKS x RED_VALS
@@ -958,6 +962,7 @@ struct Level {
     void set_side(BinJob& J, int T, int G = 0);
     void update_lds();
     int set_lds() const;
+    int solve_field(hipStream_t s);
     int bin_grid(int which, const Pick& pos, hipStream_t s);
     int tiles_grid(int which, const Pick& pos, hipStream_t s);
     int time_front(int reps, hipStream_t s, float& ms);
@@ -1064,6 +1069,11 @@ int Level::set_lds() const {
     return AP_OK;
 }
 
+// the Poisson solve of the pass: the field as (x, y) pairs with the FFT method, as two planes with the GEMM one
+int Level::solve_field(hipStream_t s) {
+    return f2 ? poisson.solve_pairs(rho, d.nby, f2, d.nby, s) : poisson.solve(rho, d.nby, fx, fy, d.nby, s);
+}
+
 // count (with the scan in its last workgroup) and scatter of one grid (0: force, 1: overflow)
 int Level::bin_grid(int which, const Pick& pos, hipStream_t s) {
     if (!which) {
@@ -1109,7 +1119,7 @@ int Level::time_front(int reps, hipStream_t s, float& ms) {
         AP_LAUNCHED();
         if (jj.count && ((st = bin_grid(1, pos, side)) || (st = tiles_grid(1, pos, side)))) return st;
         if ((st = bin_grid(0, pos, s)) || (st = tiles_grid(0, pos, s))) return st;
-        if ((st = poisson.solve(rho, d.nby, fx, fy, d.nby, s))) return st;
+        if ((st = solve_field(s))) return st;
         AP_HIP(hipEventRecord(ev_join, side));
         AP_HIP(hipStreamWaitEvent(s, ev_join, 0));
     }
@@ -1122,8 +1132,8 @@ int Level::time_front(int reps, hipStream_t s, float& ms) {
 
 // Autotuner of the tile layout: for each grid (force, then overflow), the admissible sides from half to twice the
 // set-up side (8 .. 64) with 1 or 2 tiles per workgroup are timed on the front of a pass (time_front) at the
-// current positions and the fastest kept. The map does not depend on either (integer sums), so the choice changes the time only, never
-// the results.
+// current positions and the fastest kept. The map does not depend on either (integer sums), so the choice changes
+// the time only, never the results.
 int Level::autotune(int reps, hipStream_t s, int* sides) {
     AP_HIP(hipDeviceSynchronize());
     int st;
@@ -1202,8 +1212,12 @@ int Level::create(const ap_level_desc& desc) {
         AP_LAUNCHED();
     }
     AP_HIP(hipMalloc(&rho, nb * sizeof(float)));
-    AP_HIP(hipMalloc(&fx, nb * sizeof(float)));
-    AP_HIP(hipMalloc(&fy, nb * sizeof(float)));
+    if (poisson.method == AP_POISSON_FFT) {        // the field as (x, y) pairs: one load per bin in the gather
+        AP_HIP(hipMalloc(&f2, nb * sizeof(float2)));
+    } else {
+        AP_HIP(hipMalloc(&fx, nb * sizeof(float)));
+        AP_HIP(hipMalloc(&fy, nb * sizeof(float)));
+    }
     nblk_n = grid_of(N, EB);
     if ((st = wa_plan.create(d.net_start, d.num_nets, d.pin_cell, d.pin_off, pin_slot, d.num_pins))) return st;
     nblk_wa = wa_plan.partial_count();
@@ -1253,7 +1267,8 @@ void Level::destroy() {
     last_ev = nullptr;
     for (hipEvent_t& ev : pev)
         if (ev) (void)hipEventDestroy(ev);
-    void* ptrs[] = {u1, v1, g1, pin_grad, cell_wl, rho, fx, fy, part, stats, flags, red, ctl_x, block_net, pin_slot};
+    void* ptrs[] = {u1, v1, g1, pin_grad, cell_wl, rho, fx, fy, f2, part, stats, flags, red, ctl_x, block_net,
+                    pin_slot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : bin_mem) (void)hipFree(p);
@@ -1286,6 +1301,7 @@ GatherArgs Level::gather_args(const Pick& pos, const float* kp_ptr, const float2
     g.ratio = d.ratio;
     g.fx = fx;
     g.fy = fy;
+    g.f2 = f2;
     g.ld = d.nby;
     g.pin_grad = pin_grad;
     g.cell_start = d.cell_start;
@@ -1364,7 +1380,7 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
         tend(AP_PROF_OVER_TILES, w);
         mark(AP_PROF_POISSON, s);
         tbegin(AP_PROF_POISSON, s);
-        if ((st = poisson.solve(rho, d.nby, fx, fy, d.nby, s))) return st;
+        if ((st = solve_field(s))) return st;
         tend(AP_PROF_POISSON, s);
     }
     if (two) {
@@ -1634,7 +1650,7 @@ int Batch::build() {
         T.bufs = L.bufs();
         T.num_nodes = L.d.num_nodes;
         wave[b] = L.wa_plan.wave_args(L.wa_args(pos, L.d.kp), L.p_wa);
-        ffts[b] = L.poisson.fft_tab(L.rho, L.d.nby, L.fx, L.fy, L.d.nby, L.d.S);
+        ffts[b] = L.poisson.fft_tab(L.rho, L.d.nby, L.fx, L.fy, L.f2, L.d.nby, L.d.S);
         S[b] = L.d.S;
         hist_lds = std::max(hist_lds, L.hist_lds);
         acc_lds = std::max(acc_lds, L.acc_lds);

@@ -147,12 +147,18 @@ __device__ __forceinline__ void cs_put_unit(float2* s, const float2* ph, int L, 
         cs_put(s, ph, lg, L - k, cr, c, sv, sr);
     }
 }
-// after the inverse FFT of h: y_cos[j], y_sin[j] (Makhoul's output order)
-__device__ __forceinline__ void cs_store(const float2* s, int L, float* y_cos, float* y_sin) {
+// after the inverse FFT of h: y_cos[j], y_sin[j] (Makhoul's output order), or the pairs (y_cos[j], y_sin[j]) into
+// y_pair when it is given
+__device__ __forceinline__ void cs_store(const float2* s, int L, float* y_cos, float* y_sin, float2* y_pair = nullptr) {
     for (int j = threadIdx.x; j < L; j += blockDim.x) {
         const float2 y = s[lpad(makhoul(j, L))];
-        y_cos[j] = 0.5f * y.x;
-        y_sin[j] = (j & 1) ? -0.5f * y.y : 0.5f * y.y;
+        const float c = 0.5f * y.x, sn = (j & 1) ? -0.5f * y.y : 0.5f * y.y;
+        if (y_pair) {
+            y_pair[j] = make_float2(c, sn);
+        } else {
+            y_cos[j] = c;
+            y_sin[j] = sn;
+        }
     }
 }
 
@@ -224,8 +230,8 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
 
 __global__ __launch_bounds__(FB) void fft_pass_b(PassB a) { pass_b_body(a, blockIdx.x); }
 
-__device__ __forceinline__ void pass_c_body(const float* U, const float* V, float* fx, float* fy, int ld_out,
-                                            const float2* tw, const float2* ph, int L, int lg, int n) {
+__device__ __forceinline__ void pass_c_body(const float* U, const float* V, float* fx, float* fy, float2* f2,
+                                            int ld_out, const float2* tw, const float2* ph, int L, int lg, int n) {
     extern __shared__ float2 lds[];
     const int half = L >> 1;
     float2* s = lds;
@@ -242,12 +248,16 @@ __device__ __forceinline__ void pass_c_body(const float* U, const float* V, floa
     }
     __syncthreads();
     fft_lds(s, tw_s, L, lg, true);
-    cs_store(s, L, fx + (long long)n * ld_out, fy + (long long)n * ld_out);
+    if (f2)
+        cs_store(s, L, nullptr, nullptr, f2 + (long long)n * ld_out);
+    else
+        cs_store(s, L, fx + (long long)n * ld_out, fy + (long long)n * ld_out);
 }
 
-__global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V, float* fx, float* fy, int ld_out,
+__global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V, float* fx, float* fy, float2* f2,
+                                                 int ld_out,
                                                  const float2* tw, const float2* ph, int L, int lg) {
-    pass_c_body(U, V, fx, fy, ld_out, tw, ph, L, lg, blockIdx.x);
+    pass_c_body(U, V, fx, fy, f2, ld_out, tw, ph, L, lg, blockIdx.x);
 }
 
 // out[c][r] = in[r][c] for 2 planes (blockIdx.z), 32 x 32 tiles through LDS
@@ -308,7 +318,7 @@ __global__ __launch_bounds__(FB) void fft_c_grouped(const FftTab* tab, const int
     const int4 m = map[blockIdx.x];
     const FftTab t = tab[m.x];
     if (fft_skip(t)) return;
-    pass_c_body(t.w0, t.w1, t.fx, t.fy, t.fld, t.twM, t.phM, t.M, t.lgM, m.y);
+    pass_c_body(t.w0, t.w1, t.fx, t.fy, t.f2, t.fld, t.twM, t.phM, t.M, t.lgM, m.y);
 }
 
 int FftGroup::create(const std::vector<FftTab>& tabs) {
@@ -360,7 +370,8 @@ int FftGroup::launch(const FftTab* d_tab, hipStream_t s) const {
     return AP_OK;
 }
 
-FftTab Poisson::fft_tab(const float* rho, int rho_ld, float* fx, float* fy, int fld, const double* S) const {
+FftTab Poisson::fft_tab(const float* rho, int rho_ld, float* fx, float* fy, float2* f2, int fld,
+                        const double* S) const {
     const size_t nm = (size_t)N * M;
     FftTab t{};
     t.S = S;
@@ -368,6 +379,7 @@ FftTab Poisson::fft_tab(const float* rho, int rho_ld, float* fx, float* fy, int 
     t.rho_ld = rho_ld;
     t.fx = fx;
     t.fy = fy;
+    t.f2 = f2;
     t.fld = fld;
     t.w0 = d_work;
     t.w1 = d_work + nm;
@@ -432,7 +444,7 @@ void Poisson::fft_destroy() {
     d_omega = nullptr;
 }
 
-int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s) {
+int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, float2* f2, int fld, hipStream_t s) {
     const size_t nm = (size_t)N * M;
     const int lgN = ilog2(N), lgM = ilog2(M);
     const float2* twN = d_tw;
@@ -456,8 +468,7 @@ int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, int f
                        N);
     AP_LAUNCHED();
     // pass C: rows of U (w0), V (w1) -> field
-    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), lds_m, s, w0, w1, fx, fy, fld,
-                       twM, phM, M, lgM);
+    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), lds_m, s, w0, w1, fx, fy, f2, fld, twM, phM, M, lgM);
     AP_LAUNCHED();
     return AP_OK;
 }

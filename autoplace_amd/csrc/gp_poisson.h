@@ -16,6 +16,7 @@ struct FftTab {
     int rho_ld;
     float *fx, *fy;
     int fld;
+    float2* f2;                  // non-null: the field interleaved, (x, y) pairs with row stride fld (fx, fy unused)
     float *w0, *w1, *w2, *w3;
     const float2 *twN, *phN, *twM, *phM;
     const float *omega, *nu;
@@ -51,12 +52,15 @@ struct Poisson {
     int create(int nbx, int nby, double bx, double by, int method);
     void destroy();
     int solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s);
+    // FFT method only: the field as (x, y) pairs, f[n * fld + j] (one 8-byte store per bin, one load per bin for a
+    // reader that needs both)
+    int solve_pairs(const float* rho, int rho_ld, float2* f, int fld, hipStream_t s);
     int energy(const float* rho, int rho_ld, double* out, hipStream_t s);
 
     int fft_create();
     void fft_destroy();
-    int fft_solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s);
-    FftTab fft_tab(const float* rho, int rho_ld, float* fx, float* fy, int fld, const double* S) const;
+    int fft_solve(const float* rho, int rho_ld, float* fx, float* fy, float2* f2, int fld, hipStream_t s);
+    FftTab fft_tab(const float* rho, int rho_ld, float* fx, float* fy, float2* f2, int fld, const double* S) const;
 };
 
 }  // namespace ap

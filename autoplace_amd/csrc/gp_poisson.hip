@@ -277,8 +277,13 @@ int Poisson::energy(const float* rho, int rho_ld, double* out, hipStream_t s) {
     return AP_OK;
 }
 
+int Poisson::solve_pairs(const float* rho, int rho_ld, float2* f, int fld, hipStream_t s) {
+    AP_REQUIRE(method == AP_POISSON_FFT, "poisson: the paired field needs the FFT method");
+    return fft_solve(rho, rho_ld, nullptr, nullptr, f, fld, s);
+}
+
 int Poisson::solve(const float* rho, int rho_ld, float* fx, float* fy, int fld, hipStream_t s) {
-    if (method == AP_POISSON_FFT) return fft_solve(rho, rho_ld, fx, fy, fld, s);
+    if (method == AP_POISSON_FFT) return fft_solve(rho, rho_ld, fx, fy, nullptr, fld, s);
     const size_t nm = (size_t)N * M;
     float *T1 = d_work, *Ax = d_work + nm, *Ay = d_work + 2 * nm, *U = d_work + 3 * nm;
     float* V = T1;   // T1 is dead once Ax / Ay exist
