@@ -2,12 +2,14 @@
 //
 // The cosine / sine series of gp_poisson.hip evaluated with LDS-resident complex FFTs, one workgroup per row:
 //   pass A  rows of rho (length M):   T = rho Cy^T                 DCT-II of two real rows in one complex FFT
-//   transpose T -> Tt [M][N]
-//   pass B  rows of Tt (length N):   C = DCT-II along x; Ax = wx C, Ay = wy C (weights computed in place);
-//                                     U^T = Sx^T Ax, V^T = Cx^T Ay  (sine and cosine series of one row from one
-//                                     complex inverse FFT: both outputs are real, so they share it)
-//   transpose U^T, V^T -> U, V [N][M]
-//   pass C  rows of U, V (length M):  field_x = U Cy, field_y = V Sy  (one inverse FFT per row)
+//   pass B  columns of T (length N), read in place by pairs:
+//                                     C = DCT-II along x; Ax = wx C, Ay = wy C (weights computed in place);
+//                                     U^T = Sx^T Ax, V^T = Cx^T Ay  (sine and cosine series of one column from
+//                                     one complex inverse FFT: both outputs are real, so they share it)
+//   pass C  columns of U^T, V^T (rows of U, V, length M):
+//                                     field_x = U Cy, field_y = V Sy  (one inverse FFT per row)
+// Passes B and C run consecutive workgroups on one XCD, so the lines the strided column reads share meet in one L2
+// (this replaced two transposes through LDS tiles). The batch's grouped launches keep the transposes.
 // DCT-II via Makhoul's reordering (v[n] = x[2n], v[L-1-n] = x[2n+1]; X[k] = Re(e^{-i pi k/2L} DFT(v)[k])); the
 // cosine series y[n] = sum_k a[k] cos(pi k (2n+1)/2L) via its inverse (V[k] = e^{i pi k/2L}(X'[k] - i X'[L-k]),
 // X'[0] = 2a[0]; y[2n] = v[n]/2, y[2n+1] = v[L-1-n]/2); the sine series as (-1)^n times the cosine series of the
@@ -19,6 +21,7 @@
 
 #include "gp_common.h"
 #include "gp_poisson.h"
+#include "gp_wa.h"   // xcd_remap
 
 namespace ap {
 
@@ -173,6 +176,7 @@ struct PassB {
     int N, lgN;
     const float2* tw;
     const float2* ph;
+    int col;               // > 0: Tt is T itself, [N][col] (pass A's rows), read by column pairs (no transpose)
 };
 
 // the spectral weights of row l at x-frequency k: Ay = nu / den C, Ax = omega / den C
@@ -190,10 +194,16 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
     float2* tw_s = lds + lds_len(L);
     load_twiddles(tw_s, a.tw, L);
     const int l0 = 2 * wg;
-    const float* in0 = a.Tt + (long long)l0 * L;
-    const float* in1 = in0 + L;
-    for (int m = threadIdx.x; m < L; m += blockDim.x)
-        s[lpad(bitrev(makhoul(m, L), lg))] = make_float2(in0[m], in1[m]);
+    if (a.col) {
+        // columns l0, l0 + 1 of T: one 8-byte load per row
+        for (int m = threadIdx.x; m < L; m += blockDim.x)
+            s[lpad(bitrev(makhoul(m, L), lg))] = *(const float2*)(a.Tt + (long long)m * a.col + l0);
+    } else {
+        const float* in0 = a.Tt + (long long)l0 * L;
+        const float* in1 = in0 + L;
+        for (int m = threadIdx.x; m < L; m += blockDim.x)
+            s[lpad(bitrev(makhoul(m, L), lg))] = make_float2(in0[m], in1[m]);
+    }
     __syncthreads();
     fft_lds(s, tw_s, L, lg, false);
     // DCT-II of both rows at this lane's units: c[u][row][0 at k, 1 at L - k]
@@ -228,10 +238,15 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
     }
 }
 
-__global__ __launch_bounds__(FB) void fft_pass_b(PassB a) { pass_b_body(a, blockIdx.x); }
+// the column-reading passes run consecutive workgroups on the same XCD, so the rows' lines they share meet in one L2
+__global__ __launch_bounds__(FB) void fft_pass_b(PassB a) {
+    pass_b_body(a, a.col ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x);
+}
 
+// col > 0: U and V are pass B's outputs Ut, Vt [L][col] and row n is their column n (no transpose)
 __device__ __forceinline__ void pass_c_body(const float* U, const float* V, float* fx, float* fy, float2* f2,
-                                            int ld_out, const float2* tw, const float2* ph, int L, int lg, int n) {
+                                            int ld_out, const float2* tw, const float2* ph, int L, int lg, int n,
+                                            int col) {
     extern __shared__ float2 lds[];
     const int half = L >> 1;
     float2* s = lds;
@@ -244,7 +259,12 @@ __device__ __forceinline__ void pass_c_body(const float* U, const float* V, floa
         const int k = threadIdx.x + u * FB;
         if (k > half) continue;
         const bool two = k && 2 * k != L;
-        cs_put_unit(s, ph, L, lg, k, u_row[k], two ? u_row[L - k] : 0.f, v_row[k], two ? v_row[L - k] : 0.f);
+        if (col) {
+            const long long a0 = (long long)k * col + n, a1 = (long long)(L - k) * col + n;
+            cs_put_unit(s, ph, L, lg, k, U[a0], two ? U[a1] : 0.f, V[a0], two ? V[a1] : 0.f);
+        } else {
+            cs_put_unit(s, ph, L, lg, k, u_row[k], two ? u_row[L - k] : 0.f, v_row[k], two ? v_row[L - k] : 0.f);
+        }
     }
     __syncthreads();
     fft_lds(s, tw_s, L, lg, true);
@@ -254,10 +274,11 @@ __device__ __forceinline__ void pass_c_body(const float* U, const float* V, floa
         cs_store(s, L, fx + (long long)n * ld_out, fy + (long long)n * ld_out);
 }
 
+// U, V: pass B's outputs Ut, Vt [L][col], each workgroup one column
 __global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V, float* fx, float* fy, float2* f2,
-                                                 int ld_out,
-                                                 const float2* tw, const float2* ph, int L, int lg) {
-    pass_c_body(U, V, fx, fy, f2, ld_out, tw, ph, L, lg, blockIdx.x);
+                                                 int ld_out, const float2* tw, const float2* ph, int L, int lg,
+                                                 int col) {
+    pass_c_body(U, V, fx, fy, f2, ld_out, tw, ph, L, lg, xcd_remap(blockIdx.x, gridDim.x), col);
 }
 
 // out[c][r] = in[r][c] for 2 planes (blockIdx.z), 32 x 32 tiles through LDS
@@ -305,7 +326,7 @@ __global__ __launch_bounds__(FB) void fft_b_grouped(const FftTab* tab, const int
     const int4 m = map[blockIdx.x];
     const FftTab t = tab[m.x];
     if (fft_skip(t)) return;
-    const PassB b{t.w1, t.w2, t.w3, t.omega, t.nu, t.scale, t.bx, t.N, t.lgN, t.twN, t.phN};
+    const PassB b{t.w1, t.w2, t.w3, t.omega, t.nu, t.scale, t.bx, t.N, t.lgN, t.twN, t.phN, 0};
     pass_b_body(b, m.y);
 }
 __global__ __launch_bounds__(256) void fft_t2_grouped(const FftTab* tab, const int4* map) {
@@ -318,7 +339,7 @@ __global__ __launch_bounds__(FB) void fft_c_grouped(const FftTab* tab, const int
     const int4 m = map[blockIdx.x];
     const FftTab t = tab[m.x];
     if (fft_skip(t)) return;
-    pass_c_body(t.w0, t.w1, t.fx, t.fy, t.f2, t.fld, t.twM, t.phM, t.M, t.lgM, m.y);
+    pass_c_body(t.w0, t.w1, t.fx, t.fy, t.f2, t.fld, t.twM, t.phM, t.M, t.lgM, m.y, 0);
 }
 
 int FftGroup::create(const std::vector<FftTab>& tabs) {
@@ -457,18 +478,12 @@ int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, float
     const size_t lds_n = (size_t)(lds_len(N) + N / 2) * sizeof(float2);
     hipLaunchKernelGGL(fft_pass_a, dim3(N / 2), dim3(FB), lds_m, s, rho, rho_ld, w0, M, twM, phM, M, lgM);
     AP_LAUNCHED();
-    hipLaunchKernelGGL(transpose_kernel, dim3(grid_of(M, TT), grid_of(N, TT), 1), dim3(256), 0, s, w0, w0, w1, w1, N,
-                       M);
-    AP_LAUNCHED();
-    // pass B: rows of Tt (length N) -> Ut (w2), Vt (w3)
-    PassB b{w1, w2, w3, d_omega, d_omega + N, (float)(4.0 / ((double)N * M)), (float)bx, N, lgN, twN, phN};
+    // pass B: columns of T (length N), read in place by pairs -> Ut (w2), Vt (w3); no transpose
+    PassB b{w0, w2, w3, d_omega, d_omega + N, (float)(4.0 / ((double)N * M)), (float)bx, N, lgN, twN, phN, M};
     hipLaunchKernelGGL(fft_pass_b, dim3(M / 2), dim3(FB), lds_n, s, b);
     AP_LAUNCHED();
-    hipLaunchKernelGGL(transpose_kernel, dim3(grid_of(N, TT), grid_of(M, TT), 2), dim3(256), 0, s, w2, w3, w0, w1, M,
-                       N);
-    AP_LAUNCHED();
-    // pass C: rows of U (w0), V (w1) -> field
-    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), lds_m, s, w0, w1, fx, fy, f2, fld, twM, phM, M, lgM);
+    // pass C: columns of Ut, Vt (rows of U, V) -> field; no transpose
+    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), lds_m, s, w2, w3, fx, fy, f2, fld, twM, phM, M, lgM, N);
     AP_LAUNCHED();
     return AP_OK;
 }
