@@ -53,6 +53,9 @@ _SIGNATURES = {
     "ap_level_controller_state": ([P, ctypes.POINTER(D)], I),
     "ap_level_autotune": ([P, I, P], I),
     "ap_level_set_tile_layout": ([P, I, I, I, I], I),
+    "ap_level_set_bucket_room": ([P, I, I], I),
+    "ap_level_time_every": ([P, I], I),
+    "ap_level_bucket_spills": ([P, P], I),
     "ap_batch_create": ([P, I, ctypes.POINTER(P)], I),
     "ap_batch_destroy": ([P], I),
     "ap_batch_prepare": ([P, I], I),

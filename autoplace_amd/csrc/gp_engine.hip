@@ -67,23 +67,32 @@ struct NestArgs {
     double* part;
 };
 
+// one of two pointers by a device flag, selected by arithmetic: an index into the pointer pair of a struct copy
+// becomes an indexed load from a stack copy (scratch memory)
+template <typename T>
+__device__ __forceinline__ T* pick_ptr(T* a, T* b, bool second) {
+    const uintptr_t ua = (uintptr_t)a, ub = (uintptr_t)b, m = second ? ~(uintptr_t)0 : 0;
+    return (T*)(ua ^ ((ua ^ ub) & m));
+}
+
 __device__ __forceinline__ void nesterov_body(const NestArgs& a, int wg) {
     __shared__ double red[16];
     const Bufs& B = a.B;
-    const int cur = B.flags[0], nx = 1 - cur;
+    const bool cur = B.flags[0] != 0;
     const bool copy_best = B.flags[1] != 0;
     const float alpha = a.kp[3], coef = a.kp[4];
     const int i = wg * EB + threadIdx.x;
     double m2[1] = {0.0};
     if (i < a.n) {
-        const float2 u = B.u[cur][i], v = B.v[cur][i], g = B.g[cur][i], l = a.lo[i], h = a.hi[i];
+        const float2 u = pick_ptr(B.u[0], B.u[1], cur)[i], v = pick_ptr(B.v[0], B.v[1], cur)[i];
+        const float2 g = pick_ptr(B.g[0], B.g[1], cur)[i], l = a.lo[i], h = a.hi[i];
         float2 un, vn;
         un.x = v.x - alpha * g.x;
         un.y = v.y - alpha * g.y;
         vn.x = fminf(fmaxf(un.x + coef * (un.x - u.x), l.x), h.x);
         vn.y = fminf(fmaxf(un.y + coef * (un.y - u.y), l.y), h.y);
-        B.u[nx][i] = un;
-        B.v[nx][i] = vn;
+        pick_ptr(B.u[0], B.u[1], !cur)[i] = un;
+        pick_ptr(B.v[0], B.v[1], !cur)[i] = vn;
         if (copy_best) B.best[i] = v;
         const double dx = (double)vn.x - v.x, dy = (double)vn.y - v.y;
         m2[0] = dx * dx + dy * dy;
@@ -91,23 +100,25 @@ __device__ __forceinline__ void nesterov_body(const NestArgs& a, int wg) {
     block_sum<1>(m2, red);
     if (threadIdx.x == 0) a.part[wg] = m2[0];
 }
-__global__ __launch_bounds__(EB) void eng_nesterov(NestArgs a) { nesterov_body(a, blockIdx.x); }
 
 // ------------------------------------------------------------------------------ density: tile binning
-// A grid of nbx x nby bins is cut into tiles of T x T bins. Every pass:
-//   eng_bin_count    a node whose bins fall in at most 2 x 2 tiles takes a place in the bucket of each of those tiles
-//                    (LDS histogram per workgroup, then one global atomic per workgroup and bucket); any other node
-//                    goes to the grid's list of big nodes
-//                    then bucket starts (exclusive scan of the counts) by the last workgroup to finish; the counts
-//                    are cleared for the next pass (the batch's grouped launches scan in a launch of their own)
-//   eng_bin_scatter  each node's centre, size and ratio into its buckets
-//   eng_bin_tiles    per tile (G tiles per workgroup): the contributions of the bucket's nodes and of the big nodes,
-//                    clipped to the tile and summed with LDS integer atomics; then the tile's density and overflow
-//                    partial sum
-// (eng_over_* : the same for the overflow grid, in launches of their own)
-// A bucket holds the nodes of one tile and one plane (cells, fillers). The contributions are the reference's 2^32
-// fixed-point integers (contribution() below), so the sums, and the map, do not depend on the order in which the
-// buckets were filled.
+// A grid of nbx x nby bins is cut into tiles of T x T bins. A node whose bins fall in at most 2 x 2 tiles has an
+// entry (centre, size, ratio) in the bucket of each of those tiles; any other node goes to the grid's list of big
+// nodes. A bucket holds the nodes of one tile and one plane (cells, fillers). Per tile (G tiles per workgroup), the
+// tile kernel sums the contributions of its buckets' entries and of the big nodes, clipped to the tile, with LDS
+// integer atomics, then writes the tile's density and overflow partial sum. The contributions are the reference's
+// 2^32 fixed-point integers (contribution() below), so the sums, and the map, do not depend on the order of the
+// entries. The entries are placed in one of two forms:
+//   fill form (the single-level passes, one launch per pass and grid):
+//     eng_bin_fill   every node's entries straight into their buckets: an LDS histogram ranks the workgroup's
+//                    entries, one global atomic per workgroup and bucket takes its share of the bucket, and the
+//                    entries go into the bucket's room [start[b], start[b+1]). The rooms were set before the pass
+//                    from the buckets' counts in the pass before (rooms_body, run by two extra workgroups of the
+//                    Nesterov kernel). An entry past its bucket's room adds its contributions, clipped to its tile,
+//                    to the grid's spill map with global integer atomics, and the tile kernel folds them in.
+//   counting-sort form (the batch's grouped launches): count (the same histogram and atomics), an exclusive scan of
+//     the counts, then a scatter of each node's entries to start[b] + its place.
+// (eng_over_* : the overflow grid's launches, beside the force grid's)
 constexpr int BIN_PER_LANE = 8;
 constexpr int BIN_NODES = EB * BIN_PER_LANE;
 constexpr int BIN_MAX_BUCKETS = 16384;   // planes x tiles of a grid: the count kernel's LDS histogram, 64 KB
@@ -124,20 +135,27 @@ struct BinJob {
     int G, wgs;                // tiles per workgroup of the tile kernel (1, 2, 4, 8: TB / G lanes each), its
                                // workgroups
     int planes;                // 2: cells + fillers, 1: cells only
-    int blocks;                // workgroups of the count and scatter kernels for this grid
-    int* count;                // [planes tiles] bucket sizes of the pass; zero between passes
-    int* start;                // [planes tiles + 1] bucket starts
+    int blocks;                // workgroups of the count / fill and scatter kernels for this grid
+    int* count;                // [planes tiles] entries asked of each bucket in the pass. Counting-sort form: zero
+                               // between passes. Fill form: kept until the rooms of the next pass are set from them
+    int* start;                // [planes tiles + 1] bucket starts (fill form: room of bucket b = start[b + 1] -
+                               // start[b])
     int4* off;                 // [num_nodes] place of the node's entry in the bucket of tile (+0,+0) (+0,+1) (+1,+0)
-                               // (+1,+1) from its first tile; -1: no entry
-    float4* geo;               // [4 num_nodes] entries: centre x, y, size x, y
-    float* gr;                 // [4 num_nodes] entries: ratio
-    int* big_count;            // [1] big nodes listed in the pass (cleared by the scan)
-    int* big_n;                // [1] big nodes of the pass (written by the scan)
+                               // (+1,+1) from its first tile; -1: no entry (counting-sort form)
+    float4* geo;               // [room] entries: centre x, y, size x, y
+    float* gr;                 // [room] entries: ratio
+    int* big_count;            // [1] big nodes listed in the pass (cleared by the scan / by rooms_body)
+    int* big_n;                // [1] big nodes of the pass (written by the scan; counting-sort form)
     int* big;                  // [num_nodes] big node ids
-    int* done;                 // [1] count workgroups finished in the pass (the last one scans; back to zero after)
     const float* fixed;        // [nbx][nby]
     float* rho;                // density out (null: overflow only)
     int count_over;            // 1: this grid's overflow feeds the control
+    int fill;                  // 1: fill form, 0: counting-sort form
+    int room;                  // entries geo / gr hold
+    int room_pct, room_add;    // fill form: the room of a bucket for the next pass, count x room_pct / 100 + room_add
+    unsigned long long* spill; // [planes][nbx][nby] fill form: spilled contributions (zero between passes)
+    int* spill_tile;           // [tiles] fill form: 1 when the tile has spilled contributions (zero between passes)
+    unsigned long long* spill_n;  // [1] fill form: entries spilled since the level was created
 };
 
 __device__ __forceinline__ unsigned long long contribution(float wx, const Span& sy, int by) {
@@ -154,12 +172,10 @@ __device__ __forceinline__ bool small_node(const Span& sx, const Span& sy, int T
     return wx <= 1 && wy <= 1;
 }
 
-template <int NT, bool SEEN_BY_ATOMICS>
-__device__ __forceinline__ void scan_buckets(const BinJob& J);
-
-// fused_scan: the last workgroup to finish scans the buckets (the scan kernel's work, without its launch)
-__device__ __forceinline__ void bin_count_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk,
-                                               bool fused_scan) {
+// the count kernel of the counting-sort form (the batch's grouped launches) and of the fill form's set-up: every
+// small node's place in the bucket of each tile its bins fall in (an LDS histogram per workgroup, then one global
+// atomic per workgroup and bucket), big nodes to the grid's list
+__device__ __forceinline__ void bin_count_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk) {
     extern __shared__ int hist[];   // planes x tiles of the grid
     const bool judge = blk >= f.blocks;
     const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
@@ -207,35 +223,15 @@ __device__ __forceinline__ void bin_count_body(const Pick& pos_p, const BinJob& 
         o.w = rank[q][3] >= 0 ? hist[b0[q] + J.tiles_y + 1] + rank[q][3] : -1;
         J.off[i] = o;
     }
-    if (!fused_scan) return;
-    // The workgroup's count atomics have all returned (their values were used above) before it counts itself as
-    // done, so the last workgroup to do so reads the final counts with device-scope atomic loads.
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(J.done, 1) == J.blocks - 1;
-    __syncthreads();
-    if (!last) return;
-    scan_buckets<EB, true>(J);
-    if (threadIdx.x == 0) *J.done = 0;
 }
-// The single-level kernels of the binning run one grid per launch: eng_bin_* the force grid (f.blocks / 1 / f.wgs
-// workgroups), eng_over_* the overflow grid (j.blocks / 1 / j.wgs), so that the two can run side by side and each
-// launch reserves the LDS of its own grid only.
-__global__ __launch_bounds__(EB) void eng_bin_count(Pick pos_p, BinJob f, BinJob j) {
-    bin_count_body(pos_p, f, j, xcd_remap(blockIdx.x, gridDim.x), true);
-}
-__global__ __launch_bounds__(EB) void eng_over_count(Pick pos_p, BinJob f, BinJob j) {
-    bin_count_body(pos_p, f, j, f.blocks + xcd_remap(blockIdx.x, gridDim.x), true);
+// the fill form's set-up count (one grid per launch): the counts the first rooms are set from
+__global__ __launch_bounds__(EB) void eng_rooms_count(Pick pos_p, BinJob f, BinJob j, int which) {
+    bin_count_body(pos_p, f, j, (which ? f.blocks : 0) + xcd_remap(blockIdx.x, gridDim.x));
 }
 
-// the bucket starts from the counts of the pass (exclusive scan by the NT lanes of one workgroup, 4 NT buckets a
-// round, coalesced), the counts back to zero for the next pass. SEEN_BY_ATOMICS: the counts (and the big-node count)
-// are read with device-scope atomic loads, inside the kernel whose workgroups added them.
-template <bool SEEN_BY_ATOMICS>
-__device__ __forceinline__ int bucket_load(const int* p) {
-    return SEEN_BY_ATOMICS ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
-}
-template <int NT, bool SEEN_BY_ATOMICS>
+// counting-sort form: the bucket starts from the counts of the pass (exclusive scan by the NT lanes of one
+// workgroup, 4 NT buckets a round, coalesced), the counts back to zero for the next pass
+template <int NT>
 __device__ __forceinline__ void scan_buckets(const BinJob& J) {
     __shared__ int wsum[NT / 64];
     const int n = J.planes * J.tiles;
@@ -246,7 +242,7 @@ __device__ __forceinline__ void scan_buckets(const BinJob& J) {
         int c[4], sum = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            c[i] = k0 + i < n ? bucket_load<SEEN_BY_ATOMICS>(J.count + k0 + i) : 0;
+            c[i] = k0 + i < n ? J.count[k0 + i] : 0;
             sum += c[i];
         }
         int x = sum;                                // inclusive scan of the lanes' sums
@@ -274,7 +270,7 @@ __device__ __forceinline__ void scan_buckets(const BinJob& J) {
     }
     if (threadIdx.x == 0) {
         J.start[n] = carry;
-        *J.big_n = bucket_load<SEEN_BY_ATOMICS>(J.big_count);
+        *J.big_n = *J.big_count;
         *J.big_count = 0;
     }
 }
@@ -282,7 +278,83 @@ __device__ __forceinline__ void scan_buckets(const BinJob& J) {
 // the batch's scan launch (its count launches do not scan): one workgroup per grid
 __device__ __forceinline__ void bin_scan_body(const BinJob& f, const BinJob& j, int which) {
     const BinJob J = which ? j : f;
-    scan_buckets<SCAN_T, false>(J);
+    scan_buckets<SCAN_T>(J);
+}
+
+// Fill form: the rooms of the pass from the counts of the last one, by the EB lanes of one workgroup, 4 EB buckets a
+// round. Bucket b gets count x room_pct / 100 + room_add places, the starts are the rooms' prefix sums, clamped to
+// the entry arrays (sized so that the default rooms always fit; a room cut short only spills more), and the counts
+// and the big-node count go back to zero for the pass's fill.
+__device__ __forceinline__ void rooms_body(const BinJob& J) {
+    __shared__ long long wsum[EB / 64];
+    const int n = J.planes * J.tiles;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const long long cap = J.room;
+    long long carry = 0;                            // the same in every lane
+    for (int base = 0; base < n; base += 4 * EB) {
+        const int k0 = base + 4 * threadIdx.x;      // a multiple of 4: the int4 accesses are aligned
+        int4 c = make_int4(0, 0, 0, 0);
+        if (k0 + 3 < n) {
+            c = *(const int4*)(J.count + k0);
+        } else if (k0 < n) {
+            c.x = J.count[k0];
+            c.y = k0 + 1 < n ? J.count[k0 + 1] : 0;
+            c.z = k0 + 2 < n ? J.count[k0 + 2] : 0;
+        }
+        const int cs[4] = {c.x, c.y, c.z, c.w};
+        long long v[4], sum = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[i] = k0 + i < n ? (long long)cs[i] * J.room_pct / 100 + J.room_add : 0;
+            sum += v[i];
+        }
+        long long x = sum;                          // inclusive scan of the lanes' sums
+        for (int o = 1; o < 64; o <<= 1) {
+            const long long y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        long long run = carry + x - sum, total = 0;
+        for (int w = 0; w < EB / 64; w++) {
+            const long long t = wsum[w];
+            if (w < wave) run += t;
+            total += t;
+        }
+        __syncthreads();                            // wsum is rewritten by the next round
+        int st[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            st[i] = (int)(run < cap ? run : cap);
+            run += v[i];
+        }
+        if (k0 + 3 < n) {
+            *(int4*)(J.start + k0) = make_int4(st[0], st[1], st[2], st[3]);
+            *(int4*)(J.count + k0) = make_int4(0, 0, 0, 0);
+        } else {
+            for (int i = 0; i < 4 && k0 + i < n; i++) {
+                J.start[k0 + i] = st[i];
+                J.count[k0 + i] = 0;
+            }
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        J.start[n] = (int)(carry < cap ? carry : cap);
+        *J.big_count = 0;
+    }
+}
+// the rooms of both grids before an evaluation that does not follow a Nesterov step (workgroup 0: force grid,
+// 1: overflow grid)
+__global__ __launch_bounds__(EB) void eng_rooms(BinJob f, BinJob j) { rooms_body(blockIdx.x ? j : f); }
+// the Nesterov step of a pass; its first `rooms` workgroups set the rooms of the pass's fill (force grid, then
+// overflow grid), beside the step
+__global__ __launch_bounds__(EB) void eng_nesterov(NestArgs a, BinJob f, BinJob j, int rooms) {
+    if ((int)blockIdx.x < rooms) {
+        rooms_body(blockIdx.x ? j : f);
+        return;
+    }
+    nesterov_body(a, blockIdx.x - rooms);
 }
 
 __device__ __forceinline__ void bin_scatter_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk) {
@@ -323,12 +395,6 @@ __device__ __forceinline__ void bin_scatter_body(const Pick& pos_p, const BinJob
         }
     }
 }
-__global__ __launch_bounds__(EB) void eng_bin_scatter(Pick pos_p, BinJob f, BinJob j) {
-    bin_scatter_body(pos_p, f, j, xcd_remap(blockIdx.x, gridDim.x));
-}
-__global__ __launch_bounds__(EB) void eng_over_scatter(Pick pos_p, BinJob f, BinJob j) {
-    bin_scatter_body(pos_p, f, j, f.blocks + xcd_remap(blockIdx.x, gridDim.x));
-}
 
 // a node's contributions to the bins of the tile [X0, X1] x [Y0, Y1], into the tile's LDS accumulator a (row stride T)
 __device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0, int Y0, int X1, int Y1,
@@ -363,35 +429,140 @@ __device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0
     }
 }
 
+// Fill form: a spilled entry's contributions, clipped to the tile of bucket bk, into the grid's spill map
+__device__ __forceinline__ void spill_entry(const BinJob& J, int bk, const Span& sx, const Span& sy, float r) {
+    const int plane = bk >= J.tiles, t = bk - plane * J.tiles;
+    const int X0 = (t / J.tiles_y) * J.T, Y0 = (t % J.tiles_y) * J.T;
+    const int X1 = min(X0 + J.T, J.nbx) - 1, Y1 = min(Y0 + J.T, J.nby) - 1;
+    add_clipped(J.spill + ((long long)plane * J.nbx + X0) * J.nby + Y0, J.nby, X0, Y0, X1, Y1, sx, sy, r);
+    J.spill_tile[t] = 1;
+}
+
+// Fill form, one workgroup of BIN_NODES nodes: the ranks of its entries in an LDS histogram (rank + 1 of the entries
+// in tiles (+0,+0) (+0,+1) and (+1,+0) (+1,+1) in two words, 16 bits each: a rank is below BIN_NODES; 0: no entry);
+// then hist[b] becomes the workgroup's first place in b's room, or ~place when its entries of b do not all fit (they
+// then test their own place against the room); then the entries are written, the ones past their room spilled after
+// the others (one bit per entry in a lane mask, so the spill path is one copy of its code).
+__device__ __forceinline__ void bin_fill_body(const Pick& pos_p, const BinJob& f, const BinJob& j, int blk) {
+    extern __shared__ int hist[];   // planes x tiles of the grid
+    const bool judge = blk >= f.blocks;
+    const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
+    const int base = (blk - (judge ? f.blocks : 0)) * BIN_NODES;
+    const int nbuck = J.planes * J.tiles;
+    for (int t = threadIdx.x; t < nbuck; t += EB) hist[t] = 0;
+    __syncthreads();
+    const float2* pos = pos_p.get(true);
+    unsigned rk[BIN_PER_LANE][2];
+#pragma unroll
+    for (int q = 0; q < BIN_PER_LANE; q++) {
+        rk[q][0] = rk[q][1] = 0;
+        const int i = base + q * EB + threadIdx.x;
+        if (i >= J.num_nodes) continue;
+        const float2 p = pos[i], sz = J.size[i];
+        const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
+        if (sx.b0 > sx.b1 || sy.b0 > sy.b1) continue;
+        int t0, wx, wy;
+        if (!small_node(sx, sy, J.T, J.tiles_y, t0, wx, wy)) {
+            J.big[atomicAdd(J.big_count, 1)] = i;
+            continue;
+        }
+        const int b = t0 + (i >= J.num_mov ? J.tiles : 0);
+        rk[q][0] = (unsigned)atomicAdd(&hist[b], 1) + 1u;
+        if (wy) rk[q][0] |= ((unsigned)atomicAdd(&hist[b + 1], 1) + 1u) << 16;
+        if (wx) rk[q][1] = (unsigned)atomicAdd(&hist[b + J.tiles_y], 1) + 1u;
+        if (wx && wy) rk[q][1] |= ((unsigned)atomicAdd(&hist[b + J.tiles_y + 1], 1) + 1u) << 16;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nbuck; t += EB) {
+        const int c = hist[t];
+        if (!c) continue;
+        const int at = atomicAdd(&J.count[t], c), lo = J.start[t], hi = J.start[t + 1];
+        hist[t] = hi - lo - at >= c ? lo + at : ~(lo + at);
+    }
+    __syncthreads();
+    unsigned spilled = 0;   // bit 4 q + k: entry k of node q goes to the spill map
+#pragma unroll
+    for (int q = 0; q < BIN_PER_LANE; q++) {
+        if (!rk[q][0]) continue;                                // no entries: out of range, no bins or a big node
+        const int i = base + q * EB + threadIdx.x;
+        const float2 p = pos[i], sz = J.size[i];
+        const float r = J.ratio[i];
+        const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
+        int t0, wx, wy;
+        small_node(sx, sy, J.T, J.tiles_y, t0, wx, wy);
+        const int b = t0 + (i >= J.num_mov ? J.tiles : 0);
+        const float4 g = make_float4(p.x, p.y, sz.x, sz.y);
+        const unsigned rr[4] = {rk[q][0] & 0xFFFFu, rk[q][0] >> 16, rk[q][1] & 0xFFFFu, rk[q][1] >> 16};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!rr[k]) continue;
+            const int bk = b + (k & 1) + (k >> 1) * J.tiles_y;
+            const int h = hist[bk];
+            const int e = (h >= 0 ? h : ~h) + (int)rr[k] - 1;
+            if (h >= 0 || e < J.start[bk + 1]) {
+                J.geo[e] = g;
+                J.gr[e] = r;
+            } else {
+                spilled |= 1u << (4 * q + k);
+            }
+        }
+    }
+    while (spilled) {
+        const int bit = __builtin_ctz(spilled);
+        spilled &= spilled - 1;
+        const int q = bit >> 2, k = bit & 3;
+        const int i = base + q * EB + threadIdx.x;
+        const float2 p = pos[i], sz = J.size[i];
+        const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
+        int t0, wx, wy;
+        small_node(sx, sy, J.T, J.tiles_y, t0, wx, wy);
+        const int b = t0 + (i >= J.num_mov ? J.tiles : 0);
+        spill_entry(J, b + (k & 1) + (k >> 1) * J.tiles_y, sx, sy, J.ratio[i]);
+        atomicAdd(J.spill_n, 1ull);
+    }
+}
+__global__ __launch_bounds__(EB) void eng_bin_fill(Pick pos_p, BinJob f, BinJob j) {
+    bin_fill_body(pos_p, f, j, xcd_remap(blockIdx.x, gridDim.x));
+}
+__global__ __launch_bounds__(EB) void eng_over_fill(Pick pos_p, BinJob f, BinJob j) {
+    bin_fill_body(pos_p, f, j, f.blocks + xcd_remap(blockIdx.x, gridDim.x));
+}
+
 constexpr int TB = 512;     // lanes per workgroup of the tile kernel: a crowded tile's bucket spreads over 8 waves
 constexpr int TILE_PF = 4;  // fixed-map values per lane loaded ahead (the whole tile up to T x T = 4 x lanes)
 
 // One workgroup sums J.G tiles (G = 1, 2, 4, 8), each with its own TB / G lanes (whole waves) and its own LDS
 // accumulator, so that small tiles do not each hold a workgroup (and the launch's LDS reservation) to themselves.
 // The sub-groups pass the same barriers; a sub-group past the last tile only takes part in them.
-__device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& f, const BinJob& j, float target,
-                                               double* part, int wg) {
+// (wg: the workgroup within the grid's launch; part: the grid's overflow partials)
+__device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& job, float target, double* part,
+                                               int wg) {
     extern __shared__ unsigned long long acc_all[];   // G x planes x T x T
     __shared__ double red[TB / 64];
-    const bool judge = wg >= f.wgs;
-    const BinJob J = judge ? j : f;    // by value: a reference to a kernel argument would go to scratch
+    const BinJob J = job;              // by value: a reference to a kernel argument would go to scratch
     const int S = TB / J.G;            // lanes per tile
     const int sub = threadIdx.x / S, l = threadIdx.x - sub * S;
-    const int t = (wg - (judge ? f.wgs : 0)) * J.G + sub;
+    const int t = wg * J.G + sub;
     const bool live = t < J.tiles;
     const int T = J.T, TT = T * T;
     const int X0 = (t / J.tiles_y) * T, Y0 = (t % J.tiles_y) * T;
     const int X1 = min(X0 + T, J.nbx) - 1, Y1 = min(Y0 + T, J.nby) - 1;
     unsigned long long* acc = acc_all + sub * J.planes * TT;
-    // issued before anything waits on memory: the bucket bounds of both planes and this lane's share of the fixed
-    // map (the first TILE_PF bins of its stride), so that a tile pays two dependent memory latencies, not four
-    int lo0 = 0, n0 = 0, lo1 = 0, n1 = 0;
+    // issued before anything waits on memory: the bucket bounds of both planes (fill form: with their counts and the
+    // tile's spill flag) and this lane's share of the fixed map (the first TILE_PF bins of its stride), so that a tile
+    // pays two dependent memory latencies, not four
+    int lo0 = 0, n0 = 0, lo1 = 0, n1 = 0, c0 = 0, c1 = 0, spilled = 0;
     if (live) {
         lo0 = J.start[t];
         n0 = J.start[t + 1];
         if (J.planes == 2) {
             lo1 = J.start[J.tiles + t];
             n1 = J.start[J.tiles + t + 1];
+        }
+        if (J.fill) {
+            c0 = J.count[t];
+            if (J.planes == 2) c1 = J.count[J.tiles + t];
+            spilled = J.spill_tile[t];
         }
     }
     float fix[TILE_PF];
@@ -404,6 +575,10 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
         for (int e = l; e < J.planes * TT; e += S) acc[e] = 0ull;
     n0 -= lo0;
     n1 -= lo1;
+    if (J.fill) {                      // the entries in the rooms (the rest were spilled)
+        n0 = min(n0, c0);
+        n1 = min(n1, c1);
+    }
     __syncthreads();
     // the entries of both planes as one range (plane 0 first), two per lane per round, both loaded before either is
     // used
@@ -432,7 +607,7 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
                             make_span(g[k].y, g[k].w, J.inv_by, J.nby), r[k]);
     }
     // big nodes: one at a time, their bins in this tile spread over the tile's lanes
-    const int nbig = live ? *J.big_n : 0;
+    const int nbig = live ? (J.fill ? *J.big_count : *J.big_n) : 0;
     if (nbig) {
         const float2* pos = pos_p.get(true);
         for (int k = 0; k < nbig; k++) {
@@ -450,6 +625,20 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
                 if (c) atomicAdd(a + (bx - X0) * T + (by - Y0), c);
             }
         }
+    }
+    // fill form: the spilled entries' sums of this tile's bins, added and cleared for the next pass
+    if (spilled) {
+        for (int e = l; e < J.planes * TT; e += S) {
+            const int pl = e >= TT, bx = X0 + (e - pl * TT) / T, by = Y0 + (e - pl * TT) % T;
+            if (bx > X1 || by > Y1) continue;
+            unsigned long long* sp = J.spill + ((long long)pl * J.nbx + bx) * J.nby + by;
+            const unsigned long long v = *sp;
+            if (v) {
+                atomicAdd(acc + e, v);
+                *sp = 0ull;
+            }
+        }
+        if (l == 0) J.spill_tile[t] = 0;
     }
     __syncthreads();
     double o = 0.0;
@@ -478,14 +667,15 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
     if (live && l == 0) {
         double s = 0;
         for (int w = 0; w < S / 64; w++) s += red[sub * (S / 64) + w];
-        part[(judge ? f.tiles : 0) + t] = s;
+        part[t] = s;
     }
 }
-__global__ __launch_bounds__(TB, 8) void eng_bin_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
-    bin_tiles_body(pos_p, f, j, target, part, blockIdx.x);
+// the overflow partials: the force grid's tiles, then the overflow grid's
+__global__ __launch_bounds__(TB, 8) void eng_bin_tiles(Pick pos_p, BinJob f, float target, double* part) {
+    bin_tiles_body(pos_p, f, target, part, blockIdx.x);
 }
-__global__ __launch_bounds__(TB, 8) void eng_over_tiles(Pick pos_p, BinJob f, BinJob j, float target, double* part) {
-    bin_tiles_body(pos_p, f, j, target, part, f.wgs + blockIdx.x);
+__global__ __launch_bounds__(TB, 8) void eng_over_tiles(Pick pos_p, BinJob j, float target, double* part, int off) {
+    bin_tiles_body(pos_p, j, target, part + off, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------ gather (+ cell WA sum, |g' - g|)
@@ -857,7 +1047,7 @@ struct GraphEntry {            // a captured run of `passes` passes
     int timing = -1;           // kernel class bracketed by events (-1: none)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<hipEvent_t> ev;   // timing: 2 per pass (before the class, after it)
+    std::vector<hipEvent_t> ev;   // timing: 2 per timed pass (before the class, after it)
     void clear() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -880,6 +1070,8 @@ struct Level {
     int* block_net = nullptr;  // WA workgroup -> first owned net
     int* pin_slot = nullptr;   // pin -> position in the cell CSR (-1: fixed pin)
     BinJob fj{}, jj{};         // density binning of the force grid and of the overflow grid
+    bool rooms_stale = true;   // the bucket counts are not the last evaluation's at the current layout (set-up, a
+                               // new layout): init_rooms counts before the next evaluation
     size_t hist_f = 0, hist_j = 0, acc_f = 0, acc_j = 0;   // dynamic LDS of the count / tile kernels per grid
     size_t hist_lds = 0, acc_lds = 0;                       // the larger of the two (the batch's grouped launches)
     std::vector<void*> bin_mem;
@@ -901,6 +1093,8 @@ struct Level {
     int graph_next = 0;
     // event timing of one kernel class: events before and after it in every pass of a run (graph or direct)
     int timing_cls = -1;
+    int time_every = 1;             // events in passes 0, every, 2 every, ... of a run only
+    int timed_passes(int passes) const { return (passes + time_every - 1) / time_every; }
     hipEvent_t* rec_ev = nullptr;   // events of the run being captured or launched (null: none)
     bool rec_capture = false;       // true: added to the graph being captured as event-record nodes
     int rec_pass = 0;
@@ -919,8 +1113,8 @@ struct Level {
     void tbegin(int k, hipStream_t s) { tevent(k, 0, s); }
     void tend(int k, hipStream_t s) { tevent(k, 1, s); }
     void tevent(int k, int end, hipStream_t s) {
-        if (!rec_ev || k != timing_cls) return;
-        hipEvent_t ev = rec_ev[2 * rec_pass + end];
+        if (!rec_ev || k != timing_cls || rec_pass % time_every) return;
+        hipEvent_t ev = rec_ev[2 * (rec_pass / time_every) + end];
         const hipError_t e = rec_capture ? record_in_capture(ev, s) : hipEventRecord(ev, s);
         if (e != hipSuccess && rec_err == hipSuccess) rec_err = e;
     }
@@ -962,6 +1156,7 @@ struct Level {
     void set_side(BinJob& J, int T, int G = 0);
     void update_lds();
     int set_lds() const;
+    int init_rooms(const Pick& pos, hipStream_t s);
     int solve_field(hipStream_t s);
     int bin_grid(int which, const Pick& pos, hipStream_t s);
     int tiles_grid(int which, const Pick& pos, hipStream_t s);
@@ -970,7 +1165,8 @@ struct Level {
     int create(const ap_level_desc& desc);
     void destroy();
     int objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev, const float2* g_prev_b,
-                  float2* g_out_a, float2* g_out_b, hipStream_t s, double* S, float* kp_out, int use_density);
+                  float2* g_out_a, float2* g_out_b, hipStream_t s, double* S, float* kp_out, int use_density,
+                  bool rooms_set = false);
     // the kernels' arguments (shared by the single-level launches and the batch's tables)
     WaArgs wa_args(const Pick& pos, const float* kp_ptr) const;
     GatherArgs gather_args(const Pick& pos, const float* kp_ptr, const float2* g_prev, const float2* g_prev_b,
@@ -979,6 +1175,7 @@ struct Level {
     double over_scale() const;
     NestArgs nest_args() const;
     Pick pass_pos() const;
+    Pick cur_pos() const;
     int pass(hipStream_t s);
     int finalize(hipStream_t s);
     int graph_for(int passes, int& slot);
@@ -1003,23 +1200,35 @@ int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nod
         set_error("level: density grid %d x %d is beyond the tile binning's %d buckets", nbx, nby, BIN_MAX_BUCKETS);
         return AP_ERR_UNSUPPORTED;
     }
+    AP_REQUIRE(num_nodes <= (1 << 27), "level: more than 2^27 nodes on a density grid");
     J.blocks = (int)grid_of(num_nodes, BIN_NODES);
     J.fixed = fixed;
     J.rho = out;
     J.count_over = count_over;
     set_side(J, J.T);
-    // bucket arrays for the largest bucket count any side may use, so that a new side needs no allocation (the
-    // scan leaves the counts it used at zero; the others were never used)
+    // Entry arrays for the fill form's rooms: a node has at most 4 entries, so the rooms of the default percentage
+    // (150 % of a count + 64) fit in 6 entries per node + 64 per bucket of the side with the most buckets (8).
+    // Bucket arrays for the largest bucket count any side may use, so that a new side needs no allocation.
+    J.fill = 1;
+    J.room_pct = 150;
+    J.room_add = 64;
+    const long long nbuck_max =
+        std::min<long long>(BIN_MAX_BUCKETS, (long long)planes * grid_of(nbx, 8) * grid_of(nby, 8));
+    J.room = (int)(6LL * num_nodes + 64LL * nbuck_max);
     int st;
     if ((st = alloc(&J.count, BIN_MAX_BUCKETS)) || (st = alloc(&J.start, BIN_MAX_BUCKETS + 1)) ||
-        (st = alloc(&J.off, num_nodes)) || (st = alloc(&J.geo, 4LL * num_nodes)) ||
-        (st = alloc(&J.gr, 4LL * num_nodes)) || (st = alloc(&J.big_count, 1)) || (st = alloc(&J.big_n, 1)) ||
-        (st = alloc(&J.big, num_nodes)) || (st = alloc(&J.done, 1)))
+        (st = alloc(&J.off, num_nodes)) || (st = alloc(&J.geo, J.room)) || (st = alloc(&J.gr, J.room)) ||
+        (st = alloc(&J.big_count, 1)) || (st = alloc(&J.big_n, 1)) || (st = alloc(&J.big, num_nodes)) ||
+        (st = alloc(&J.spill, (long long)planes * nbx * nby)) || (st = alloc(&J.spill_tile, BIN_MAX_BUCKETS)) ||
+        (st = alloc(&J.spill_n, 1)))
         return st;
+    AP_HIP(hipMemset(J.spill_n, 0, sizeof(unsigned long long)));
     AP_HIP(hipMemset(J.count, 0, BIN_MAX_BUCKETS * sizeof(int)));
+    AP_HIP(hipMemset(J.start, 0, (BIN_MAX_BUCKETS + 1) * sizeof(int)));
     AP_HIP(hipMemset(J.big_count, 0, sizeof(int)));
     AP_HIP(hipMemset(J.big_n, 0, sizeof(int)));
-    AP_HIP(hipMemset(J.done, 0, sizeof(int)));
+    AP_HIP(hipMemset(J.spill, 0, (size_t)planes * nbx * nby * sizeof(unsigned long long)));
+    AP_HIP(hipMemset(J.spill_tile, 0, BIN_MAX_BUCKETS * sizeof(int)));
     update_lds();
     return AP_OK;
 }
@@ -1039,6 +1248,7 @@ static bool group_ok(int planes, int T, int G) {
 }
 
 void Level::set_side(BinJob& J, int T, int G) {
+    rooms_stale = true;
     J.T = T;
     J.tiles_y = (int)grid_of(J.nby, T);
     J.tiles = (int)grid_of(J.nbx, T) * J.tiles_y;
@@ -1059,13 +1269,31 @@ void Level::update_lds() {
     acc_lds = std::max(acc_f, acc_j);
 }
 
-// the dynamic LDS limits of the single-level count and tile kernels, for the current layout
+// the dynamic LDS limits of the single-level fill, count and tile kernels, for the current layout
 int Level::set_lds() const {
-    const void* fn[4] = {(const void*)eng_bin_count, (const void*)eng_over_count, (const void*)eng_bin_tiles,
-                         (const void*)eng_over_tiles};
-    const size_t lds[4] = {hist_f, hist_j, acc_f, acc_j};
-    for (int k = 0; k < 4; k++)
+    const void* fn[5] = {(const void*)eng_bin_fill, (const void*)eng_over_fill, (const void*)eng_bin_tiles,
+                         (const void*)eng_over_tiles, (const void*)eng_rooms_count};
+    const size_t lds[5] = {hist_f, hist_j, acc_f, acc_j, hist_lds};
+    for (int k = 0; k < 5; k++)
         if (lds[k]) AP_HIP(hipFuncSetAttribute(fn[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds[k]));
+    return AP_OK;
+}
+
+// The fill form's counts at pos for the current layout (set-up, or a new layout), outside any graph: the rooms of
+// the next evaluation are set from them. Spill map and flags are zero between evaluations whatever the layout.
+int Level::init_rooms(const Pick& pos, hipStream_t s) {
+    if (!rooms_stale) return AP_OK;
+    int st;
+    if ((st = set_lds())) return st;
+    for (int w = 0; w < 2; w++) {
+        const BinJob& J = w ? jj : fj;
+        if (!J.count) continue;
+        AP_HIP(hipMemsetAsync(J.count, 0, BIN_MAX_BUCKETS * sizeof(int), s));
+        AP_HIP(hipMemsetAsync(J.big_count, 0, sizeof(int), s));
+        hipLaunchKernelGGL(eng_rooms_count, dim3(J.blocks), dim3(EB), w ? hist_j : hist_f, s, pos, fj, jj, w);
+        AP_LAUNCHED();
+    }
+    rooms_stale = false;
     return AP_OK;
 }
 
@@ -1074,58 +1302,75 @@ int Level::solve_field(hipStream_t s) {
     return f2 ? poisson.solve_pairs(rho, d.nby, f2, d.nby, s) : poisson.solve(rho, d.nby, fx, fy, d.nby, s);
 }
 
-// count (with the scan in its last workgroup) and scatter of one grid (0: force, 1: overflow)
+// the fill of one grid (0: force, 1: overflow); its rooms are set
 int Level::bin_grid(int which, const Pick& pos, hipStream_t s) {
-    if (!which) {
-        hipLaunchKernelGGL(eng_bin_count, dim3(fj.blocks), dim3(EB), hist_f, s, pos, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_bin_scatter, dim3(fj.blocks), dim3(EB), 0, s, pos, fj, jj);
-        AP_LAUNCHED();
-    } else {
-        hipLaunchKernelGGL(eng_over_count, dim3(jj.blocks), dim3(EB), hist_j, s, pos, fj, jj);
-        AP_LAUNCHED();
-        hipLaunchKernelGGL(eng_over_scatter, dim3(jj.blocks), dim3(EB), 0, s, pos, fj, jj);
-        AP_LAUNCHED();
-    }
+    if (!which)
+        hipLaunchKernelGGL(eng_bin_fill, dim3(fj.blocks), dim3(EB), hist_f, s, pos, fj, jj);
+    else
+        hipLaunchKernelGGL(eng_over_fill, dim3(jj.blocks), dim3(EB), hist_j, s, pos, fj, jj);
+    AP_LAUNCHED();
     return AP_OK;
 }
 
 // the tile sums of one grid: density and overflow partials
 int Level::tiles_grid(int which, const Pick& pos, hipStream_t s) {
     if (!which)
-        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.wgs), dim3(TB), acc_f, s, pos, fj, jj, d.target, p_over);
+        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.wgs), dim3(TB), acc_f, s, pos, fj, d.target, p_over);
     else
-        hipLaunchKernelGGL(eng_over_tiles, dim3(jj.wgs), dim3(TB), acc_j, s, pos, fj, jj, d.target, p_over);
+        hipLaunchKernelGGL(eng_over_tiles, dim3(jj.wgs), dim3(TB), acc_j, s, pos, jj, d.target, p_over, fj.tiles);
     AP_LAUNCHED();
     return AP_OK;
 }
 
-// The front of a pass as the passes run it -- WA, the cells' WA gradients and the overflow grid's binning and tiles
-// on the side stream, the force grid's binning, tiles and Poisson solve on s -- at the caller's current point v
-// (between runs the current point is always set 0), `reps` times: ms per run. It changes nothing a pass reads before
-// writing it again (bucket counts end at zero; pin and cell gradients, partials, rho and the field are rewritten by
+// The front of a pass as the passes run it, captured into a graph and replayed `reps` times: ms per replay. The
+// rooms, then the force grid's fill, tiles and Poisson solve on s and, forked beside them, WA, the cells' WA gradients
+// and the overflow grid's fill and tiles on the side stream -- in the order objective() captures them, so that the
+// replay runs on the hardware queues the passes' graphs run on -- at the caller's current point v (between runs the
+// current point is always set 0). It changes nothing a pass reads before writing it again (the bucket counts are the
+// last evaluation's, which set the next rooms; pin and cell gradients, partials, rho and the field are rewritten by
 // every pass).
 int Level::time_front(int reps, hipStream_t s, float& ms) {
     const Pick pos{(const float2*)d.v, nullptr, nullptr};
     int st;
-    if ((st = set_lds())) return st;
-    AP_HIP(hipEventRecord(t0, s));
-    for (int r = 0; r < reps; r++) {
+    if ((st = set_lds()) || (st = init_rooms(pos, s))) return st;
+    AP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    auto front = [&]() -> int {
+        hipLaunchKernelGGL(eng_rooms, dim3(jj.count ? 2 : 1), dim3(EB), 0, s, fj, jj);
+        AP_LAUNCHED();
         AP_HIP(hipEventRecord(ev_fork, s));
         AP_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-        if ((st = wa_plan.launch(wa_args(pos, d.kp), p_wa, side))) return st;
+        int e;
+        if ((e = bin_grid(0, pos, s))) return e;
+        if ((e = wa_plan.launch(wa_args(pos, d.kp), p_wa, side))) return e;
         hipLaunchKernelGGL(eng_cell_wl, dim3(nblk_n), dim3(EB), 0, side,
                            gather_args(pos, d.kp, nullptr, nullptr, nullptr, nullptr, 1), cell_wl, p_wl2);
         AP_LAUNCHED();
-        if (jj.count && ((st = bin_grid(1, pos, side)) || (st = tiles_grid(1, pos, side)))) return st;
-        if ((st = bin_grid(0, pos, s)) || (st = tiles_grid(0, pos, s))) return st;
-        if ((st = solve_field(s))) return st;
+        if (jj.count && (e = bin_grid(1, pos, side))) return e;
+        if ((e = tiles_grid(0, pos, s))) return e;
+        if (jj.count && (e = tiles_grid(1, pos, side))) return e;
+        if ((e = solve_field(s))) return e;
         AP_HIP(hipEventRecord(ev_join, side));
         AP_HIP(hipStreamWaitEvent(s, ev_join, 0));
+        return AP_OK;
+    };
+    st = front();
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    hipGraphExec_t x = nullptr;
+    hipError_t e = st ? hipSuccess : ec;
+    if (!st && e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (!st && e == hipSuccess) e = hipEventRecord(t0, s);
+    for (int r = 0; r < reps && !st && e == hipSuccess; r++) e = hipGraphLaunch(x, s);
+    if (!st && e == hipSuccess) e = hipEventRecord(t1, s);
+    if (!st && e == hipSuccess) e = hipEventSynchronize(t1);
+    if (!st && e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    if (st) {
+        (void)hipGetLastError();
+        return st;
     }
-    AP_HIP(hipEventRecord(t1, s));
-    AP_HIP(hipEventSynchronize(t1));
-    AP_HIP(hipEventElapsedTime(&ms, t0, t1));
+    AP_HIP(e);
     ms /= reps;
     return AP_OK;
 }
@@ -1167,6 +1412,14 @@ int Level::autotune(int reps, hipStream_t s, int* sides) {
         }
         set_side(*J, best, best_g);
         update_lds();
+    }
+    // the timing runs leave the fill form's counts behind: back to zero, the state the counting-sort form (the
+    // batch's levels) starts from; the fill form counts again at the new layout anyway (rooms_stale)
+    AP_HIP(hipDeviceSynchronize());
+    for (BinJob* J : {&fj, &jj}) {
+        if (!J->count) continue;
+        AP_HIP(hipMemset(J->count, 0, BIN_MAX_BUCKETS * sizeof(int)));
+        AP_HIP(hipMemset(J->big_count, 0, sizeof(int)));
     }
     if ((st = set_lds())) return st;
     for (GraphEntry& G : graphs) {                    // the kernels' arguments change: capture again
@@ -1331,6 +1584,9 @@ NestArgs Level::nest_args() const {
 }
 
 Pick Level::pass_pos() const { return Pick{(const float2*)d.v, (const float2*)v1, flags}; }
+// the current point, as the kernels that read get(true) see it (the sets swapped against pass_pos: before a pass,
+// the point the pass's Nesterov step starts from)
+Pick Level::cur_pos() const { return Pick{(const float2*)v1, (const float2*)d.v, flags}; }
 
 // objective at pos with the parameters kp; g_prev (optional) gives |g' - g|. With S the control pass follows.
 // The objective's kernels. Passes that are not profiled run two streams: the caller's stream s takes the force grid
@@ -1339,11 +1595,15 @@ Pick Level::pass_pos() const { return Pick{(const float2*)d.v, (const float2*)v1
 // passes run every kernel class in order on s, so that the events between the classes time each of them alone.
 int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev,
                      const float2* g_prev_b, float2* g_out_a, float2* g_out_b, hipStream_t s, double* S, float* kp_out,
-                     int use_density) {
+                     int use_density, bool rooms_set) {
     int st;
     const bool two = !prof;
     const bool over = use_density && jj.count;
     hipStream_t w = s;                                       // the stream of WA and the overflow grid
+    if (use_density && !rooms_set) {                         // not after a Nesterov step (which sets them)
+        hipLaunchKernelGGL(eng_rooms, dim3(jj.count ? 2 : 1), dim3(EB), 0, s, fj, jj);
+        AP_LAUNCHED();
+    }
     if (two) {
         w = side;
         AP_HIP(hipEventRecord(ev_fork, s));
@@ -1353,6 +1613,14 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     // other order on the 2M-cell design: WA's streaming overlaps the force grid's binning, and the overflow grid's
     // latency-bound tiles the force grid's); profiled passes keep the class order on s
     GatherArgs g = gather_args(pos, kp_ptr, g_prev, g_prev_b, g_out_a, g_out_b, use_density);
+    // two streams: the force grid's fill is captured first, so that the graph's first branch after the fork -- the
+    // one that keeps the caller's hardware queue, with no cross-queue wait to start -- is the force grid's chain
+    // (fill, tiles, Poisson: the longer one), and the side stream's chain takes the other queue
+    if (two && use_density) {
+        tbegin(AP_PROF_MAP, s);
+        if ((st = bin_grid(0, pos, s))) return st;
+        tend(AP_PROF_MAP, s);
+    }
     mark(AP_PROF_WA, s);
     tbegin(AP_PROF_WA, w);
     if ((st = wa_plan.launch(wa_args(pos, kp_ptr), p_wa, w))) return st;
@@ -1366,10 +1634,8 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     g.cell_wl = cell_wl;
     if (use_density) {
         mark(AP_PROF_MAP, s);
-        tbegin(AP_PROF_MAP, s);
         if (over && (st = bin_grid(1, pos, w))) return st;
-        if ((st = bin_grid(0, pos, s))) return st;
-        tend(AP_PROF_MAP, s);
+        if (!two && (st = bin_grid(0, pos, s))) return st;
         mark(AP_PROF_POST, s);
         tbegin(AP_PROF_POST, s);
         if ((st = tiles_grid(0, pos, s))) return st;
@@ -1410,10 +1676,11 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
 int Level::pass(hipStream_t s) {
     mark(AP_PROF_NESTEROV, s);
     tbegin(AP_PROF_NESTEROV, s);
-    hipLaunchKernelGGL(eng_nesterov, dim3(nblk_n), dim3(EB), 0, s, nest_args());
+    const int rooms = jj.count ? 2 : 1;                      // the rooms of the pass's fill, beside the step
+    hipLaunchKernelGGL(eng_nesterov, dim3(rooms + nblk_n), dim3(EB), 0, s, nest_args(), fj, jj, rooms);
     AP_LAUNCHED();
     tend(AP_PROF_NESTEROV, s);
-    return objective(pass_pos(), d.kp, true, (const float2*)d.g, g1, (float2*)d.g, g1, s, d.S, d.kp, 1);
+    return objective(pass_pos(), d.kp, true, (const float2*)d.g, g1, (float2*)d.g, g1, s, d.S, d.kp, 1, true);
 }
 
 int Level::finalize(hipStream_t s) {
@@ -1437,7 +1704,7 @@ int Level::graph_for(int passes, int& slot) {
     if (last_ev == &G.ev) last_ev = nullptr;
     G.clear();
     if (timing_cls >= 0) {
-        G.ev.resize(2 * (size_t)passes, nullptr);
+        G.ev.resize(2 * (size_t)timed_passes(passes), nullptr);
         for (hipEvent_t& e : G.ev) AP_HIP(hipEventCreate(&e));
     }
     rec_ev = timing_cls >= 0 ? G.ev.data() : nullptr;
@@ -1510,7 +1777,7 @@ __global__ __launch_bounds__(EB) void eng_bin_count_grouped(const PassTab* tab, 
     const int4 m = map[blockIdx.x];
     if (stopped(tab, m.x)) return;
     const PassTab& T = tab[m.x];
-    bin_count_body(T.pos, T.fj, T.jj, m.y, false);
+    bin_count_body(T.pos, T.fj, T.jj, m.y);
 }
 __global__ __launch_bounds__(SCAN_T) void eng_bin_scan_grouped(const PassTab* tab, const int4* map) {
     const int4 m = map[blockIdx.x];
@@ -1528,7 +1795,9 @@ __global__ __launch_bounds__(TB) void eng_bin_tiles_grouped(const PassTab* tab, 
     const int4 m = map[blockIdx.x];
     if (stopped(tab, m.x)) return;
     const PassTab& T = tab[m.x];
-    bin_tiles_body(T.pos, T.fj, T.jj, T.target, T.p_over, m.y);
+    const bool judge = m.y >= T.fj.wgs;
+    bin_tiles_body(T.pos, judge ? T.jj : T.fj, T.target, T.p_over + (judge ? T.fj.tiles : 0),
+                   m.y - (judge ? T.fj.wgs : 0));
 }
 __global__ __launch_bounds__(EB) void eng_gather_grouped(const PassTab* tab, const int4* map) {
     const int4 m = map[blockIdx.x];
@@ -1632,6 +1901,8 @@ int Batch::build() {
         T.pos = pos;
         T.fj = L.fj;
         T.jj = L.jj;
+        T.fj.fill = 0;                                 // the grouped launches: counting-sort form
+        T.jj.fill = 0;
         T.target = L.d.target;
         T.p_over = L.p_over;
         T.gather = L.gather_args(pos, L.d.kp, (const float2*)L.d.g, L.g1, (float2*)L.d.g, L.g1, 1);
@@ -1805,17 +2076,18 @@ int ap_level_run(ap_level* level, int passes, int use_graph, void* stream) {
     if (use_graph && passes > 0 && (st = L.graph_for(passes, slot))) return st;
     AP_HIP(hipEventRecord(L.ev_in, caller));
     AP_HIP(hipStreamWaitEvent(L.stream, L.ev_in, 0));
+    if ((st = L.init_rooms(L.cur_pos(), L.stream))) return st;
     AP_HIP(hipEventRecord(L.t0, L.stream));
     if (slot >= 0) {
         GraphEntry& G = L.graphs[slot];
         AP_HIP(hipGraphLaunch(G.exec, L.stream));
         L.last_ev = G.timing >= 0 ? &G.ev : nullptr;
-        L.last_ev_passes = G.passes;
+        L.last_ev_passes = (int)(G.ev.size() / 2);
     } else {
         const bool timed = L.timing_cls >= 0 && passes > 0;
         if (timed) {
             if (L.last_ev == &L.direct_ev) L.last_ev = nullptr;
-            while (L.direct_ev.size() < 2 * (size_t)passes) {
+            while (L.direct_ev.size() < 2 * (size_t)L.timed_passes(passes)) {
                 hipEvent_t e = nullptr;
                 AP_HIP(hipEventCreate(&e));
                 L.direct_ev.push_back(e);
@@ -1836,7 +2108,7 @@ int ap_level_run(ap_level* level, int passes, int use_graph, void* stream) {
             return AP_ERR_HIP;
         }
         L.last_ev = timed ? &L.direct_ev : nullptr;
-        L.last_ev_passes = passes;
+        L.last_ev_passes = timed ? L.timed_passes(passes) : 0;
     }
     AP_HIP(hipEventRecord(L.t1, L.stream));
     AP_HIP(hipEventRecord(L.ev_out, L.stream));
@@ -1847,6 +2119,18 @@ int ap_level_run(ap_level* level, int passes, int use_graph, void* stream) {
 int ap_level_time_class(ap_level* level, int cls) {
     AP_REQUIRE(level && cls >= -1 && cls < AP_PROF_CLASSES, "level_time_class: bad arguments");
     level->L.timing_cls = cls;
+    return AP_OK;
+}
+
+int ap_level_time_every(ap_level* level, int every) {
+    AP_REQUIRE(level && every > 0, "level_time_every: bad arguments");
+    Level& L = level->L;
+    AP_HIP(hipDeviceSynchronize());
+    L.time_every = every;
+    for (GraphEntry& G : L.graphs) {                 // the timed passes change: capture again
+        if (L.last_ev == &G.ev) L.last_ev = nullptr;
+        G.clear();
+    }
     return AP_OK;
 }
 
@@ -1877,8 +2161,9 @@ int ap_level_profile(ap_level* level, int passes, double* ms, void* stream) {
     for (int k = 0; k < AP_PROF_CLASSES; k++) L.prof_ms[k] = 0;
     AP_HIP(hipEventRecord(L.ev_in, caller));
     AP_HIP(hipStreamWaitEvent(L.stream, L.ev_in, 0));
+    int st = L.init_rooms(L.cur_pos(), L.stream);
+    if (st) return st;
     L.prof = true;
-    int st = AP_OK;
     for (int p = 0; p < passes && !st; p++) {
         st = L.pass(L.stream);
         if (st) break;
@@ -1916,7 +2201,9 @@ int ap_level_objective(ap_level* level, const float* pos, float* grad, double* s
     Level& L = level->L;
     hipStream_t s = (hipStream_t)stream;
     Pick p{(const float2*)pos, nullptr, nullptr};
-    int st = L.objective(p, L.d.kp, false, nullptr, nullptr, (float2*)grad, (float2*)grad, s, nullptr, nullptr, 1);
+    int st = L.init_rooms(p, s);
+    if (st) return st;
+    st = L.objective(p, L.d.kp, false, nullptr, nullptr, (float2*)grad, (float2*)grad, s, nullptr, nullptr, 1);
     if (st) return st;
     AP_HIP(hipMemcpyAsync(stats, L.stats, 5 * sizeof(double), hipMemcpyDeviceToDevice, s));
     return AP_OK;
@@ -2064,6 +2351,32 @@ int ap_level_set_tile_layout(ap_level* level, int force_side, int force_group, i
         if (L.last_ev == &G.ev) L.last_ev = nullptr;
         G.clear();
     }
+    return AP_OK;
+}
+
+int ap_level_set_bucket_room(ap_level* level, int percent, int extra) {
+    AP_REQUIRE(level && percent >= 0 && percent <= 150 && extra >= 0 && extra <= 64,
+               "level_set_bucket_room: percent %d, extra %d outside 0..150, 0..64", percent, extra);
+    Level& L = level->L;
+    AP_HIP(hipDeviceSynchronize());
+    for (BinJob* J : {&L.fj, &L.jj}) {
+        J->room_pct = percent;
+        J->room_add = extra;
+    }
+    for (GraphEntry& G : L.graphs) {                 // the kernels' arguments change: capture again
+        if (L.last_ev == &G.ev) L.last_ev = nullptr;
+        G.clear();
+    }
+    return AP_OK;
+}
+
+int ap_level_bucket_spills(ap_level* level, unsigned long long* spills) {
+    AP_REQUIRE(level && spills, "level_bucket_spills: null argument");
+    Level& L = level->L;
+    AP_HIP(hipDeviceSynchronize());
+    spills[0] = spills[1] = 0;
+    if (L.fj.spill_n) AP_HIP(hipMemcpy(&spills[0], L.fj.spill_n, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (L.jj.spill_n) AP_HIP(hipMemcpy(&spills[1], L.jj.spill_n, sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return AP_OK;
 }
 

@@ -110,6 +110,16 @@ class Engine(object):
         _lib.check(_lib.lib().ap_level_set_tile_layout(self._h, int(force[0]), int(force[1]), int(overflow[0]),
                                                        int(overflow[1])), "level_set_tile_layout")
 
+    def set_bucket_room(self, percent=150, extra=64):
+        """Each density bucket's room per pass: percent % of its last count + extra (smaller: more entries spill)."""
+        _lib.check(_lib.lib().ap_level_set_bucket_room(self._h, int(percent), int(extra)), "level_set_bucket_room")
+
+    def bucket_spills(self):
+        """(force, overflow): entries that took the density binning's spill path since the engine was created."""
+        out = (ctypes.c_ulonglong * 2)()
+        _lib.check(_lib.lib().ap_level_bucket_spills(self._h, ctypes.cast(out, ctypes.c_void_p)), "level_bucket_spills")
+        return int(out[0]), int(out[1])
+
     def finish(self):
         """Write u / v / g / best back into the caller's tensors (caller's node order)."""
         if self.perm is None:
@@ -133,8 +143,12 @@ class Engine(object):
         cls = -1 if name is None else _lib.PROF_CLASSES.index(name)
         _lib.check(_lib.lib().ap_level_time_class(self._h, cls), "level_time_class")
 
+    def time_every(self, every):
+        """Events around the timed class in passes 0, every, 2 every, ... of a run only (1: every pass)."""
+        _lib.check(_lib.lib().ap_level_time_every(self._h, int(every)), "level_time_every")
+
     def class_ms(self):
-        """(average ms per pass of the timed class over the last graph run, passes of that run)."""
+        """(average ms of the timed class over the last run's timed passes, number of timed passes)."""
         ms, n = ctypes.c_double(), ctypes.c_int()
         _lib.check(_lib.lib().ap_level_class_ms(self._h, ctypes.byref(ms), ctypes.byref(n)), "level_class_ms")
         return ms.value, n.value
@@ -241,6 +255,7 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
             if stop:
                 break
         eng.finish()
+        objective.bucket_spills = eng.bucket_spills()   # density entries past their bucket's room (instrumentation)
         if controller is not None:
             st = eng.controller_state()
             n = int(st[7])

@@ -102,6 +102,7 @@ def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=N
                 % (level.iters, level.reason, level.hpwl, level.overflow, weight))
         evals, first_iter, start_iters = cobjective.evals, level.iters, level.start_iters
         stats["coarse_iters"] = level.iters
+        stats["coarse_spills"] = getattr(cobjective, "bucket_spills", None)
         stats["cluster_ids"] = coarse.cid
         del coarse, cobjective, cps, cpos
     level = _place_level(pos, objective, ps, args, log, first_iter, opts)
@@ -112,7 +113,8 @@ def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=N
             % (level.reason, iters, level.hpwl, level.overflow, gp_time, evals + objective.evals))
     stats.update({"stop_reason": level.reason, "iters": iters, "start_iters": start_iters or level.start_iters,
                   "gp_time_s": gp_time, "per_iter_s": gp_time / max(iters, 1), "grad_calls": evals + objective.evals,
-                  "gp_hpwl": level.hpwl, "gp_overflow": level.overflow})
+                  "gp_hpwl": level.hpwl, "gp_overflow": level.overflow,
+                  "spills": getattr(objective, "bucket_spills", None)})
     return stats
 
 

@@ -38,11 +38,13 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12            # B/s
 PEAK_FP32_MFMA = 157.3e12    # flop/s
+TIME_EVERY = 8               # the roofline kernel's events: in passes 0, 8, 16, ... of the timed run
 
-# kernel classes made of one kernel (the others launch several: binning = count + scan + scatter per grid,
-# Poisson = 5, control = reduce + control)
+# kernel classes whose events in graph runs bracket one kernel (the binning class brackets the force grid's fill, the
+# overflow grid's running on the other stream; the Poisson solve and the control launch several)
 SINGLE_KERNEL = {"nesterov": "eng_nesterov", "wa": "wa_wave_kernel", "wa_cells": "eng_cell_wl",
-                 "density_tiles": "eng_bin_tiles", "overflow_tiles": "eng_over_tiles", "gather": "eng_gather"}
+                 "density_bin": "eng_bin_fill", "density_tiles": "eng_bin_tiles", "overflow_tiles": "eng_over_tiles",
+                 "gather": "eng_gather"}
 
 
 def algorithmic_bytes(cls, dims):
@@ -58,8 +60,9 @@ def algorithmic_bytes(cls, dims):
         "wa": 28 * P + 4 * nets,
         # per pin: its gradient 8 read; per cell: cell_start 4 read, its gradient 8 written
         "wa_cells": 8 * P + 12 * M,
-        # per node and grid: position 8 + size 8 + ratio 4 read (force grid: all nodes; overflow grid: cells)
-        "density_bin": 20 * N + 20 * M,
+        # the force grid's fill: position 8 + size 8 + ratio 4 read per node (the bucket entries it writes and the
+        # tiles read back are intermediates)
+        "density_bin": 20 * N,
         # the force map: fixed 4 read and density 4 written per bin
         "density_tiles": 8 * nb2,
         # the overflow map: fixed 4 read per bin
@@ -118,7 +121,8 @@ def full_run(data, log):
     m.pop("cluster_ids", None)
     torch.cuda.synchronize()
     keys = ("gp_time_s", "iters", "grad_calls", "stop_reason", "hpwl_sites", "overflow", "gp_overflow", "clusters",
-            "coarse_iters", "start_iters", "force_num_bin", "num_bin", "num_fillers", "poisson")
+            "coarse_iters", "start_iters", "force_num_bin", "num_bin", "num_fillers", "poisson", "coarse_spills",
+            "spills")
     out = {k: m[k] for k in keys if k in m}
     out["grad_evals_per_s"] = m["grad_calls"] / m["gp_time_s"]
     return out, snap
@@ -184,6 +188,9 @@ def gpu_passes(snap, steps, warmup, layout=None):
     dom = max(in_graph_ms, key=in_graph_ms.get)
     print("single-kernel classes in graph runs (ms per pass): "
           + ", ".join("%s %.4f" % kv for kv in in_graph_ms.items()), file=sys.stderr, flush=True)
+    # event-record nodes around the roofline kernel in every TIME_EVERY-th pass of the timed graph: an event record
+    # holds its queue for a few microseconds, which the timed passes would otherwise carry in every pass
+    eng.time_every(TIME_EVERY)
     eng.time_class(dom)
     try:
         eng.prepare(steps)                           # graph with event-record nodes around the roofline kernel
@@ -203,12 +210,14 @@ def gpu_passes(snap, steps, warmup, layout=None):
         eng.time_class(dom)
         eng.run(steps, False)
     dom_ms, n = eng.class_ms()
-    assert n == steps
+    assert n == (steps + TIME_EVERY - 1) // TIME_EVERY
     stop = int(S[gpu_loop.K["stop"]].item())
     d = den
     dims = {"nodes": d.num_nodes, "mov": d.num_mov, "pins": int(wl.opt["cell32"].numel()), "nets": wl.num_opt_nets,
             "nb": d.nbx, "nj": d.judge.nbx if d.judge is not None else 0}
     method = d.plan.method
+    print("density entries that took the spill path since the engine's set-up (force grid, overflow grid): %s"
+          % (eng.bucket_spills(),), file=sys.stderr, flush=True)
     eng.close()
     return elapsed, prof, dom, dom_ms, dims, method, stop, in_graph, in_graph_ms
 
@@ -286,7 +295,8 @@ def main():
     by = algorithmic_bytes(dom, dims)
     roof = {"kernel": SINGLE_KERNEL[dom], "bound": "hbm", "achieved": by / (dom_ms * 1e-3) / 1e9,
             "peak": PEAK_HBM / 1e9, "unit": "GB/s", "traffic": None, "algorithmic_bytes": by, "launch_ms": dom_ms,
-            "events": "event-record nodes in the timed graph" if in_graph else "direct launches after the timed region"}
+            "events": ("event-record nodes in the timed graph, every %d-th pass" % TIME_EVERY if in_graph
+                       else "direct launches after the timed region, every %d-th pass" % TIME_EVERY)}
     roof["frac"] = roof["achieved"] / roof["peak"]
     if a.config == "C":
         roof["traffic"] = profile_traffic(SINGLE_KERNEL[dom])

@@ -167,9 +167,14 @@ AP_API int ap_level_prepare(ap_level* level, int passes);
 /* Run `passes` passes with direct launches and HIP events around every kernel class; ms[AP_PROF_CLASSES] receives
  * the average milliseconds per pass of each class. Blocks the calling thread until the passes are done. */
 AP_API int ap_level_profile(ap_level* level, int passes, double* ms, void* stream);
-/* Graphs captured from now on record a HIP event before and after the kernels of class cls in every pass (-1: none).
- * After a graph run, ap_level_class_ms gives the class's average ms per pass over that run (blocks until it ends). */
+/* Graphs captured from now on record a HIP event before and after the kernels of class cls in every timed pass (-1:
+ * none). After a run, ap_level_class_ms gives the class's average ms over the run's timed passes and their number
+ * (blocks until the run ends). */
 AP_API int ap_level_time_class(ap_level* level, int cls);
+/* No counterpart in the reference. The timed passes of a run: 0, every, 2 every, ... (default 1: all). An event
+ * record in a graph holds its queue for a few microseconds, so sparse samples disturb a timed run less. Drops the
+ * level's captured graphs. */
+AP_API int ap_level_time_every(ap_level* level, int every);
 AP_API int ap_level_class_ms(ap_level* level, double* ms, int* passes);
 /* Timing of the last ap_level_run (ms between its first and last kernel), for the caller's instrumentation. */
 AP_API int ap_level_last_ms(ap_level* level, float* ms);
@@ -193,6 +198,15 @@ AP_API int ap_level_autotune(ap_level* level, int reps, int* layout);
  * captured graphs. */
 AP_API int ap_level_set_tile_layout(ap_level* level, int force_side, int force_group, int overflow_side,
                                     int overflow_group);
+/* No counterpart in the reference. The room the density binning gives each tile's bucket in a pass: percent % of the
+ * entries the bucket had in the pass before, plus extra (defaults 150 and 64; percent 0..150, extra 0..64). Entries
+ * past a bucket's room take the slower spill path (global integer atomics), so a smaller room changes the time, never
+ * the density map; a room of 0 % + 0 sends every entry down that path, which is what it is for (tests). Drops the
+ * level's captured graphs. */
+AP_API int ap_level_set_bucket_room(ap_level* level, int percent, int extra);
+/* No counterpart in the reference. Entries that took the spill path since the level was created, u64[2]: force grid,
+ * overflow grid (0 without one). Synchronizes the device. */
+AP_API int ap_level_bucket_spills(ap_level* level, unsigned long long* spills);
 
 /* ---------------------------------------------------------------- batch: many independent levels per launch
  * No counterpart in the reference. count levels (blocks) described as for ap_level_create (each with its own

@@ -122,9 +122,22 @@ def test_engine_density_bit_exact(small_design, case):
         assert np.array_equal(r, rho_dev), (fl, jl)
         assert rel_err(o, ovf_dev) < 1e-12, (fl, jl, o, ovf_dev)   # FP64 sums of per-tile partials
         tried.append((fl, jl))
-    eng.close()
     print("tile layouts with the same map and overflow: %s" % tried)
     assert len(tried) >= 6
+    # rooms cut short (0 %: every entry; 50 %: about half of them) send entries down the spill path: the same map
+    spills = []
+    for room in ((0, 0), (50, 0), (100, 0), (150, 64)):
+        eng.set_bucket_room(*room)
+        for _ in range(2):                           # rooms from the counts of the evaluation before
+            before = eng.bucket_spills()
+            r, o = evaluate()
+            assert np.array_equal(r, rho_dev), room
+            assert rel_err(o, ovf_dev) < 1e-12, (room, o, ovf_dev)
+            spills.append(sum(eng.bucket_spills()) - sum(before))
+    print("spilled entries per evaluation at rooms 0 %%, 50 %%, 100 %%, default: %s" % spills)
+    assert min(spills[:4]) > 0                       # 0 % and 50 %
+    assert max(spills[4:]) == 0                      # rooms of at least the counts at the same positions
+    eng.close()
     smooth, ratio = den.smooth["size"].cpu().numpy(), den.smooth["ratio"].cpu().numpy()
     bx, by = float(den.bin_np[0]), float(den.bin_np[1])
     m0, m1 = oden.density_maps(pos, smooth, ratio, data.num_mov, bx, by, den.nbx, den.nby)
@@ -150,7 +163,8 @@ def test_engine_density_bit_exact(small_design, case):
     assert rel_err(ovf_dev, ovf) < 1e-12
 
 
-def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cls=None, profile=False, grids=None):
+def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cls=None, profile=False, grids=None,
+         room=None, spills=None, every=1):
     args, prob, wl, den, pos, S, kp, c = _setup(data, grids=grids)
     g0 = np.zeros_like(pos)
     eng, bufs = _engine(wl, den, pos, S, kp, c, g0, reorder=False, poisson=poisson)
@@ -161,7 +175,10 @@ def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cl
     eng.close()
     g0 = grad.cpu().numpy()
     eng, bufs = _engine(wl, den, pos, S, kp, c, g0, reorder=reorder, poisson=poisson)
+    if room:
+        eng.set_bucket_room(*room)
     if time_cls:
+        eng.time_every(every)
         eng.time_class(time_cls)
     for _ in range(calls):
         if profile:
@@ -172,6 +189,8 @@ def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cl
     eng.finish()
     torch.cuda.synchronize()
     out = [b.cpu().numpy().copy() for b in bufs]
+    if spills is not None:
+        spills.append(eng.bucket_spills())
     eng.close()
     return out, (args, prob, pos, S, kp, c, grad.cpu().numpy(), timing)
 
@@ -209,6 +228,24 @@ def test_engine_two_streams_equal_one_stream(small_design, grids):
             assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("room", [(0, 0), (50, 0)])
+def test_engine_spilled_entries_change_nothing(small_design, room):
+    """Passes whose density buckets are given too little room (every entry, or about half of them, past it: the
+    spill path) end bit for bit where passes with the default rooms end, graph and direct."""
+    _, _, data = small_design
+    spills = []
+    ref, _ = _run(data, 16, True, grids=(32, 64), spills=spills)
+    for use_graph in (True, False):
+        out, _ = _run(data, 16, use_graph, grids=(32, 64), room=room, spills=spills)
+        for x, y in zip(out, ref):
+            assert np.array_equal(x, y)
+    print("spilled entries (force, overflow) at the default rooms, then at %s %% + %d: %s" % (room + (spills,)))
+    # the default rooms are set from counts at the current point (the set-up count included): small moves do not
+    # outgrow them
+    assert spills[0] == (0, 0)
+    assert min(spills[1]) > 0 and min(spills[2]) > 0
+
+
 def test_engine_timing_events_change_nothing(small_design):
     """Event-record nodes around a kernel class (captured graph) and event records (direct launches) leave the results
     bit for bit as they are, and time the class."""
@@ -222,6 +259,11 @@ def test_engine_timing_events_change_nothing(small_design):
             assert n == 16 and 0 < ms < 50
             for x, y in zip(out, ref):
                 assert np.array_equal(x, y)
+        # events in every 5th pass only: passes 0, 5, 10, 15 timed
+        out, (_, _, _, _, _, _, _, (ms, n)) = _run(data, 16, use_graph, time_cls="density_tiles", every=5)
+        assert n == 4 and 0 < ms < 50
+        for x, y in zip(out, ref):
+            assert np.array_equal(x, y)
 
 
 @pytest.mark.parametrize("poisson", ["gemm", "fft"])
