@@ -6,7 +6,7 @@
 //                                     C = DCT-II along x; Ax = wx C, Ay = wy C (weights computed in place);
 //                                     U^T = Sx^T Ax, V^T = Cx^T Ay  (sine and cosine series of one column from
 //                                     one complex inverse FFT: both outputs are real, so they share it)
-//   pass C  columns of U^T, V^T (rows of U, V, length M):
+//   pass C  columns of U^T, V^T (rows of U, V, length M), read in place by pairs:
 //                                     field_x = U Cy, field_y = V Sy  (one inverse FFT per row)
 // Passes B and C run consecutive workgroups on one XCD, so the lines the strided column reads share meet in one L2
 // (this replaced two transposes through LDS tiles). The batch's grouped launches keep the transposes.
@@ -31,27 +31,41 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ int bitrev(int n, int lg) { return (int)(__brev((unsigned)n) >> (32 - lg)); }
-// LDS place of FFT element i: one pad slot per 8 elements, which breaks the power-of-two strides of the butterflies'
-// accesses across the LDS banks (their conflict degree about halves); an FFT of L elements takes lds_len(L) slots
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
-__host__ __device__ __forceinline__ int lds_len(int L) { return L + (L >> 3); }
+// LDS place of FFT element i: bits 0-3 flipped by bits 3-6 (a bijection inside every 128-element block) and one pad
+// slot per block. The butterflies' accesses -- runs of 1, 4, 16, 64, ... consecutive elements 4 runs apart -- then
+// spread over the banks: by a model of 8-byte accesses on 32 or 64 four-byte banks, about half the cycles of one
+// pad slot per 8 elements, whose slot breaks every 64-byte run. An FFT of L elements takes lds_len(L) slots.
+__device__ __forceinline__ int lpad(int i) { return (i ^ ((i >> 3) & 15)) + (i >> 7); }
+__host__ __device__ __forceinline__ int lds_len(int L) { return L + (L >> 7) + 1; }
+// The twiddles in LDS: tw[j] (j < L / 2, the last radix-2 stage's) at lds_len(L), then, 16-byte aligned at
+// tws_off(L), the radix-4 stage pairs' (w1, w2)(j) for j < h as one float4 each, pair by pair (h = 1, 4, 16, ...):
+// a butterfly reads its two twiddles in one access, and consecutive lanes read consecutive entries, where the
+// table's stride-L / 2h reads hit one bank.
+__host__ __device__ __forceinline__ int stage_len(int lg) {
+    int n = 0;
+    for (int st = 1; st + 1 <= lg; st += 2) n += 1 << (st - 1);
+    return n;
+}
+__host__ __device__ __forceinline__ int tws_off(int L) { return (lds_len(L) + (L >> 1) + 1) & ~1; }
+// float2 slots of a workgroup's LDS for an FFT of L = 2^lg elements
+__host__ __device__ __forceinline__ int fft_slots(int L, int lg) { return tws_off(L) + 2 * stage_len(lg); }
 // position of input element m in Makhoul's order: even m -> m / 2, odd m -> L - 1 - (m - 1) / 2
 __device__ __forceinline__ int makhoul(int m, int L) { return (m & 1) ? L - 1 - (m >> 1) : (m >> 1); }
 
-// in-place decimation in time on bit-reversed input; tw[j] = e^{-2 pi i j / L} (j < L / 2) in LDS; inverse:
-// conjugate twiddles (no 1 / L scale). Two radix-2 stages (half-sizes h and 2h) are done together as one radix-4
-// butterfly in registers on x[i], x[i+h], x[i+2h], x[i+3h]: one barrier and one LDS round trip per stage pair; an odd
-// stage count ends with one radix-2 stage.
-__device__ void fft_lds(float2* s, const float2* tw, int L, int lg, bool inverse) {
+// in-place decimation in time on bit-reversed input; tw[j] = e^{-2 pi i j / L} (j < L / 2) in LDS, tws the stage
+// pairs' table (load_twiddles); inverse: conjugate twiddles (no 1 / L scale). Two radix-2 stages (half-sizes h and 2h)
+// are done together as one radix-4 butterfly in registers on x[i], x[i+h], x[i+2h], x[i+3h]: one barrier and one LDS
+// round trip per stage pair; an odd stage count ends with one radix-2 stage.
+__device__ void fft_lds(float2* s, const float2* tw, const float4* tws, int L, int lg, bool inverse) {
     const float sg = inverse ? -1.f : 1.f;
     int st = 1;
-    for (; st + 1 <= lg; st += 2) {
+    for (int base = 0; st + 1 <= lg; base += 1 << (st - 1), st += 2) {
         const int h = 1 << (st - 1);
-        const int s1 = L >> st, s2 = L >> (st + 1);           // twiddle strides of the two stages
         for (int b = threadIdx.x; b < (L >> 2); b += blockDim.x) {
             const int j = b & (h - 1);
             const int i0 = ((b >> (st - 1)) << (st + 1)) + j;
-            float2 w1 = tw[j * s1], w2 = tw[j * s2];
+            const float4 w = tws[base + j];                     // tw[j L / 2h], tw[j L / 4h]
+            float2 w1 = make_float2(w.x, w.y), w2 = make_float2(w.z, w.w);
             w1.y *= sg;
             w2.y *= sg;
             // W_{4h}^{j+h} = W_{4h}^j * W_4^1, and W_4^1 = -i (forward) or +i (inverse)
@@ -83,8 +97,16 @@ __device__ void fft_lds(float2* s, const float2* tw, int L, int lg, bool inverse
     }
 }
 
-__device__ __forceinline__ void load_twiddles(float2* tw_s, const float2* tw, int L) {
+// tw_s[j] = tw[j] (j < L / 2) and the stage pairs' table tws from the twiddles in global memory
+__device__ __forceinline__ void load_twiddles(float2* tw_s, float4* tws, const float2* tw, int L, int lg) {
     for (int j = threadIdx.x; j < (L >> 1); j += blockDim.x) tw_s[j] = tw[j];
+    for (int st = 1, base = 0; st + 1 <= lg; base += 1 << (st - 1), st += 2) {
+        const int h = 1 << (st - 1), s1 = L >> st, s2 = L >> (st + 1);
+        for (int j = threadIdx.x; j < h; j += blockDim.x) {
+            const float2 a = tw[j * s1], b = tw[j * s2];
+            tws[base + j] = make_float4(a.x, a.y, b.x, b.y);
+        }
+    }
 }
 
 // DCT-II values X0[k], X1[k] of the row pair packed in one forward FFT Z (Zr = Z[(L - k) mod L], p = ph[k] =
@@ -97,12 +119,12 @@ __device__ __forceinline__ void dct_pair_at(float2 Z, float2 Zr, float2 p, float
 }
 
 // DCT-II of the rows r0 = 2 blockIdx.x, r0 + 1 (real, length L) -> out rows; spectrum of the pair from one FFT
-__device__ void dct2_pair_to_lds(const float* in0, const float* in1, float2* s, const float2* tw_s, const float2* ph,
-                                 int L, int lg, float* X0, float* X1) {
+__device__ void dct2_pair_to_lds(const float* in0, const float* in1, float2* s, const float2* tw_s, const float4* tws,
+                                 const float2* ph, int L, int lg, float* X0, float* X1) {
     for (int m = threadIdx.x; m < L; m += blockDim.x)
         s[lpad(bitrev(makhoul(m, L), lg))] = make_float2(in0[m], in1[m]);
     __syncthreads();
-    fft_lds(s, tw_s, L, lg, false);
+    fft_lds(s, tw_s, tws, L, lg, false);
     for (int k = threadIdx.x; k < L; k += blockDim.x)
         dct_pair_at(s[lpad(k)], s[lpad((L - k) & (L - 1))], ph[k], X0[k], X1[k]);
 }
@@ -112,10 +134,11 @@ __device__ __forceinline__ void pass_a_body(const float* rho, int ld_in, float* 
     extern __shared__ float2 lds[];
     float2* s = lds;
     float2* tw_s = lds + lds_len(L);
-    load_twiddles(tw_s, tw, L);
+    float4* tws = (float4*)(lds + tws_off(L));
+    load_twiddles(tw_s, tws, tw, L, lg);
     const int r0 = 2 * wg;
     __syncthreads();
-    dct2_pair_to_lds(rho + (long long)r0 * ld_in, rho + (long long)(r0 + 1) * ld_in, s, tw_s, ph, L, lg,
+    dct2_pair_to_lds(rho + (long long)r0 * ld_in, rho + (long long)(r0 + 1) * ld_in, s, tw_s, tws, ph, L, lg,
                      T + (long long)r0 * ld_out, T + (long long)(r0 + 1) * ld_out);
 }
 
@@ -192,7 +215,8 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
     const int L = a.N, lg = a.lgN, half = L >> 1;
     float2* s = lds;
     float2* tw_s = lds + lds_len(L);
-    load_twiddles(tw_s, a.tw, L);
+    float4* tws = (float4*)(lds + tws_off(L));
+    load_twiddles(tw_s, tws, a.tw, L, lg);
     const int l0 = 2 * wg;
     if (a.col) {
         // columns l0, l0 + 1 of T: one 8-byte load per row
@@ -205,7 +229,7 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
             s[lpad(bitrev(makhoul(m, L), lg))] = make_float2(in0[m], in1[m]);
     }
     __syncthreads();
-    fft_lds(s, tw_s, L, lg, false);
+    fft_lds(s, tw_s, tws, L, lg, false);
     // DCT-II of both rows at this lane's units: c[u][row][0 at k, 1 at L - k]
     float c[FFT_UNITS][2][2];
 #pragma unroll
@@ -232,7 +256,7 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
             cs_put_unit(s, a.ph, L, lg, k, ay, ayr, ax, axr);   // cosine series of Ay, sine series of Ax
         }
         __syncthreads();
-        fft_lds(s, tw_s, L, lg, true);
+        fft_lds(s, tw_s, tws, L, lg, true);
         cs_store(s, L, a.Vt + (long long)l * L, a.Ut + (long long)l * L);
         __syncthreads();
     }
@@ -243,15 +267,15 @@ __global__ __launch_bounds__(FB) void fft_pass_b(PassB a) {
     pass_b_body(a, a.col ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x);
 }
 
-// col > 0: U and V are pass B's outputs Ut, Vt [L][col] and row n is their column n (no transpose)
+// row n of U, V (the batch's grouped launches, after their transpose): one inverse FFT
 __device__ __forceinline__ void pass_c_body(const float* U, const float* V, float* fx, float* fy, float2* f2,
-                                            int ld_out, const float2* tw, const float2* ph, int L, int lg, int n,
-                                            int col) {
+                                            int ld_out, const float2* tw, const float2* ph, int L, int lg, int n) {
     extern __shared__ float2 lds[];
     const int half = L >> 1;
     float2* s = lds;
     float2* tw_s = lds + lds_len(L);
-    load_twiddles(tw_s, tw, L);
+    float4* tws = (float4*)(lds + tws_off(L));
+    load_twiddles(tw_s, tws, tw, L, lg);
     const float* u_row = U + (long long)n * L;
     const float* v_row = V + (long long)n * L;
 #pragma unroll
@@ -259,26 +283,68 @@ __device__ __forceinline__ void pass_c_body(const float* U, const float* V, floa
         const int k = threadIdx.x + u * FB;
         if (k > half) continue;
         const bool two = k && 2 * k != L;
-        if (col) {
-            const long long a0 = (long long)k * col + n, a1 = (long long)(L - k) * col + n;
-            cs_put_unit(s, ph, L, lg, k, U[a0], two ? U[a1] : 0.f, V[a0], two ? V[a1] : 0.f);
-        } else {
-            cs_put_unit(s, ph, L, lg, k, u_row[k], two ? u_row[L - k] : 0.f, v_row[k], two ? v_row[L - k] : 0.f);
-        }
+        cs_put_unit(s, ph, L, lg, k, u_row[k], two ? u_row[L - k] : 0.f, v_row[k], two ? v_row[L - k] : 0.f);
     }
     __syncthreads();
-    fft_lds(s, tw_s, L, lg, true);
+    fft_lds(s, tw_s, tws, L, lg, true);
     if (f2)
         cs_store(s, L, nullptr, nullptr, f2 + (long long)n * ld_out);
     else
         cs_store(s, L, fx + (long long)n * ld_out, fy + (long long)n * ld_out);
 }
 
-// U, V: pass B's outputs Ut, Vt [L][col], each workgroup one column
+// Rows n0 = 2 wg, n0 + 1 of U, V read in place as columns n0, n0 + 1 of pass B's outputs Ut, Vt [L][col] (no
+// transpose): one 8-byte load per element pair, kept in registers; then one inverse FFT per row, as pass B does its two
+__device__ __forceinline__ void pass_c_pair_body(const float* U, const float* V, float* fx, float* fy, float2* f2,
+                                                 int ld_out, const float2* tw, const float2* ph, int L, int lg, int wg,
+                                                 int col) {
+    extern __shared__ float2 lds[];
+    const int half = L >> 1;
+    float2* s = lds;
+    float2* tw_s = lds + lds_len(L);
+    float4* tws = (float4*)(lds + tws_off(L));
+    load_twiddles(tw_s, tws, tw, L, lg);
+    const int n0 = 2 * wg;
+    float2 cu[FFT_UNITS][2], cv[FFT_UNITS][2];   // rows (n0, n0 + 1) of U and V at k [0] and at L - k [1]
+#pragma unroll
+    for (int u = 0; u < FFT_UNITS; u++) {
+        const int k = threadIdx.x + u * FB;
+        cu[u][0] = cu[u][1] = cv[u][0] = cv[u][1] = make_float2(0.f, 0.f);
+        if (k > half) continue;
+        const long long a0 = (long long)k * col + n0;
+        cu[u][0] = *(const float2*)(U + a0);
+        cv[u][0] = *(const float2*)(V + a0);
+        if (k && 2 * k != L) {
+            const long long a1 = (long long)(L - k) * col + n0;
+            cu[u][1] = *(const float2*)(U + a1);
+            cv[u][1] = *(const float2*)(V + a1);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {   // unrolled: the pairs stay in registers
+#pragma unroll
+        for (int u = 0; u < FFT_UNITS; u++) {
+            const int k = threadIdx.x + u * FB;
+            if (k > half) continue;
+            cs_put_unit(s, ph, L, lg, k, q ? cu[u][0].y : cu[u][0].x, q ? cu[u][1].y : cu[u][1].x,
+                        q ? cv[u][0].y : cv[u][0].x, q ? cv[u][1].y : cv[u][1].x);
+        }
+        __syncthreads();
+        fft_lds(s, tw_s, tws, L, lg, true);
+        const int n = n0 + q;
+        if (f2)
+            cs_store(s, L, nullptr, nullptr, f2 + (long long)n * ld_out);
+        else
+            cs_store(s, L, fx + (long long)n * ld_out, fy + (long long)n * ld_out);
+        __syncthreads();
+    }
+}
+
+// U, V: pass B's outputs Ut, Vt [L][col], each workgroup two columns
 __global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V, float* fx, float* fy, float2* f2,
                                                  int ld_out, const float2* tw, const float2* ph, int L, int lg,
                                                  int col) {
-    pass_c_body(U, V, fx, fy, f2, ld_out, tw, ph, L, lg, xcd_remap(blockIdx.x, gridDim.x), col);
+    pass_c_pair_body(U, V, fx, fy, f2, ld_out, tw, ph, L, lg, xcd_remap(blockIdx.x, gridDim.x), col);
 }
 
 // out[c][r] = in[r][c] for 2 planes (blockIdx.z), 32 x 32 tiles through LDS
@@ -339,10 +405,25 @@ __global__ __launch_bounds__(FB) void fft_c_grouped(const FftTab* tab, const int
     const int4 m = map[blockIdx.x];
     const FftTab t = tab[m.x];
     if (fft_skip(t)) return;
-    pass_c_body(t.w0, t.w1, t.fx, t.fy, t.f2, t.fld, t.twM, t.phM, t.M, t.lgM, m.y, 0);
+    pass_c_body(t.w0, t.w1, t.fx, t.fy, t.f2, t.fld, t.twM, t.phM, t.M, t.lgM, m.y);
+}
+
+// The FFT kernels' dynamic LDS limit: the largest side's need (above the 64 KB default from side 4096 on), set once
+static int fft_lds_limit() {
+    static bool done = false;
+    if (done) return AP_OK;
+    int lg = 0;
+    while ((1 << lg) < Poisson::fft_max) lg++;
+    const int bytes = fft_slots(Poisson::fft_max, lg) * (int)sizeof(float2);
+    const void* fn[6] = {(const void*)fft_pass_a, (const void*)fft_pass_b, (const void*)fft_pass_c,
+                         (const void*)fft_a_grouped, (const void*)fft_b_grouped, (const void*)fft_c_grouped};
+    for (const void* f : fn) AP_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    done = true;
+    return AP_OK;
 }
 
 int FftGroup::create(const std::vector<FftTab>& tabs) {
+    if (int st = fft_lds_limit()) return st;
     std::vector<int4> m[5];
     for (int g = 0; g < (int)tabs.size(); g++) {
         const FftTab& t = tabs[g];
@@ -356,8 +437,8 @@ int FftGroup::create(const std::vector<FftTab>& tabs) {
             for (int y = 0; y < (int)grid_of(t.M, TT); y++)
                 for (int x = 0; x < (int)grid_of(t.N, TT); x++) m[3].push_back(make_int4(g, x, y, z));
         for (int y = 0; y < t.N; y++) m[4].push_back(make_int4(g, y, 0, 0));
-        lds_ac = std::max(lds_ac, (size_t)(lds_len(t.M) + t.M / 2) * sizeof(float2));
-        lds_b = std::max(lds_b, (size_t)(lds_len(t.N) + t.N / 2) * sizeof(float2));
+        lds_ac = std::max(lds_ac, (size_t)fft_slots(t.M, t.lgM) * sizeof(float2));
+        lds_b = std::max(lds_b, (size_t)fft_slots(t.N, t.lgN) * sizeof(float2));
     }
     for (int k = 0; k < 5; k++) {
         count[k] = (int)m[k].size();
@@ -437,6 +518,7 @@ static int ilog2(int n) {
 }
 
 int Poisson::fft_create() {
+    if (int st = fft_lds_limit()) return st;
     // per side: twiddles e^{-2 pi i j / L} (L / 2) then phases e^{i pi k / 2L} (L); omega / nu tables
     std::vector<float2> h;
     auto add = [&](int L) {
@@ -474,8 +556,8 @@ int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, float
     const float2* phM = twM + M / 2;
     float *w0 = d_work, *w1 = d_work + nm, *w2 = d_work + 2 * nm, *w3 = d_work + 3 * nm;
     // pass A: T = rho Cy^T (rows of length M), into w0
-    const size_t lds_m = (size_t)(lds_len(M) + M / 2) * sizeof(float2);
-    const size_t lds_n = (size_t)(lds_len(N) + N / 2) * sizeof(float2);
+    const size_t lds_m = (size_t)fft_slots(M, lgM) * sizeof(float2);
+    const size_t lds_n = (size_t)fft_slots(N, lgN) * sizeof(float2);
     hipLaunchKernelGGL(fft_pass_a, dim3(N / 2), dim3(FB), lds_m, s, rho, rho_ld, w0, M, twM, phM, M, lgM);
     AP_LAUNCHED();
     // pass B: columns of T (length N), read in place by pairs -> Ut (w2), Vt (w3); no transpose
@@ -483,7 +565,7 @@ int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, float
     hipLaunchKernelGGL(fft_pass_b, dim3(M / 2), dim3(FB), lds_n, s, b);
     AP_LAUNCHED();
     // pass C: columns of Ut, Vt (rows of U, V) -> field; no transpose
-    hipLaunchKernelGGL(fft_pass_c, dim3(N), dim3(FB), lds_m, s, w2, w3, fx, fy, f2, fld, twM, phM, M, lgM, N);
+    hipLaunchKernelGGL(fft_pass_c, dim3(N / 2), dim3(FB), lds_m, s, w2, w3, fx, fy, f2, fld, twM, phM, M, lgM, N);
     AP_LAUNCHED();
     return AP_OK;
 }
