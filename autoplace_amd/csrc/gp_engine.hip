@@ -528,19 +528,23 @@ __global__ __launch_bounds__(EB) void eng_over_fill(Pick pos_p, BinJob f, BinJob
     bin_fill_body(pos_p, f, j, f.blocks + xcd_remap(blockIdx.x, gridDim.x));
 }
 
-constexpr int TB = 512;     // lanes per workgroup of the tile kernel: a crowded tile's bucket spreads over 8 waves
+// Lanes per workgroup of the tile kernel: the single-level passes' (TB_LEVEL) and the batch's grouped launches' (TB).
+// A crowded tile's bucket spreads over 8 waves.
+constexpr int TB_LEVEL = 512;
+constexpr int TB = 512;
 constexpr int TILE_PF = 4;  // fixed-map values per lane loaded ahead (the whole tile up to T x T = 4 x lanes)
 
-// One workgroup sums J.G tiles (G = 1, 2, 4, 8), each with its own TB / G lanes (whole waves) and its own LDS
-// accumulator, so that small tiles do not each hold a workgroup (and the launch's LDS reservation) to themselves.
-// The sub-groups pass the same barriers; a sub-group past the last tile only takes part in them.
+// One workgroup of NT lanes sums J.G tiles (G = 1, 2, 4, 8), each with its own NT / G lanes (whole waves) and its
+// own LDS accumulator, so that small tiles do not each hold a workgroup (and the launch's LDS reservation) to
+// themselves. The sub-groups pass the same barriers; a sub-group past the last tile only takes part in them.
 // (wg: the workgroup within the grid's launch; part: the grid's overflow partials)
+template <int NT>
 __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& job, float target, double* part,
                                                int wg) {
     extern __shared__ unsigned long long acc_all[];   // G x planes x T x T
-    __shared__ double red[TB / 64];
+    __shared__ double red[NT / 64];
     const BinJob J = job;              // by value: a reference to a kernel argument would go to scratch
-    const int S = TB / J.G;            // lanes per tile
+    const int S = NT / J.G;            // lanes per tile
     const int sub = threadIdx.x / S, l = threadIdx.x - sub * S;
     const int t = wg * J.G + sub;
     const bool live = t < J.tiles;
@@ -671,11 +675,12 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
     }
 }
 // the overflow partials: the force grid's tiles, then the overflow grid's
-__global__ __launch_bounds__(TB, 8) void eng_bin_tiles(Pick pos_p, BinJob f, float target, double* part) {
-    bin_tiles_body(pos_p, f, target, part, blockIdx.x);
+__global__ __launch_bounds__(TB_LEVEL, 8) void eng_bin_tiles(Pick pos_p, BinJob f, float target, double* part) {
+    bin_tiles_body<TB_LEVEL>(pos_p, f, target, part, blockIdx.x);
 }
-__global__ __launch_bounds__(TB, 8) void eng_over_tiles(Pick pos_p, BinJob j, float target, double* part, int off) {
-    bin_tiles_body(pos_p, j, target, part + off, blockIdx.x);
+__global__ __launch_bounds__(TB_LEVEL, 8) void eng_over_tiles(Pick pos_p, BinJob j, float target, double* part,
+                                                              int off) {
+    bin_tiles_body<TB_LEVEL>(pos_p, j, target, part + off, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------ gather (+ cell WA sum, |g' - g|)
@@ -1238,13 +1243,15 @@ bool Level::side_ok(const BinJob& J, int T) const {
     return T >= 8 && T <= 64 && (long long)J.planes * grid_of(J.nbx, T) * grid_of(J.nby, T) <= BIN_MAX_BUCKETS;
 }
 
-// tiles per workgroup of the tile kernel, by default: one (a tile's bucket spread over all 512 lanes; on the 2M-cell
-// design this measured best or within 2 % of the best for every side of both grids)
+// tiles per workgroup of the tile kernel, by default: one (a tile's bucket spread over all the workgroup's lanes; on
+// the 2M-cell design this measured best or within 2 % of the best for every side of both grids)
 static int default_group(int, int) { return 1; }
 
-// a group the tile kernel can use for side T: 1, 2, 4 or 8 tiles per workgroup, within 64 KB of LDS
+// a group the tile kernel can use for side T: 1, 2, 4 or 8 tiles per workgroup (each tile whole waves of the
+// single-level kernel's lanes), within 64 KB of LDS
 static bool group_ok(int planes, int T, int G) {
-    return (G == 1 || G == 2 || G == 4 || G == 8) && (size_t)G * planes * T * T * sizeof(unsigned long long) <= 65536;
+    return (G == 1 || G == 2 || G == 4 || G == 8) && TB_LEVEL / G >= 64 &&
+           (size_t)G * planes * T * T * sizeof(unsigned long long) <= 65536;
 }
 
 void Level::set_side(BinJob& J, int T, int G) {
@@ -1315,9 +1322,10 @@ int Level::bin_grid(int which, const Pick& pos, hipStream_t s) {
 // the tile sums of one grid: density and overflow partials
 int Level::tiles_grid(int which, const Pick& pos, hipStream_t s) {
     if (!which)
-        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.wgs), dim3(TB), acc_f, s, pos, fj, d.target, p_over);
+        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.wgs), dim3(TB_LEVEL), acc_f, s, pos, fj, d.target, p_over);
     else
-        hipLaunchKernelGGL(eng_over_tiles, dim3(jj.wgs), dim3(TB), acc_j, s, pos, jj, d.target, p_over, fj.tiles);
+        hipLaunchKernelGGL(eng_over_tiles, dim3(jj.wgs), dim3(TB_LEVEL), acc_j, s, pos, jj, d.target, p_over,
+                           fj.tiles);
     AP_LAUNCHED();
     return AP_OK;
 }
@@ -1796,7 +1804,7 @@ __global__ __launch_bounds__(TB) void eng_bin_tiles_grouped(const PassTab* tab, 
     if (stopped(tab, m.x)) return;
     const PassTab& T = tab[m.x];
     const bool judge = m.y >= T.fj.wgs;
-    bin_tiles_body(T.pos, judge ? T.jj : T.fj, T.target, T.p_over + (judge ? T.fj.tiles : 0),
+    bin_tiles_body<TB>(T.pos, judge ? T.jj : T.fj, T.target, T.p_over + (judge ? T.fj.tiles : 0),
                    m.y - (judge ? T.fj.wgs : 0));
 }
 __global__ __launch_bounds__(EB) void eng_gather_grouped(const PassTab* tab, const int4* map) {
