@@ -453,12 +453,16 @@ __device__ __forceinline__ void bin_fill_body(const Pick& pos_p, const BinJob& f
     __syncthreads();
     const float2* pos = pos_p.get(true);
     unsigned rk[BIN_PER_LANE][2];
+    float4 nd[BIN_PER_LANE];   // the nodes' centre and size, kept for the second phase (LDS, not registers, bounds the
+                               // force grid's occupancy)
 #pragma unroll
     for (int q = 0; q < BIN_PER_LANE; q++) {
         rk[q][0] = rk[q][1] = 0;
+        nd[q] = make_float4(0.f, 0.f, 0.f, 0.f);
         const int i = base + q * EB + threadIdx.x;
         if (i >= J.num_nodes) continue;
         const float2 p = pos[i], sz = J.size[i];
+        nd[q] = make_float4(p.x, p.y, sz.x, sz.y);
         const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
         if (sx.b0 > sx.b1 || sy.b0 > sy.b1) continue;
         int t0, wx, wy;
@@ -485,13 +489,12 @@ __device__ __forceinline__ void bin_fill_body(const Pick& pos_p, const BinJob& f
     for (int q = 0; q < BIN_PER_LANE; q++) {
         if (!rk[q][0]) continue;                                // no entries: out of range, no bins or a big node
         const int i = base + q * EB + threadIdx.x;
-        const float2 p = pos[i], sz = J.size[i];
+        const float4 g = nd[q];
         const float r = J.ratio[i];
-        const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
+        const Span sx = make_span(g.x, g.z, J.inv_bx, J.nbx), sy = make_span(g.y, g.w, J.inv_by, J.nby);
         int t0, wx, wy;
         small_node(sx, sy, J.T, J.tiles_y, t0, wx, wy);
         const int b = t0 + (i >= J.num_mov ? J.tiles : 0);
-        const float4 g = make_float4(p.x, p.y, sz.x, sz.y);
         const unsigned rr[4] = {rk[q][0] & 0xFFFFu, rk[q][0] >> 16, rk[q][1] & 0xFFFFu, rk[q][1] >> 16};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -747,6 +750,14 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
     if (i < a.num_nodes) {
         int cur = a.flags ? a.flags[0] : 0;
         const float2* pos = a.pos.get(true);
+        // issued with the node's position, before the field's loads wait: the rest of what the node's gradient needs
+        // (behind the uniform conditions below the compiler would not issue them early)
+        const float pins = a.pins[i];
+        const float area = a.use_density ? a.area[i] : 0.f;
+        float2 w_cell = make_float2(0.f, 0.f);
+        if (a.cell_wl) w_cell = a.cell_wl[min(i, a.num_mov)];   // num_mov + 1 entries
+        float2 gp = make_float2(0.f, 0.f);
+        if (a.g_prev) gp = ((a.flags && cur == 1) ? a.g_prev_b : a.g_prev)[i];
         float gx = 0, gy = 0;
         if (a.use_density) {
             const float weight = a.kp[1];
@@ -798,7 +809,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
             t2[2] = (double)gx * gx + (double)gy * gy;
         }
         if (i < a.num_mov) {
-            const float2 w = a.cell_wl ? a.cell_wl[i] : cell_pin_sum(a, i);
+            const float2 w = a.cell_wl ? w_cell : cell_pin_sum(a, i);
             gx += w.x;
             gy += w.y;
             t2[1] = (double)w.x * w.x + (double)w.y * w.y;
@@ -806,15 +817,14 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
         float scale;
         if (a.use_density) {
             const float precond = a.kp[2];
-            scale = precond < 0 ? 1.f : 1.f / fmaxf(a.pins[i] + precond * a.area[i], 1.f);
+            scale = precond < 0 ? 1.f : 1.f / fmaxf(pins + precond * area, 1.f);
         } else {
-            scale = 1.f / fmaxf(a.pins[i], 1.f);
+            scale = 1.f / fmaxf(pins, 1.f);
         }
         const float2 gn = make_float2(gx * scale, gy * scale);
         float2* out = (a.flags && cur == 0) ? a.g_out_b : a.g_out_a;
         out[i] = gn;
         if (a.g_prev) {
-            const float2 gp = (a.flags && cur == 1) ? a.g_prev_b[i] : a.g_prev[i];
             const double dx = (double)gn.x - gp.x, dy = (double)gn.y - gp.y;
             t2[0] = dx * dx + dy * dy;
         }
