@@ -1104,6 +1104,10 @@ struct Level {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;                             // WA and the overflow grid, beside the force grid
     hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr, ev_fork = nullptr, ev_join = nullptr;
+    // the fork / join events of graph captures, kept apart from the ones direct launches record: an event is used
+    // either inside captures or outside them, never both
+    hipEvent_t cap_fork = nullptr, cap_join = nullptr;
+    bool capturing = false;
     GraphEntry graphs[GRAPH_CACHE];
     int graph_next = 0;
     // event timing of one kernel class: events before and after it in every pass of a run (graph or direct)
@@ -1340,55 +1344,37 @@ int Level::tiles_grid(int which, const Pick& pos, hipStream_t s) {
     return AP_OK;
 }
 
-// The front of a pass as the passes run it, captured into a graph and replayed `reps` times: ms per replay. The
-// rooms, then the force grid's fill, tiles and Poisson solve on s and, forked beside them, WA, the cells' WA gradients
-// and the overflow grid's fill and tiles on the side stream -- in the order objective() captures them, so that the
-// replay runs on the hardware queues the passes' graphs run on -- at the caller's current point v (between runs the
-// current point is always set 0). It changes nothing a pass reads before writing it again (the bucket counts are the
-// last evaluation's, which set the next rooms; pin and cell gradients, partials, rho and the field are rewritten by
-// every pass).
+// The front of a pass as the passes run it, `reps` times: ms per run. The rooms, then the force grid's fill, tiles
+// and Poisson solve on s and, forked beside them, WA, the cells' WA gradients and the overflow grid's fill and tiles
+// on the side stream, launched in the order objective() captures them, at the caller's current point v (between runs
+// the current point is always set 0). It changes nothing a pass reads before writing it again (the bucket counts are
+// the last evaluation's, which set the next rooms; pin and cell gradients, partials, rho and the field are rewritten
+// by every pass).
 int Level::time_front(int reps, hipStream_t s, float& ms) {
     const Pick pos{(const float2*)d.v, nullptr, nullptr};
     int st;
     if ((st = set_lds()) || (st = init_rooms(pos, s))) return st;
-    AP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    auto front = [&]() -> int {
+    AP_HIP(hipEventRecord(t0, s));
+    for (int r = 0; r < reps; r++) {
         hipLaunchKernelGGL(eng_rooms, dim3(jj.count ? 2 : 1), dim3(EB), 0, s, fj, jj);
         AP_LAUNCHED();
         AP_HIP(hipEventRecord(ev_fork, s));
         AP_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-        int e;
-        if ((e = bin_grid(0, pos, s))) return e;
-        if ((e = wa_plan.launch(wa_args(pos, d.kp), p_wa, side))) return e;
+        if ((st = bin_grid(0, pos, s))) return st;
+        if ((st = wa_plan.launch(wa_args(pos, d.kp), p_wa, side))) return st;
         hipLaunchKernelGGL(eng_cell_wl, dim3(nblk_n), dim3(EB), 0, side,
                            gather_args(pos, d.kp, nullptr, nullptr, nullptr, nullptr, 1), cell_wl, p_wl2);
         AP_LAUNCHED();
-        if (jj.count && (e = bin_grid(1, pos, side))) return e;
-        if ((e = tiles_grid(0, pos, s))) return e;
-        if (jj.count && (e = tiles_grid(1, pos, side))) return e;
-        if ((e = solve_field(s))) return e;
+        if (jj.count && (st = bin_grid(1, pos, side))) return st;
+        if ((st = tiles_grid(0, pos, s))) return st;
+        if (jj.count && (st = tiles_grid(1, pos, side))) return st;
+        if ((st = solve_field(s))) return st;
         AP_HIP(hipEventRecord(ev_join, side));
         AP_HIP(hipStreamWaitEvent(s, ev_join, 0));
-        return AP_OK;
-    };
-    st = front();
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(s, &g);
-    hipGraphExec_t x = nullptr;
-    hipError_t e = st ? hipSuccess : ec;
-    if (!st && e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    if (!st && e == hipSuccess) e = hipEventRecord(t0, s);
-    for (int r = 0; r < reps && !st && e == hipSuccess; r++) e = hipGraphLaunch(x, s);
-    if (!st && e == hipSuccess) e = hipEventRecord(t1, s);
-    if (!st && e == hipSuccess) e = hipEventSynchronize(t1);
-    if (!st && e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
-    if (x) (void)hipGraphExecDestroy(x);
-    if (g) (void)hipGraphDestroy(g);
-    if (st) {
-        (void)hipGetLastError();
-        return st;
     }
-    AP_HIP(e);
+    AP_HIP(hipEventRecord(t1, s));
+    AP_HIP(hipEventSynchronize(t1));
+    AP_HIP(hipEventElapsedTime(&ms, t0, t1));
     ms /= reps;
     return AP_OK;
 }
@@ -1525,6 +1511,8 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
     AP_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     AP_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    AP_HIP(hipEventCreateWithFlags(&cap_fork, hipEventDisableTiming));
+    AP_HIP(hipEventCreateWithFlags(&cap_join, hipEventDisableTiming));
     AP_HIP(hipEventCreate(&t0));
     AP_HIP(hipEventCreate(&t1));
     AP_HIP(hipDeviceSynchronize());
@@ -1548,7 +1536,7 @@ void Level::destroy() {
     wa_plan.destroy();
     if (stream) (void)hipStreamDestroy(stream);
     if (side) (void)hipStreamDestroy(side);
-    hipEvent_t evs[] = {ev_in, ev_out, t0, t1, ev_fork, ev_join};
+    hipEvent_t evs[] = {ev_in, ev_out, t0, t1, ev_fork, ev_join, cap_fork, cap_join};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
 }
@@ -1624,8 +1612,8 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     }
     if (two) {
         w = side;
-        AP_HIP(hipEventRecord(ev_fork, s));
-        AP_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+        AP_HIP(hipEventRecord(capturing ? cap_fork : ev_fork, s));
+        AP_HIP(hipStreamWaitEvent(side, capturing ? cap_fork : ev_fork, 0));
     }
     // the side stream: WA and the cells' WA sums first, then the overflow grid (measured 4 % faster per pass than the
     // other order on the 2M-cell design: WA's streaming overlaps the force grid's binning, and the overflow grid's
@@ -1668,8 +1656,8 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
         tend(AP_PROF_POISSON, s);
     }
     if (two) {
-        AP_HIP(hipEventRecord(ev_join, side));
-        AP_HIP(hipStreamWaitEvent(s, ev_join, 0));
+        AP_HIP(hipEventRecord(capturing ? cap_join : ev_join, side));
+        AP_HIP(hipStreamWaitEvent(s, capturing ? cap_join : ev_join, 0));
     }
     mark(AP_PROF_GATHER, s);
     tbegin(AP_PROF_GATHER, s);
@@ -1729,6 +1717,7 @@ int Level::graph_for(int passes, int& slot) {
     rec_capture = true;
     rec_err = hipSuccess;
     AP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    capturing = true;
     int st = AP_OK;
     for (int k = 0; k < passes && !st; k++) {
         rec_pass = k;
@@ -1736,6 +1725,7 @@ int Level::graph_for(int passes, int& slot) {
     }
     if (!st) st = finalize(stream);
     rec_ev = nullptr;
+    capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(stream, &g);
     if (st || e != hipSuccess || rec_err != hipSuccess) {

@@ -157,7 +157,7 @@ AP_API int ap_level_prepare(ap_level* level, int passes);
 #define AP_PROF_NESTEROV 0
 #define AP_PROF_WA 1
 #define AP_PROF_WA_CELLS 2    /* the cells' WA gradients (sums of their pins') */
-#define AP_PROF_MAP 3         /* density: tile binning of both grids (count, scan, scatter) */
+#define AP_PROF_MAP 3         /* density: tile binning of both grids (the fill of each grid's buckets) */
 #define AP_PROF_POST 4        /* density: per-tile sums of the force grid, density and overflow */
 #define AP_PROF_OVER_TILES 5  /* density: per-tile sums of the overflow grid (when it has one) */
 #define AP_PROF_POISSON 6
