@@ -37,13 +37,15 @@ __device__ __forceinline__ int bitrev(int n, int lg) { return (int)(__brev((unsi
 // pad slot per 8 elements, whose slot breaks every 64-byte run. An FFT of L elements takes lds_len(L) slots.
 __device__ __forceinline__ int lpad(int i) { return (i ^ ((i >> 3) & 15)) + (i >> 7); }
 __host__ __device__ __forceinline__ int lds_len(int L) { return L + (L >> 7) + 1; }
-// The twiddles in LDS: tw[j] (j < L / 2, the last radix-2 stage's) at lds_len(L), then, 16-byte aligned at
-// tws_off(L), the radix-4 stage pairs' (w1, w2)(j) for j < h as one float4 each, pair by pair (h = 1, 4, 16, ...):
-// a butterfly reads its two twiddles in one access, and consecutive lanes read consecutive entries, where the
-// table's stride-L / 2h reads hit one bank.
+// The twiddles in LDS: tw[j] (j < L / 2, for a last radix-2 stage) at lds_len(L), then, 16-byte aligned at
+// tws_off(L), the per-stage table (float4 entries), stage by stage: a radix-8 stage of half-size h (h = 1, 8, 64,
+// ...) holds (W_2h^j, W_4h^j) for j < h, then (W_8h^j, W_8h^(j+h)) for j < h; a last radix-4 stage (W_2h^j, W_4h^j)
+// for j < h. A butterfly reads its twiddles in one or two 16-byte accesses, and consecutive lanes read consecutive
+// entries, where the plain table's stride-L / 2h reads hit one bank.
 __host__ __device__ __forceinline__ int stage_len(int lg) {
-    int n = 0;
-    for (int st = 1; st + 1 <= lg; st += 2) n += 1 << (st - 1);
+    int n = 0, st = 1;
+    for (; st + 2 <= lg; st += 3) n += 2 << (st - 1);
+    if (st + 1 <= lg) n += 1 << (st - 1);
     return n;
 }
 __host__ __device__ __forceinline__ int tws_off(int L) { return (lds_len(L) + (L >> 1) + 1) & ~1; }
@@ -52,35 +54,71 @@ __host__ __device__ __forceinline__ int fft_slots(int L, int lg) { return tws_of
 // position of input element m in Makhoul's order: even m -> m / 2, odd m -> L - 1 - (m - 1) / 2
 __device__ __forceinline__ int makhoul(int m, int L) { return (m & 1) ? L - 1 - (m >> 1) : (m >> 1); }
 
-// in-place decimation in time on bit-reversed input; tw[j] = e^{-2 pi i j / L} (j < L / 2) in LDS, tws the stage
-// pairs' table (load_twiddles); inverse: conjugate twiddles (no 1 / L scale). Two radix-2 stages (half-sizes h and 2h)
-// are done together as one radix-4 butterfly in registers on x[i], x[i+h], x[i+2h], x[i+3h]: one barrier and one LDS
-// round trip per stage pair; an odd stage count ends with one radix-2 stage.
+// the radix-2 butterfly (x, y) <- (x + w y, x - w y)
+__device__ __forceinline__ void bfly(float2& x, float2& y, float2 w) {
+    const float2 t = cmul(y, w);
+    y = make_float2(x.x - t.x, x.y - t.y);
+    x = make_float2(x.x + t.x, x.y + t.y);
+}
+// w W_4^1: W_4^1 = -i (forward) or +i (inverse, sg = -1), exact
+__device__ __forceinline__ float2 quarter(float2 w, float sg) { return make_float2(sg * w.y, -sg * w.x); }
+
+// In-place decimation in time on bit-reversed input, the twiddles from the stage table tws (and tw for a last radix-2
+// stage, load_twiddles); inverse: conjugate twiddles (no 1 / L scale). Three radix-2 stages (half-sizes h, 2h, 4h) are
+// done together as one radix-8 butterfly in registers on x[i + k h], k < 8: one barrier and one LDS round trip per
+// three stages (an FFT of 2048 points: 3 radix-8 stages and one radix-4, 4 round trips instead of 11); the twiddles at
+// j + h of the second stage and at j + 2h, j + 3h of the third are the ones at j and j + h times W_4^1 (exact).
+// One or two stages left end as one radix-4 or radix-2 stage.
 __device__ void fft_lds(float2* s, const float2* tw, const float4* tws, int L, int lg, bool inverse) {
     const float sg = inverse ? -1.f : 1.f;
-    int st = 1;
-    for (int base = 0; st + 1 <= lg; base += 1 << (st - 1), st += 2) {
+    int st = 1, base = 0;
+    for (; st + 2 <= lg; base += 2 << (st - 1), st += 3) {
+        const int h = 1 << (st - 1);
+        for (int b = threadIdx.x; b < (L >> 3); b += blockDim.x) {
+            const int j = b & (h - 1);
+            const int i0 = ((b >> (st - 1)) << (st + 2)) + j;
+            const float4 ta = tws[base + j], tb = tws[base + h + j];
+            const float2 w1 = make_float2(ta.x, sg * ta.y), w2 = make_float2(ta.z, sg * ta.w);
+            const float2 w4 = make_float2(tb.x, sg * tb.y), w5 = make_float2(tb.z, sg * tb.w);
+            const float2 w3 = quarter(w2, sg), w6 = quarter(w4, sg), w7 = quarter(w5, sg);
+            float2 x[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) x[k] = s[lpad(i0 + k * h)];
+#pragma unroll
+            for (int k = 0; k < 8; k += 2) bfly(x[k], x[k + 1], w1);
+            bfly(x[0], x[2], w2);
+            bfly(x[1], x[3], w3);
+            bfly(x[4], x[6], w2);
+            bfly(x[5], x[7], w3);
+            bfly(x[0], x[4], w4);
+            bfly(x[1], x[5], w5);
+            bfly(x[2], x[6], w6);
+            bfly(x[3], x[7], w7);
+#pragma unroll
+            for (int k = 0; k < 8; k++) s[lpad(i0 + k * h)] = x[k];
+        }
+        __syncthreads();
+    }
+    if (st + 1 <= lg) {
         const int h = 1 << (st - 1);
         for (int b = threadIdx.x; b < (L >> 2); b += blockDim.x) {
             const int j = b & (h - 1);
             const int i0 = ((b >> (st - 1)) << (st + 1)) + j;
-            const float4 w = tws[base + j];                     // tw[j L / 2h], tw[j L / 4h]
-            float2 w1 = make_float2(w.x, w.y), w2 = make_float2(w.z, w.w);
-            w1.y *= sg;
-            w2.y *= sg;
-            // W_{4h}^{j+h} = W_{4h}^j * W_4^1, and W_4^1 = -i (forward) or +i (inverse)
-            const float2 w3 = make_float2(sg * w2.y, -sg * w2.x);
-            const float2 a0 = s[lpad(i0)], a1 = cmul(s[lpad(i0 + h)], w1);
-            const float2 a2 = s[lpad(i0 + 2 * h)], a3 = cmul(s[lpad(i0 + 3 * h)], w1);
-            const float2 b0 = make_float2(a0.x + a1.x, a0.y + a1.y), b1 = make_float2(a0.x - a1.x, a0.y - a1.y);
-            const float2 b2 = cmul(make_float2(a2.x + a3.x, a2.y + a3.y), w2);
-            const float2 b3 = cmul(make_float2(a2.x - a3.x, a2.y - a3.y), w3);
-            s[lpad(i0)] = make_float2(b0.x + b2.x, b0.y + b2.y);
-            s[lpad(i0 + 2 * h)] = make_float2(b0.x - b2.x, b0.y - b2.y);
-            s[lpad(i0 + h)] = make_float2(b1.x + b3.x, b1.y + b3.y);
-            s[lpad(i0 + 3 * h)] = make_float2(b1.x - b3.x, b1.y - b3.y);
+            const float4 t = tws[base + j];
+            const float2 w1 = make_float2(t.x, sg * t.y), w2 = make_float2(t.z, sg * t.w);
+            const float2 w3 = quarter(w2, sg);
+            float2 x[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) x[k] = s[lpad(i0 + k * h)];
+            bfly(x[0], x[1], w1);
+            bfly(x[2], x[3], w1);
+            bfly(x[0], x[2], w2);
+            bfly(x[1], x[3], w3);
+#pragma unroll
+            for (int k = 0; k < 4; k++) s[lpad(i0 + k * h)] = x[k];
         }
         __syncthreads();
+        st += 2;
     }
     if (st == lg) {
         const int half = 1 << (st - 1), stride = L >> st;
@@ -89,24 +127,36 @@ __device__ void fft_lds(float2* s, const float2* tw, const float4* tws, int L, i
             const int i0 = ((b >> (st - 1)) << st) + j, i1 = i0 + half;
             float2 w = tw[j * stride];
             w.y *= sg;
-            const float2 a = s[lpad(i0)], c = cmul(s[lpad(i1)], w);
-            s[lpad(i0)] = make_float2(a.x + c.x, a.y + c.y);
-            s[lpad(i1)] = make_float2(a.x - c.x, a.y - c.y);
+            float2 x0 = s[lpad(i0)], x1 = s[lpad(i1)];
+            bfly(x0, x1, w);
+            s[lpad(i0)] = x0;
+            s[lpad(i1)] = x1;
         }
         __syncthreads();
     }
 }
 
-// tw_s[j] = tw[j] (j < L / 2) and the stage pairs' table tws from the twiddles in global memory
+// the stage table tws (and tw_s[j] = tw[j], j < L / 2, for a last radix-2 stage) from the twiddles in global memory
 __device__ __forceinline__ void load_twiddles(float2* tw_s, float4* tws, const float2* tw, int L, int lg) {
-    for (int j = threadIdx.x; j < (L >> 1); j += blockDim.x) tw_s[j] = tw[j];
-    for (int st = 1, base = 0; st + 1 <= lg; base += 1 << (st - 1), st += 2) {
+    int st = 1, base = 0;
+    for (; st + 2 <= lg; base += 2 << (st - 1), st += 3) {
+        const int h = 1 << (st - 1), s1 = L >> st, s2 = L >> (st + 1), s3 = L >> (st + 2);
+        for (int j = threadIdx.x; j < h; j += blockDim.x) {
+            const float2 a = tw[j * s1], b = tw[j * s2], c = tw[j * s3], d = tw[(j + h) * s3];
+            tws[base + j] = make_float4(a.x, a.y, b.x, b.y);
+            tws[base + h + j] = make_float4(c.x, c.y, d.x, d.y);
+        }
+    }
+    if (st + 1 <= lg) {
         const int h = 1 << (st - 1), s1 = L >> st, s2 = L >> (st + 1);
         for (int j = threadIdx.x; j < h; j += blockDim.x) {
             const float2 a = tw[j * s1], b = tw[j * s2];
             tws[base + j] = make_float4(a.x, a.y, b.x, b.y);
         }
+        st += 2;
     }
+    if (st == lg)
+        for (int j = threadIdx.x; j < (L >> 1); j += blockDim.x) tw_s[j] = tw[j];
 }
 
 // DCT-II values X0[k], X1[k] of the row pair packed in one forward FFT Z (Zr = Z[(L - k) mod L], p = ph[k] =
