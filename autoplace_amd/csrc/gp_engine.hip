@@ -1179,7 +1179,9 @@ struct Level {
     int solve_field(hipStream_t s);
     int bin_grid(int which, const Pick& pos, hipStream_t s);
     int tiles_grid(int which, const Pick& pos, hipStream_t s);
-    int time_front(int reps, hipStream_t s, float& ms);
+    size_t state_bytes() const;
+    int state_copy(char* keep, bool restore, hipStream_t s);
+    int time_passes(int reps, hipStream_t s, float& ms);
     int autotune(int reps, hipStream_t s, int* sides);
     int create(const ap_level_desc& desc);
     void destroy();
@@ -1344,34 +1346,42 @@ int Level::tiles_grid(int which, const Pick& pos, hipStream_t s) {
     return AP_OK;
 }
 
-// The front of a pass as the passes run it, `reps` times: ms per run. The rooms, then the force grid's fill, tiles
-// and Poisson solve on s and, forked beside them, WA, the cells' WA gradients and the overflow grid's fill and tiles
-// on the side stream, launched in the order objective() captures them, at the caller's current point v (between runs
-// the current point is always set 0). It changes nothing a pass reads before writing it again (the bucket counts are
-// the last evaluation's, which set the next rooms; pin and cell gradients, partials, rho and the field are rewritten
-// by every pass).
-int Level::time_front(int reps, hipStream_t s, float& ms) {
-    const Pick pos{(const float2*)d.v, nullptr, nullptr};
-    int st;
-    if ((st = set_lds()) || (st = init_rooms(pos, s))) return st;
-    AP_HIP(hipEventRecord(t0, s));
-    for (int r = 0; r < reps; r++) {
-        hipLaunchKernelGGL(eng_rooms, dim3(jj.count ? 2 : 1), dim3(EB), 0, s, fj, jj);
-        AP_LAUNCHED();
-        AP_HIP(hipEventRecord(ev_fork, s));
-        AP_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-        if ((st = bin_grid(0, pos, s))) return st;
-        if ((st = wa_plan.launch(wa_args(pos, d.kp), p_wa, side))) return st;
-        hipLaunchKernelGGL(eng_cell_wl, dim3(nblk_n), dim3(EB), 0, side,
-                           gather_args(pos, d.kp, nullptr, nullptr, nullptr, nullptr, 1), cell_wl, p_wl2);
-        AP_LAUNCHED();
-        if (jj.count && (st = bin_grid(1, pos, side))) return st;
-        if ((st = tiles_grid(0, pos, s))) return st;
-        if (jj.count && (st = tiles_grid(1, pos, side))) return st;
-        if ((st = solve_field(s))) return st;
-        AP_HIP(hipEventRecord(ev_join, side));
-        AP_HIP(hipStreamWaitEvent(s, ev_join, 0));
+// The state whole passes change: both point sets (u, v, g), the best copy, the control state S / kp, the set flags
+// and the learned controller's state. Everything else a pass writes it recomputes before reading (gradients, maps,
+// field, partials), or changes only the time (the bucket rooms).
+// (state_copy's layout: 7 point-set arrays, then 4 small arrays of at most 256 bytes, each start 256-byte aligned)
+size_t Level::state_bytes() const {
+    const size_t n2 = ((size_t)d.num_nodes * sizeof(float2) + 255) & ~(size_t)255;
+    return 7 * n2 + 4 * 256;
+}
+
+int Level::state_copy(char* keep, bool restore, hipStream_t s) {
+    const size_t n2 = (size_t)d.num_nodes * sizeof(float2);
+    struct Part {
+        void* p;
+        size_t n;
+    } parts[] = {{d.u, n2}, {d.v, n2}, {d.g, n2}, {u1, n2}, {v1, n2}, {g1, n2}, {d.best, n2},
+                 {d.S, NUM_S * sizeof(double)}, {d.kp, 5 * sizeof(float)}, {flags, 4 * sizeof(int)},
+                 {ctl_x, NUM_X * sizeof(double)}};
+    static_assert(NUM_S * sizeof(double) <= 256 && NUM_X * sizeof(double) <= 256, "state_bytes: small arrays");
+    size_t off = 0;
+    for (const Part& q : parts) {
+        void* dst = restore ? q.p : (void*)(keep + off);
+        const void* src = restore ? (const void*)(keep + off) : q.p;
+        AP_HIP(hipMemcpyAsync(dst, src, q.n, hipMemcpyDeviceToDevice, s));
+        off += (q.n + 255) & ~(size_t)255;
     }
+    return AP_OK;
+}
+
+// `reps` whole passes as direct runs launch them (the two streams), from the current state: ms per pass. The caller
+// saves the state before and restores it after (state_copy).
+int Level::time_passes(int reps, hipStream_t s, float& ms) {
+    int st;
+    if ((st = set_lds()) || (st = init_rooms(cur_pos(), s))) return st;
+    AP_HIP(hipEventRecord(t0, s));
+    for (int r = 0; r < reps; r++)
+        if ((st = pass(s))) return st;
     AP_HIP(hipEventRecord(t1, s));
     AP_HIP(hipEventSynchronize(t1));
     AP_HIP(hipEventElapsedTime(&ms, t0, t1));
@@ -1380,12 +1390,24 @@ int Level::time_front(int reps, hipStream_t s, float& ms) {
 }
 
 // Autotuner of the tile layout: for each grid (force, then overflow), the admissible sides from half to twice the
-// set-up side (8 .. 64) with 1 or 2 tiles per workgroup are timed on the front of a pass (time_front) at the
-// current positions and the fastest kept. The map does not depend on either (integer sums), so the choice changes
-// the time only, never the results.
+// set-up side (8 .. 64) with 1 or 2 tiles per workgroup are timed on whole passes from the current state (saved
+// once, restored before every timing and at the end) and the fastest kept. The map does not depend on either
+// (integer sums), so the choice changes the time only, never the results. (Timing the front of a pass alone --
+// WA, binning, tiles, Poisson -- ranked side 32 over side 64 on the 2M-cell design, where whole passes run 1 %
+// faster at 64.)
 int Level::autotune(int reps, hipStream_t s, int* sides) {
     AP_HIP(hipDeviceSynchronize());
     int st;
+    char* keep = nullptr;
+    AP_HIP(hipMalloc(&keep, state_bytes()));
+    if ((st = state_copy(keep, false, s))) {
+        (void)hipFree(keep);
+        return st;
+    }
+    auto timed = [&](int n, float& ms) -> int {
+        int e = state_copy(keep, true, s);
+        return e ? e : time_passes(n, s, ms);
+    };
     // coordinate descent: the force grid, the overflow grid, then the force grid again with the overflow grid's
     // choice in place (the two share the GPU in a pass, so each one's best depends on the other's)
     BinJob* order[3] = {&fj, &jj, &fj};
@@ -1400,11 +1422,11 @@ int Level::autotune(int reps, hipStream_t s, int* sides) {
                 set_side(*J, T, G);
                 update_lds();
                 float warm = 0, ms = 0, again = 0;
-                if ((st = time_front(1, s, warm))) return st;   // warm-up at this layout
+                if ((st = timed(1, warm))) break;                 // warm-up at this layout
                 if (warm > 1.5f * best_ms) continue;              // clearly slower: not timed further
-                if ((st = time_front(reps, s, ms))) return st;
+                if ((st = timed(reps, ms))) break;
                 if (ms < 1.1f * best_ms) {                        // a contender: the better of two timings
-                    if ((st = time_front(reps, s, again))) return st;
+                    if ((st = timed(reps, again))) break;
                     ms = std::min(ms, again);
                 }
                 if (ms < best_ms) {
@@ -1413,10 +1435,17 @@ int Level::autotune(int reps, hipStream_t s, int* sides) {
                     best_g = G;
                 }
             }
+            if (st) break;
         }
+        if (st) break;
         set_side(*J, best, best_g);
         update_lds();
     }
+    // the state the timing passes changed, back; then the keep buffer goes
+    if (!st) st = state_copy(keep, true, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(keep);
+    if (st) return st;
     // the timing runs leave the fill form's counts behind: back to zero, the state the counting-sort form (the
     // batch's levels) starts from; the fill form counts again at the new layout anyway (rooms_stale)
     AP_HIP(hipDeviceSynchronize());

@@ -45,7 +45,7 @@ def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None, phase=0
             reason, iters, hpwl, overflow = gpu_loop.run_level_on_gpu(
                 optimizer, objective, ps, args, log, iteration, use_graph=opts.get("graph", True),
                 poisson=opts.get("poisson", "auto"), reorder=opts.get("reorder", True), hook=opts.get("hook"),
-                controller=opts.get("controller"), phase=phase)
+                controller=opts.get("controller"), phase=phase, autotune=opts.get("autotune"))
             return SimpleNamespace(reason=reason, iters=iters, start_iters=start_iters, hpwl=hpwl, overflow=overflow)
         optimizer.step()
         if optimizer.diverged:

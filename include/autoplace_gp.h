@@ -188,8 +188,8 @@ AP_API int ap_level_density(ap_level* level, float* rho, void* stream);
 
 /* Autotuner of the density tile layout: the admissible sides from half to twice the set-up side (8 .. 64 bins) with
  * 1 or 2 tiles per workgroup of the tile kernel, for the force grid, then the overflow grid, then the force grid
- * again, timed on the front of a pass (WA, binning, tiles, Poisson solve, as the passes overlap them) at the caller's
- * current point (reps runs each); the fastest is kept. layout i32[4] (optional): force side, force group, overflow
+ * again, timed on whole passes (reps each) from the level's current state, which is saved first and restored after
+ * every timing; the fastest is kept. layout i32[4] (optional): force side, force group, overflow
  * side, overflow group (0, 0: no overflow grid). The density map depends on neither, so the results of later passes
  * are the same whatever the choice. Drops the level's captured graphs. */
 AP_API int ap_level_autotune(ap_level* level, int reps, int* layout);

@@ -66,3 +66,16 @@ def test_run_is_bit_for_bit_repeatable(design_a, runs):
     p0, m0 = runs[0]
     assert np.array_equal(pos.cpu().numpy(), p0)
     assert m["hpwl_sites"] == m0["hpwl_sites"] and m["iters"] == m0["iters"]
+
+
+def test_autotuned_run_ends_where_the_plain_run_ends(design_a, runs):
+    """The tile-layout autotuner (forced on for both levels of this 10k-cell design) times whole passes from the
+    level's state and restores it: the run ends bit for bit where the run without it ends."""
+    from autoplace_amd.problem import default_args
+    from autoplace_amd.run_placement import place
+    pos, m = place(design_a, default_args(seed=0), "cuda:0", opts={"autotune": True})
+    ref_pos, ref = runs[0]
+    print("autotuned run: %d iterations, HPWL %.6e (plain: %d, %.6e)" % (m["iters"], m["hpwl_sites"], ref["iters"],
+                                                                        ref["hpwl_sites"]))
+    assert np.array_equal(pos.cpu().numpy(), ref_pos)
+    assert m["iters"] == ref["iters"] and m["hpwl_sites"] == ref["hpwl_sites"]
