@@ -1392,9 +1392,7 @@ int Level::time_passes(int reps, hipStream_t s, float& ms) {
 // Autotuner of the tile layout: for each grid (force, then overflow), the admissible sides from half to twice the
 // set-up side (8 .. 64) with 1 or 2 tiles per workgroup are timed on whole passes from the current state (saved
 // once, restored before every timing and at the end) and the fastest kept. The map does not depend on either
-// (integer sums), so the choice changes the time only, never the results. (Timing the front of a pass alone --
-// WA, binning, tiles, Poisson -- ranked side 32 over side 64 on the 2M-cell design, where whole passes run 1 %
-// faster at 64.)
+// (integer sums), so the choice changes the time only, never the results.
 int Level::autotune(int reps, hipStream_t s, int* sides) {
     AP_HIP(hipDeviceSynchronize());
     int st;
