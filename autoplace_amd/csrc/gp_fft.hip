@@ -260,6 +260,7 @@ __device__ __forceinline__ void weights_at(const PassB& a, int k, int l, float n
     ax = w * inv * (l == 0 ? 0.5f : 1.f) * C;
 }
 
+template <int U>
 __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
     extern __shared__ float2 lds[];
     const int L = a.N, lg = a.lgN, half = L >> 1;
@@ -281,9 +282,9 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
     __syncthreads();
     fft_lds(s, tw_s, tws, L, lg, false);
     // DCT-II of both rows at this lane's units: c[u][row][0 at k, 1 at L - k]
-    float c[FFT_UNITS][2][2];
+    float c[U][2][2];
 #pragma unroll
-    for (int u = 0; u < FFT_UNITS; u++) {
+    for (int u = 0; u < U; u++) {
         const int k = threadIdx.x + u * FB;
         c[u][0][0] = c[u][0][1] = c[u][1][0] = c[u][1][1] = 0.f;
         if (k > half) continue;
@@ -297,7 +298,7 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
         const int l = l0 + q;
         const float nu = a.nu[l];
 #pragma unroll
-        for (int u = 0; u < FFT_UNITS; u++) {
+        for (int u = 0; u < U; u++) {
             const int k = threadIdx.x + u * FB;
             if (k > half) continue;
             float ay, ax, ayr = 0.f, axr = 0.f;
@@ -313,8 +314,10 @@ __device__ __forceinline__ void pass_b_body(const PassB& a, int wg) {
 }
 
 // the column-reading passes run consecutive workgroups on the same XCD, so the rows' lines they share meet in one L2
+// (U: pair units per lane, (L / 2) / FB + 1 for the grid's side)
+template <int U>
 __global__ __launch_bounds__(FB) void fft_pass_b(PassB a) {
-    pass_b_body(a, a.col ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x);
+    pass_b_body<U>(a, a.col ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x);
 }
 
 // row n of U, V (the batch's grouped launches, after their transpose): one inverse FFT
@@ -345,6 +348,7 @@ __device__ __forceinline__ void pass_c_body(const float* U, const float* V, floa
 
 // Rows n0 = 2 wg, n0 + 1 of U, V read in place as columns n0, n0 + 1 of pass B's outputs Ut, Vt [L][col] (no
 // transpose): one 8-byte load per element pair, kept in registers; then one inverse FFT per row, as pass B does its two
+template <int NU>
 __device__ __forceinline__ void pass_c_pair_body(const float* U, const float* V, float* fx, float* fy, float2* f2,
                                                  int ld_out, const float2* tw, const float2* ph, int L, int lg, int wg,
                                                  int col) {
@@ -355,9 +359,9 @@ __device__ __forceinline__ void pass_c_pair_body(const float* U, const float* V,
     float4* tws = (float4*)(lds + tws_off(L));
     load_twiddles(tw_s, tws, tw, L, lg);
     const int n0 = 2 * wg;
-    float2 cu[FFT_UNITS][2], cv[FFT_UNITS][2];   // rows (n0, n0 + 1) of U and V at k [0] and at L - k [1]
+    float2 cu[NU][2], cv[NU][2];   // rows (n0, n0 + 1) of U and V at k [0] and at L - k [1]
 #pragma unroll
-    for (int u = 0; u < FFT_UNITS; u++) {
+    for (int u = 0; u < NU; u++) {
         const int k = threadIdx.x + u * FB;
         cu[u][0] = cu[u][1] = cv[u][0] = cv[u][1] = make_float2(0.f, 0.f);
         if (k > half) continue;
@@ -373,7 +377,7 @@ __device__ __forceinline__ void pass_c_pair_body(const float* U, const float* V,
 #pragma unroll
     for (int q = 0; q < 2; q++) {   // unrolled: the pairs stay in registers
 #pragma unroll
-        for (int u = 0; u < FFT_UNITS; u++) {
+        for (int u = 0; u < NU; u++) {
             const int k = threadIdx.x + u * FB;
             if (k > half) continue;
             cs_put_unit(s, ph, L, lg, k, q ? cu[u][0].y : cu[u][0].x, q ? cu[u][1].y : cu[u][1].x,
@@ -391,10 +395,11 @@ __device__ __forceinline__ void pass_c_pair_body(const float* U, const float* V,
 }
 
 // U, V: pass B's outputs Ut, Vt [L][col], each workgroup two columns
+template <int NU>
 __global__ __launch_bounds__(FB) void fft_pass_c(const float* U, const float* V, float* fx, float* fy, float2* f2,
                                                  int ld_out, const float2* tw, const float2* ph, int L, int lg,
                                                  int col) {
-    pass_c_pair_body(U, V, fx, fy, f2, ld_out, tw, ph, L, lg, xcd_remap(blockIdx.x, gridDim.x), col);
+    pass_c_pair_body<NU>(U, V, fx, fy, f2, ld_out, tw, ph, L, lg, xcd_remap(blockIdx.x, gridDim.x), col);
 }
 
 // out[c][r] = in[r][c] for 2 planes (blockIdx.z), 32 x 32 tiles through LDS
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(FB) void fft_b_grouped(const FftTab* tab, const int
     const FftTab t = tab[m.x];
     if (fft_skip(t)) return;
     const PassB b{t.w1, t.w2, t.w3, t.omega, t.nu, t.scale, t.bx, t.N, t.lgN, t.twN, t.phN, 0};
-    pass_b_body(b, m.y);
+    pass_b_body<FFT_UNITS>(b, m.y);
 }
 __global__ __launch_bounds__(256) void fft_t2_grouped(const FftTab* tab, const int4* map) {
     const int4 m = map[blockIdx.x];
@@ -465,8 +470,10 @@ static int fft_lds_limit() {
     int lg = 0;
     while ((1 << lg) < Poisson::fft_max) lg++;
     const int bytes = fft_slots(Poisson::fft_max, lg) * (int)sizeof(float2);
-    const void* fn[6] = {(const void*)fft_pass_a, (const void*)fft_pass_b, (const void*)fft_pass_c,
-                         (const void*)fft_a_grouped, (const void*)fft_b_grouped, (const void*)fft_c_grouped};
+    const void* fn[12] = {(const void*)fft_pass_a,      (const void*)fft_pass_b<1>,  (const void*)fft_pass_b<2>,
+                          (const void*)fft_pass_b<3>,   (const void*)fft_pass_b<5>,  (const void*)fft_pass_c<1>,
+                          (const void*)fft_pass_c<2>,   (const void*)fft_pass_c<3>,  (const void*)fft_pass_c<5>,
+                          (const void*)fft_a_grouped,   (const void*)fft_b_grouped,  (const void*)fft_c_grouped};
     for (const void* f : fn) AP_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     done = true;
     return AP_OK;
@@ -612,10 +619,27 @@ int Poisson::fft_solve(const float* rho, int rho_ld, float* fx, float* fy, float
     AP_LAUNCHED();
     // pass B: columns of T (length N), read in place by pairs -> Ut (w2), Vt (w3); no transpose
     PassB b{w0, w2, w3, d_omega, d_omega + N, (float)(4.0 / ((double)N * M)), (float)bx, N, lgN, twN, phN, M};
-    hipLaunchKernelGGL(fft_pass_b, dim3(M / 2), dim3(FB), lds_n, s, b);
+    // the pair units per lane a side needs ((L / 2) / FB + 1), as a template argument: registers for those only
+    auto units = [](int L) { return (L / 2) / FB + 1; };
+    const int ub = units(N), uc = units(M);
+    if (ub <= 1)
+        hipLaunchKernelGGL(fft_pass_b<1>, dim3(M / 2), dim3(FB), lds_n, s, b);
+    else if (ub == 2)
+        hipLaunchKernelGGL(fft_pass_b<2>, dim3(M / 2), dim3(FB), lds_n, s, b);
+    else if (ub == 3)
+        hipLaunchKernelGGL(fft_pass_b<3>, dim3(M / 2), dim3(FB), lds_n, s, b);
+    else
+        hipLaunchKernelGGL(fft_pass_b<5>, dim3(M / 2), dim3(FB), lds_n, s, b);
     AP_LAUNCHED();
     // pass C: columns of Ut, Vt (rows of U, V) -> field; no transpose
-    hipLaunchKernelGGL(fft_pass_c, dim3(N / 2), dim3(FB), lds_m, s, w2, w3, fx, fy, f2, fld, twM, phM, M, lgM, N);
+    if (uc <= 1)
+        hipLaunchKernelGGL(fft_pass_c<1>, dim3(N / 2), dim3(FB), lds_m, s, w2, w3, fx, fy, f2, fld, twM, phM, M, lgM, N);
+    else if (uc == 2)
+        hipLaunchKernelGGL(fft_pass_c<2>, dim3(N / 2), dim3(FB), lds_m, s, w2, w3, fx, fy, f2, fld, twM, phM, M, lgM, N);
+    else if (uc == 3)
+        hipLaunchKernelGGL(fft_pass_c<3>, dim3(N / 2), dim3(FB), lds_m, s, w2, w3, fx, fy, f2, fld, twM, phM, M, lgM, N);
+    else
+        hipLaunchKernelGGL(fft_pass_c<5>, dim3(N / 2), dim3(FB), lds_m, s, w2, w3, fx, fy, f2, fld, twM, phM, M, lgM, N);
     AP_LAUNCHED();
     return AP_OK;
 }
