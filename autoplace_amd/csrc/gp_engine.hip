@@ -121,7 +121,7 @@ __device__ __forceinline__ void nesterov_body(const NestArgs& a, int wg) {
 // (eng_over_* : the overflow grid's launches, beside the force grid's)
 constexpr int BIN_PER_LANE = 8;
 constexpr int BIN_NODES = EB * BIN_PER_LANE;
-constexpr int BIN_MAX_BUCKETS = 16384;   // planes x tiles of a grid: the count kernel's LDS histogram, 64 KB
+constexpr int BIN_MAX_BUCKETS = 16384;   // planes x tiles of a grid: the count / fill kernels' LDS histogram, 64 KB
 constexpr int SCAN_T = 1024;
 
 struct BinJob {
@@ -1034,7 +1034,7 @@ __global__ void eng_finalize_flags(int* flags) {
 // Tile side of a grid: the power of two from 8 to 64 at or below the side that gives BIN_TILE_NODES nodes per tile
 // on average (2048 x 2048 bins and 2.9M nodes: 32; 512 x 512 bins and 2M cells: 8), raised while more than
 // max(64, nodes / 1024) nodes could span more than T + 1 bins along an axis (those take the big-node path, which
-// every tile's workgroup scans) or while the buckets do not fit the count kernel's histogram. 0: no side fits.
+// every tile's workgroup scans) or while the buckets do not fit the count / fill kernels' histogram. 0: no side fits.
 constexpr int BIN_TILE_NODES = 1024;
 static int tile_side(const std::vector<float2>& size, double bx, double by, int nbx, int nby, int planes) {
     const long long n = (long long)size.size();
@@ -1254,7 +1254,7 @@ int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nod
     return AP_OK;
 }
 
-// a tile side the binning can use for J: 8..64 and the buckets within the count kernel's histogram
+// a tile side the binning can use for J: 8..64 and the buckets within the count / fill kernels' histogram
 bool Level::side_ok(const BinJob& J, int T) const {
     return T >= 8 && T <= 64 && (long long)J.planes * grid_of(J.nbx, T) * grid_of(J.nby, T) <= BIN_MAX_BUCKETS;
 }
