@@ -15,7 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libautoplace_gp.so")
-SOURCES = ["gp_kernels.hip", "gp_wa.hip", "gp_poisson.hip", "gp_fft.hip", "gp_engine.hip", "gp_batched.hip", "gp_mlp.hip"]
+SOURCES = ["gp_kernels.hip", "gp_wa.hip", "gp_poisson.hip", "gp_fft.hip", "gp_engine.hip", "gp_mlp.hip"]
 ARCH = os.environ.get("AUTOPLACE_ARCH", "gfx950")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-fvisibility=hidden", "-Wall", "-Wno-unused-function"]
@@ -31,7 +31,10 @@ def _hipcc():
 def build(verbose=False):
     """Compile every HIP source and link the library (always rebuilds). -> path of the library."""
     hipcc = _hipcc()
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    missing = [s for s in SOURCES if not os.path.exists(os.path.join(CSRC, s))]
+    if missing:
+        raise RuntimeError("HIP sources missing from %s: %s" % (CSRC, ", ".join(missing)))
+    srcs = list(SOURCES)
     with tempfile.TemporaryDirectory() as tmp:
         def compile_one(src):
             obj = os.path.join(tmp, src.replace(".hip", ".o"))

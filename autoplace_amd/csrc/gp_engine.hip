@@ -1734,6 +1734,12 @@ int Level::graph_for(int passes, int& slot) {
     slot = graph_next;
     graph_next = (graph_next + 1) % GRAPH_CACHE;
     GraphEntry& G = graphs[slot];
+    if (G.exec) {
+        // the evicted graph may still run (a replay launched without a host sync since): destroying its exec, its
+        // graph or its timing events while it runs frees what the runtime still reads, so wait for both streams
+        AP_HIP(hipStreamSynchronize(stream));
+        AP_HIP(hipStreamSynchronize(side));
+    }
     if (last_ev == &G.ev) last_ev = nullptr;
     G.clear();
     if (timing_cls >= 0) {
@@ -2047,6 +2053,7 @@ int Batch::graph_for(int passes, int& slot) {
     slot = graph_next;
     graph_next = (graph_next + 1) % GRAPH_CACHE;
     GraphEntry& G = graphs[slot];
+    if (G.exec) AP_HIP(hipStreamSynchronize(stream));   // the evicted graph may still run: wait before destroying it
     G.clear();
     AP_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     int st = AP_OK;

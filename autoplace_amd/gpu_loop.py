@@ -24,6 +24,16 @@ def usable(optimizer):
     return optimizer.kernel
 
 
+def as_log(log):
+    """A line sink from what the caller passes: a logging.Logger (the reference passes one, gpu_loop.py:35,66-68, and
+    calls its info), any callable taking one string, or None (silent)."""
+    if log is None or callable(log) and not hasattr(log, "info"):
+        return log
+    if hasattr(log, "info"):
+        return log.info
+    raise TypeError("log must be a logging.Logger, a callable or None, not %s" % type(log).__name__)
+
+
 def next_coef(a):
     a_next = (1 + math.sqrt(4 * a * a + 1)) / 2
     return (a - 1) / a_next
@@ -228,6 +238,7 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
     autotune: time the density tile layouts at the start and keep the fastest (None: for the cell level when it has
     100k nodes or more -- the long level, where the timing pays for itself); the results do not depend on it."""
     device, pos = optimizer.v.device, optimizer.v
+    log = as_log(log)
     S, kp, c = control_state(ps, optimizer, iteration, args)
     S = torch.tensor(S, dtype=torch.float64, device=device)
     kp = torch.tensor(kp, dtype=torch.float32, device=device)

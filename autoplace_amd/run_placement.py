@@ -34,6 +34,7 @@ def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None, phase=0
     """One level (run_placement.py:41-76): host loop through the wirelength-only start, then the device loop.
     -> namespace(reason, iters, start_iters, hpwl, overflow)."""
     opts = opts or {}
+    log = gpu_loop.as_log(log)
     bounds = (objective.density.pos_lo, objective.density.pos_hi)
     lr = estimate_initial_learning_rate(pos, objective, args.lr, bounds)
     optimizer = NesterovOptimizer(pos, objective, lr, bounds, stats=lambda: (objective.hpwl, objective.overflow,
@@ -71,6 +72,7 @@ def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=N
     device, n = pos.device, prob.num_mov
     data, node_size, fixed_fraction = prob.data, prob.node_size, prob.fixed_fraction
     opts = opts or {}
+    log = gpu_loop.as_log(log)
     _sync(device)
     start = time.time()
     ps = _scheduler(density, n, data.mov_area, float(data.cell_pins.sum()), float(node_size.prod(1).sum()),
@@ -147,6 +149,7 @@ def place(data, args, device="cuda", log=None, opts=None):
     if device.type == "cuda" and device.index is not None:
         torch.cuda.set_device(device)
     opts = opts or {}
+    log = gpu_loop.as_log(log)
     prob = build_problem(data, args, log)
     wirelength, density = build_operators(prob, args, device, opts.get("poisson", "auto"))
     metrics = {"design": data.name, "num_movable": data.num_mov, "num_nets": data.num_nets,

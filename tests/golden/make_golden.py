@@ -12,6 +12,9 @@ reader. What is recorded (inputs and outputs only, no reference code):
     golden_sched.npz   ParamScheduler.step / need_to_stop over a scripted input sequence (both --wa_rule values)
     golden_runA.json   --gpu -1 runs of the 10k-cell design, placer seeds 0-5: metrics (with --full-run)
     golden_runA.npz    the final cell positions of the seed-0 run (with --full-run)
+    golden_runB.json   --gpu -1 run of the 200k-cell design (--num_bin 512 --force_num_bin 1024), placer seed 0
+                       (with --run B)
+    golden_runC.json   --gpu -1 run of the 2M-cell design (auto grids: 512 / 2048), placer seed 0 (with --run C)
 """
 import argparse
 import json
@@ -169,12 +172,24 @@ def full_run(tmp, seeds=(0, 1, 2, 3, 4, 5)):
         json.dump(runs, f, indent=1, sort_keys=True)
 
 
-def _one_run(lp, dp, seed):
+def config_run(tmp, name):
+    """One reference run of a scale config: B at its stated grids, C at the automatic ones."""
+    grids = {"B": (512, 1024), "C": (0, 0)}[name]
+    lef, d = synth.generate_config(name, seed=0)
+    lp, dp = write_design(lef, d, tmp, "synth_" + name)
+    del lef, d
+    m = _one_run(lp, dp, 0, num_bin=grids[0], force_num_bin=grids[1], keep_pos=False)
+    with open(os.path.join(HERE, "golden_run%s.json" % name), "w") as f:
+        json.dump({"0": m}, f, indent=1, sort_keys=True)
+
+
+def _one_run(lp, dp, seed, num_bin=0, force_num_bin=0, keep_pos=True):
     from src.run_placement import run_placement_main
     import logging
 
     args = SimpleNamespace(lef=[lp], def_file=dp, out_dir="", eval_def="", write_def=False, seed=seed, gpu=-1,
-                           target_density=1.0, num_bin=0, force_num_bin=0, stop_overflow=0.07, inner_iter=10000,
+                           target_density=1.0, num_bin=num_bin, force_num_bin=force_num_bin, stop_overflow=0.07,
+                           inner_iter=10000,
                            settle_iter=30, stall_iter=200, min_iter=100, ignore_net_degree=100, start_tol=1e-3,
                            density_weight=1e-3, wa_coeff=4.0, wa_rule="accuracy", cluster_ratio=5.0, lr=0.01,
                            log_freq=100)
@@ -192,7 +207,7 @@ def _one_run(lp, dp, seed):
     rp.evaluate_placement = spy
     m = run_placement_main(args, logger)
     m = {k: v for k, v in m.items() if isinstance(v, (int, float, str, bool))}
-    if seed == 0:
+    if seed == 0 and keep_pos:
         np.savez_compressed(os.path.join(HERE, "golden_runA.npz"), pos=keep["pos"].astype(np.float32))
     return m
 
@@ -200,11 +215,15 @@ def _one_run(lp, dp, seed):
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--full-run", action="store_true")
+    p.add_argument("--run", choices=("B", "C"), help="only the reference run of this scale config")
     a = p.parse_args()
     sys.path.insert(0, REF)
     import torch
     torch.set_num_threads(8)
     with tempfile.TemporaryDirectory() as tmp:
+        if a.run:
+            config_run(tmp, a.run)
+            return
         small_golden(tmp)
         sched_golden()
         if a.full_run:

@@ -292,3 +292,27 @@ def test_engine_trajectory_vs_oracle(small_design, poisson):
         assert rel_err(S_dev[K[name]], S[K[name]]) < 1e-4, (name, S_dev[K[name]], S[K[name]])
     print("engine vs oracle after 16 passes: v err %.2e" % norm_err(v, vo))
     assert norm_err(v, vo) < 1e-4
+
+
+def test_graph_cache_eviction_without_host_syncs(small_design):
+    """More distinct captured runs than the level keeps (4 graphs: pass counts 1..6, each with another timed kernel
+    class) replayed back to back with no host synchronisation between them: every eviction destroys a graph whose
+    replay may still be running, which the engine waits for first. The state ends bit for bit where direct launches
+    of the same passes end."""
+    _, _, data = small_design
+    args, prob, wl, den, pos, S, kp, c = _setup(data)
+    classes = ("wa", "density_tiles", "gather", None, "nesterov", "overflow_tiles")
+    outs = []
+    for use_graph in (True, False):
+        eng, bufs = _engine(wl, den, pos, S, kp, c, np.zeros_like(pos), reorder=True, poisson="fft")
+        for passes, cls in zip(range(1, 7), classes):
+            eng.time_class(cls)
+            eng.run(passes, use_graph)
+        eng.run(3, use_graph)                        # a cached graph again (3 passes, timing class of the last run)
+        eng.finish()
+        torch.cuda.synchronize()
+        outs.append([b.cpu().numpy().copy() for b in bufs])
+        eng.close()
+    for x, y in zip(*outs):
+        assert np.array_equal(x, y)
+    assert outs[0][4][octl.K["evals"]] == 24
