@@ -1,156 +1,191 @@
-"""The placement problem as arrays, built from parsed LEF + DEF data.
+"""The placement problem as arrays, from a parsed LEF library and DEF design (the behaviour of src/database.py:11-154
+of the reference: units, region, movable / fixed split, blocked area, pin indexing, free-area fractions, DEF
+coordinates).
 
-Follows src/database.py:11-154 of the reference (units, region, movable / fixed split, blocked rectangles, pin
-indexing, free-area fractions, DEF coordinates). Length unit: one site width of the narrowest ROW site; origin at
-the lower left of the placement region (bounding box of the rows); positions are cell centres.
+Units. One placer length unit is the width of the narrowest site a ROW uses; the origin is the lower-left corner of
+the placement region (the bounding box of the rows); a cell's position is its centre.
 
-Pin indexing (database.py:65-100): a movable pin is (cell index, offset from the cell centre); a pin of a fixed
-component or a design pin is (num_mov, absolute position). Pins of a net are contiguous, nets in DEF order.
+Pins (database.py:65-100). Every connection of a net becomes one pin row: a pin of a movable cell is (its cell
+index, offset from the cell centre); a pin of a fixed component or a design pin is (num_mov, absolute position).
+The pins of a net are contiguous, nets in DEF order. The arithmetic of the offsets is the reference's, operation for
+operation, so the offsets agree bit for bit.
 """
 import numpy as np
 
-_SWAP = ("W", "E", "FW", "FE")
+# orientation -> (swap the axes, mirror the first coordinate, mirror the second), applied to a point of an N-oriented
+# w x h master: after the optional swap the point is (a, b) in an A x B box, and a mirror maps a to A - a
+_ORIENT = {"N": (False, False, False), "S": (False, True, True), "FN": (False, True, False),
+           "FS": (False, False, True), "W": (True, True, False), "E": (True, False, True),
+           "FW": (True, False, False), "FE": (True, True, True)}
 
 
 def orient_size(w, h, orient):
-    return (h, w) if orient in _SWAP else (w, h)
+    """Footprint (width, height) of a w x h master placed with `orient`."""
+    return (h, w) if _ORIENT[orient][0] else (w, h)
 
 
 def orient_point(x, y, w, h, orient):
-    """Point (x, y) of a w x h cell (from its lower left, orientation N) after placing it with `orient`."""
-    return {"N": (x, y), "S": (w - x, h - y), "FN": (w - x, y), "FS": (x, h - y),
-            "W": (h - y, x), "E": (y, w - x), "FW": (y, x), "FE": (h - y, w - x)}[orient]
+    """Where the point (x, y) of a w x h master (from its lower left, orientation N) lands, from the lower left of
+    the placed footprint."""
+    swap, mirror_a, mirror_b = _ORIENT[orient]
+    a, b, A, B = (y, x, h, w) if swap else (x, y, w, h)
+    return (A - a if mirror_a else a), (B - b if mirror_b else b)
+
+
+def _site_units(lef, d):
+    """-> (site width, row height) in DBU: the narrowest site width and the lowest site height of the rows."""
+    if not d.rows:
+        raise ValueError("cannot place %s: its DEF defines no ROW" % d.design)
+    used = sorted({row[0] for row in d.rows})
+    unknown = [s for s in used if s not in lef.sites]
+    if unknown:
+        raise ValueError("the LEF does not define the site(s) the rows use: %s" % ", ".join(unknown))
+    widths = [lef.sites[s][0] * d.dbu for s in used]
+    heights = [lef.sites[s][1] * d.dbu for s in used]
+    return int(round(min(widths))), int(round(min(heights)))
+
+
+def _row_boxes(lef, d):
+    """[rows, 4] int64 (x0, y0, x1, y1) DBU: each row's sites, the last one ending its box."""
+    out = np.empty((len(d.rows), 4), dtype=np.int64)
+    for k, (site, x, y, nx, ny, sx, sy) in enumerate(d.rows):
+        site_w, site_h = (int(round(v * d.dbu)) for v in lef.sites[site])
+        out[k] = (x, y, x + (nx - 1) * sx + site_w, y + (ny - 1) * sy + site_h)
+    return out
+
+
+def _mark_union(boxes, xs, ys, x0, y0, x1, y1):
+    """Cells of the grid of edges xs x ys (intervals [xs[i], xs[i+1]) x [ys[j], ys[j+1])) covered by the union of
+    the boxes, clipped to the region. -> bool [len(xs) - 1, len(ys) - 1]."""
+    cover = np.zeros((len(xs) - 1, len(ys) - 1), dtype=bool)
+    if len(boxes):
+        lo_x = np.searchsorted(xs, np.clip(boxes[:, 0], x0, x1))
+        hi_x = np.searchsorted(xs, np.clip(boxes[:, 2], x0, x1))
+        lo_y = np.searchsorted(ys, np.clip(boxes[:, 1], y0, y1))
+        hi_y = np.searchsorted(ys, np.clip(boxes[:, 3], y0, y1))
+        for a, b, c, e in zip(lo_x, hi_x, lo_y, hi_y):
+            cover[a:b, c:e] = True
+    return cover
+
+
+def _interval_bin_overlap(edges, lo, hi, nb):
+    """[len(edges) - 1, nb] length of each edge interval inside each of nb equal bins of [lo, hi]."""
+    bins = lo + (hi - lo) * np.arange(nb + 1) / float(nb)
+    left = np.maximum(edges[:-1, None], bins[None, :-1])
+    right = np.minimum(edges[1:, None], bins[None, 1:])
+    return np.clip(right - left, 0, None)
 
 
 class PlaceData(object):
+    """The arrays of one design (see the module docstring for units and pin indexing)."""
+
     def __init__(self, lef, d):
-        self.d = d
-        self.name = d.design
-        if not d.rows:
-            raise ValueError("the DEF has no ROW: no placement region")
-        sites = sorted({r[0] for r in d.rows})
-        missing = [s for s in sites if s not in lef.sites]
-        if missing:
-            raise ValueError("ROW site(s) not in the LEF: %s" % ", ".join(missing))
-        dbu = d.dbu
-        self.unit_dbu = int(round(min(lef.sites[s][0] for s in sites) * dbu))
-        self.row_height_dbu = int(round(min(lef.sites[s][1] for s in sites) * dbu))
+        self.d, self.name = d, d.design
+        self.unit_dbu, self.row_height_dbu = _site_units(lef, d)
+        self.row_rects = _row_boxes(lef, d)
+        lo, hi = self.row_rects[:, :2].min(0), self.row_rects[:, 2:].max(0)
+        self.region = (int(lo[0]), int(lo[1]), int(hi[0]), int(hi[1]))
+        unit = float(self.unit_dbu)
+        self.width = (self.region[2] - self.region[0]) / unit
+        self.height = (self.region[3] - self.region[1]) / unit
+        self.row_height = self.row_height_dbu / unit
+        self._split_components(lef, d, unit)
+        self._index_pins(lef, d, unit)
 
-        rects = []
-        for site, x, y, nx, ny, sx, sy in d.rows:
-            sw, sh = (int(round(v * dbu)) for v in lef.sites[site])
-            rects.append((x, y, x + (nx - 1) * sx + sw, y + (ny - 1) * sy + sh))
-        self.row_rects = np.array(rects, dtype=np.int64)
-        self.region = (int(self.row_rects[:, 0].min()), int(self.row_rects[:, 1].min()),
-                       int(self.row_rects[:, 2].max()), int(self.row_rects[:, 3].max()))
-        u = float(self.unit_dbu)
-        self.width = (self.region[2] - self.region[0]) / u
-        self.height = (self.region[3] - self.region[1]) / u
-        self.row_height = self.row_height_dbu / u
-
-        names, comp_macro = d.macro_names, d.comp_macro_id
-        unknown = [m for m in names if m not in lef.macros]
-        if unknown:
-            raise ValueError("%d macro(s) of the DEF are not in the LEF: %s" % (len(unknown), ", ".join(unknown[:5])))
-        master_size = np.array([[lef.macros[m].width, lef.macros[m].height] for m in names]).reshape(-1, 2) * dbu
-        size_dbu = master_size[comp_macro]
-        core = np.array([lef.macros[m].cls.startswith("CORE") for m in names], dtype=bool)
-        # fixed: FIXED / COVER, and placed components that are not core cells (they stay where they are)
-        self.fixed = d.comp_fixed | (d.comp_placed & ~core[comp_macro])
-        self.kept_in_place = int((self.fixed & ~d.comp_fixed).sum())
-        self.mov_index = np.nonzero(~self.fixed)[0]
+    def _split_components(self, lef, d, unit):
+        """Movable cells and blocked area. A component stays where the DEF puts it when it is FIXED / COVER, or
+        placed without being a core cell (pads, endcaps, macro blocks)."""
+        masters = d.macro_names
+        absent = [m for m in masters if m not in lef.macros]
+        if absent:
+            raise ValueError("the LEF lacks %d master(s) the DEF instantiates, e.g. %s"
+                             % (len(absent), ", ".join(absent[:5])))
+        dims = np.array([[lef.macros[m].width, lef.macros[m].height] for m in masters]).reshape(-1, 2) * d.dbu
+        self._comp_dbu = dims[d.comp_macro_id]
+        is_core = np.array([lef.macros[m].cls.startswith("CORE") for m in masters], dtype=bool)[d.comp_macro_id]
+        self.fixed = d.comp_fixed | (d.comp_placed & ~is_core)
+        self.kept_in_place = int(np.count_nonzero(self.fixed & ~d.comp_fixed))
+        self.mov_index = np.flatnonzero(~self.fixed)
         self.num_mov = len(self.mov_index)
-        mov_id = np.full(len(comp_macro), self.num_mov, dtype=np.int64)
-        mov_id[self.mov_index] = np.arange(self.num_mov)
-        self.size = size_dbu[self.mov_index] / u
+        self._cell_of = np.full(len(self.fixed), self.num_mov, dtype=np.int64)
+        self._cell_of[self.mov_index] = np.arange(self.num_mov)
+        self.size = self._comp_dbu[self.mov_index] / unit
         self.mov_area = float(self.size.prod(1).sum())
+        boxes = [tuple(b) for b in d.blockages]
+        for i in np.flatnonzero(self.fixed):
+            fw, fh = orient_size(self._comp_dbu[i, 0], self._comp_dbu[i, 1], d.comp_orient[i])
+            boxes.append((d.comp_xy[i, 0], d.comp_xy[i, 1], d.comp_xy[i, 0] + fw, d.comp_xy[i, 1] + fh))
+        self.block_rects = np.array(boxes, dtype=np.int64).reshape(-1, 4)
 
-        blocked = [tuple(b) for b in d.blockages]
-        for i in np.nonzero(self.fixed)[0]:
-            w, h = orient_size(size_dbu[i, 0], size_dbu[i, 1], d.comp_orient[i])
-            x, y = d.comp_xy[i]
-            blocked.append((x, y, x + w, y + h))
-        self.block_rects = np.array(blocked, dtype=np.int64).reshape(-1, 4)
-
+    def _index_pins(self, lef, d, unit):
+        """pin_cell / pin_off / pin_net of every connection (design pins without a location dropped)."""
+        masters, names = d.macro_names, d.pin_names
+        design_pin = d.pin_comp < 0
+        comp = np.where(design_pin, 0, d.pin_comp)
+        # each (master, pin name) the netlist uses is looked up in the LEF once
+        pair = d.comp_macro_id[comp].astype(np.int64) * len(names) + d.pin_name
+        uniq, which = np.unique(pair, return_inverse=True)
+        where = {}
+        for p in uniq.tolist():
+            where[p] = lef.macros[masters[p // len(names)]].pins.get(names[p % len(names)])
+        centre = np.array([where[p] if where[p] is not None else (np.nan, np.nan) for p in uniq.tolist()],
+                          dtype=np.float64).reshape(-1, 2)
+        xy = centre[which.reshape(-1)] * d.dbu
+        lost = np.flatnonzero(np.isnan(xy[:, 0]) & ~design_pin)
+        if len(lost):
+            k = lost[0]
+            raise ValueError("the LEF master %s has no pin %s (a connection of the DEF's nets)"
+                             % (masters[d.comp_macro_id[comp[k]]], names[d.pin_name[k]]))
         origin = np.array(self.region[:2], dtype=np.float64)
-        io = d.pin_comp < 0
-        comp = np.where(io, 0, d.pin_comp)
-        npn = len(d.pin_names)
-        key = comp_macro[comp] * npn + d.pin_name                  # (master, pin name) of every connection
-        pairs, slot = np.unique(key, return_inverse=True)
-        pair_xy = np.full((len(pairs), 2), np.nan)
-        for k, pk in enumerate(pairs):
-            xy_m = lef.macros[names[pk // npn]].pins.get(d.pin_names[pk % npn])
-            if xy_m is not None:
-                pair_xy[k] = xy_m
-        xy = pair_xy[slot.reshape(-1)] * dbu
-        bad = np.nonzero(np.isnan(xy[:, 0]) & ~io)[0]
-        if len(bad):
-            raise ValueError("pin %s of macro %s is not in the LEF"
-                             % (d.pin_names[d.pin_name[bad[0]]], names[comp_macro[comp[bad[0]]]]))
-        pin_cell = np.where(io, self.num_mov, mov_id[comp])
-        pin_off = (xy - size_dbu[comp] / 2) / u
-        if io.any():
-            pin_off[io] = (d.io_xy[-1 - d.pin_comp[io]] - origin) / u
-        for k in np.nonzero(~io & (pin_cell == self.num_mov))[0]:
+        cell = np.where(design_pin, self.num_mov, self._cell_of[comp])
+        off = (xy - self._comp_dbu[comp] / 2) / unit                   # from the cell centre
+        if design_pin.any():
+            off[design_pin] = (d.io_xy[-1 - d.pin_comp[design_pin]] - origin) / unit
+        for k in np.flatnonzero(~design_pin & (cell == self.num_mov)):   # pins of fixed components: where they are
             c = comp[k]
-            ox, oy = orient_point(xy[k, 0], xy[k, 1], size_dbu[c, 0], size_dbu[c, 1], d.comp_orient[c])
-            pin_off[k] = (d.comp_xy[c] + (ox, oy) - origin) / u
-        keep = ~np.isnan(pin_off[:, 0])
-        pin_net = np.repeat(np.arange(len(d.net_name)), np.diff(d.net_start))
-        self.unplaced_io_pins = int((~keep).sum())
-        self.pin_cell, self.pin_off, self.pin_net = pin_cell[keep], pin_off[keep], pin_net[keep]
-        self.num_nets = len(d.net_name)
-        if not self.num_mov or not self.num_nets:
-            raise ValueError("nothing to place: %d movable cells, %d nets" % (self.num_mov, self.num_nets))
-        self.num_pins = len(self.pin_cell)
+            px, py = orient_point(xy[k, 0], xy[k, 1], self._comp_dbu[c, 0], self._comp_dbu[c, 1], d.comp_orient[c])
+            off[k] = (d.comp_xy[c] + (px, py) - origin) / unit
+        located = ~np.isnan(off[:, 0])
+        net = np.repeat(np.arange(len(d.net_name)), np.diff(d.net_start))
+        self.unplaced_io_pins = int(np.count_nonzero(~located))
+        self.pin_cell, self.pin_off, self.pin_net = cell[located], off[located], net[located]
+        self.num_nets, self.num_pins = len(d.net_name), int(np.count_nonzero(located))
+        if self.num_mov == 0 or self.num_nets == 0:
+            raise ValueError("%s has %d movable cells and %d nets: nothing to place"
+                             % (d.design, self.num_mov, self.num_nets))
         self.cell_pins = np.bincount(self.pin_cell, minlength=self.num_mov + 1)[:self.num_mov].astype(np.float64)
 
     def fixed_fraction(self, nbx, nby):
-        """[nbx, nby] share of every bin that is not placeable (outside the rows or under a fixed object).
-
-        Exact (database.py:103-134): union of rows minus union of blockers on the grid of all rectangle edges, then
-        the overlap of every edge interval with every bin."""
+        """[nbx, nby] share of every bin of the region that no cell may use: outside every row, or under a fixed
+        component or a blockage (database.py:103-134). Exact: the union of the rows minus the union of the blockers on
+        the grid of all box edges, each grid cell's area split over the bins by the separable interval overlaps."""
         x0, y0, x1, y1 = self.region
-        rects = np.concatenate([self.row_rects, self.block_rects])
-        xs = np.unique(np.clip(np.concatenate([rects[:, 0], rects[:, 2], [x0, x1]]), x0, x1))
-        ys = np.unique(np.clip(np.concatenate([rects[:, 1], rects[:, 3], [y0, y1]]), y0, y1))
-
-        def covered(r):
-            cnt = np.zeros((len(xs), len(ys)), dtype=np.int32)
-            if len(r):
-                a = np.searchsorted(xs, np.clip(r[:, 0], x0, x1))
-                b = np.searchsorted(xs, np.clip(r[:, 2], x0, x1))
-                c = np.searchsorted(ys, np.clip(r[:, 1], y0, y1))
-                e = np.searchsorted(ys, np.clip(r[:, 3], y0, y1))
-                np.add.at(cnt, (a, c), 1)
-                np.add.at(cnt, (b, e), 1)
-                np.add.at(cnt, (a, e), -1)
-                np.add.at(cnt, (b, c), -1)
-            return (cnt.cumsum(0).cumsum(1) > 0)[:-1, :-1]
-
-        free = (covered(self.row_rects) & ~covered(self.block_rects)).astype(np.float64)
-
-        def overlap(edges, lo, hi, nb):
-            b = lo + (hi - lo) * np.arange(nb + 1) / float(nb)
-            return np.clip(np.minimum(edges[1:, None], b[None, 1:]) - np.maximum(edges[:-1, None], b[None, :-1]), 0, None)
-
-        cap = overlap(xs, x0, x1, nbx).T.dot(free).dot(overlap(ys, y0, y1, nby))
+        boxes = np.concatenate([self.row_rects, self.block_rects])
+        xs = np.unique(np.clip(np.concatenate([boxes[:, 0], boxes[:, 2], [x0, x1]]), x0, x1))
+        ys = np.unique(np.clip(np.concatenate([boxes[:, 1], boxes[:, 3], [y0, y1]]), y0, y1))
+        usable = _mark_union(self.row_rects, xs, ys, x0, y0, x1, y1) & \
+            ~_mark_union(self.block_rects, xs, ys, x0, y0, x1, y1)
+        ox = _interval_bin_overlap(xs, x0, x1, nbx)
+        oy = _interval_bin_overlap(ys, y0, y1, nby)
+        free = ox.T.dot(usable.astype(np.float64)).dot(oy)
         bin_area = (x1 - x0) / float(nbx) * (y1 - y0) / float(nby)
-        return np.clip(1.0 - cap / bin_area, 0.0, 1.0)
+        return np.clip(1.0 - free / bin_area, 0.0, 1.0)
 
     def to_def_xy(self, pos):
-        """Cell centres (placer units) -> lower-left corners in database units (rounded)."""
-        ll = (np.asarray(pos, dtype=np.float64) - self.size / 2) * self.unit_dbu
-        return np.rint(ll + np.array(self.region[:2], dtype=np.float64)).astype(np.int64)
+        """Cell centres (placer units) -> lower-left corners in DBU, rounded to the nearest integer."""
+        corner = (np.asarray(pos, dtype=np.float64) - self.size / 2) * self.unit_dbu
+        return np.rint(corner + np.array(self.region[:2], dtype=np.float64)).astype(np.int64)
 
     def placed_pos(self, placed=None):
-        """Cell centres of the movable cells as placed in a DEF of the same netlist."""
-        d, i = placed or self.d, self.mov_index
+        """Centres (placer units) of this design's movable cells as another DEF of the same netlist places them (the
+        design's own DEF by default); the movable set stays this design's."""
+        d = placed or self.d
         if list(d.comp_name) != list(self.d.comp_name):
-            raise ValueError("the placed DEF has other components than the design")
-        if not d.comp_placed[i].all():
-            raise ValueError("%d movable cells of the DEF have no location" % int((~d.comp_placed[i]).sum()))
-        swap = np.array([d.comp_orient[k] in _SWAP for k in i], dtype=bool)
-        size = np.where(swap[:, None], self.size[:, ::-1], self.size)
-        return (d.comp_xy[i] - np.array(self.region[:2])) / float(self.unit_dbu) + size / 2
+            raise ValueError("the placed DEF's components differ from the design's")
+        i = self.mov_index
+        missing = int(np.count_nonzero(~d.comp_placed[i]))
+        if missing:
+            raise ValueError("%d movable cells have no location in the placed DEF" % missing)
+        swapped = np.array([_ORIENT[d.comp_orient[k]][0] for k in i], dtype=bool)
+        footprint = np.where(swapped[:, None], self.size[:, ::-1], self.size)
+        return (d.comp_xy[i] - np.array(self.region[:2])) / float(self.unit_dbu) + footprint / 2

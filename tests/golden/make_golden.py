@@ -13,7 +13,7 @@ reader. What is recorded (inputs and outputs only, no reference code):
     golden_runA.json   --gpu -1 runs of the 10k-cell design, placer seeds 0-5: metrics (with --full-run)
     golden_runA.npz    the final cell positions of the seed-0 run (with --full-run)
     golden_runB.json   --gpu -1 run of the 200k-cell design (--num_bin 512 --force_num_bin 1024), placer seed 0
-                       (with --run B)
+                       (with --run B; more placer seeds with --seeds 1,2,...)
     golden_runC.json   --gpu -1 run of the 2M-cell design (auto grids: 512 / 2048), placer seed 0 (with --run C)
 """
 import argparse
@@ -172,15 +172,20 @@ def full_run(tmp, seeds=(0, 1, 2, 3, 4, 5)):
         json.dump(runs, f, indent=1, sort_keys=True)
 
 
-def config_run(tmp, name):
-    """One reference run of a scale config: B at its stated grids, C at the automatic ones."""
+def config_run(tmp, name, seeds=(0,)):
+    """Reference runs of a scale config (B at its stated grids, C at the automatic ones), one per placer seed, merged
+    into the fixture's earlier seeds."""
     grids = {"B": (512, 1024), "C": (0, 0)}[name]
     lef, d = synth.generate_config(name, seed=0)
     lp, dp = write_design(lef, d, tmp, "synth_" + name)
     del lef, d
-    m = _one_run(lp, dp, 0, num_bin=grids[0], force_num_bin=grids[1], keep_pos=False)
-    with open(os.path.join(HERE, "golden_run%s.json" % name), "w") as f:
-        json.dump({"0": m}, f, indent=1, sort_keys=True)
+    path = os.path.join(HERE, "golden_run%s.json" % name)
+    for seed in seeds:
+        m = _one_run(lp, dp, seed, num_bin=grids[0], force_num_bin=grids[1], keep_pos=False)
+        runs = json.load(open(path)) if os.path.exists(path) else {}
+        runs[str(seed)] = m
+        with open(path, "w") as f:
+            json.dump(runs, f, indent=1, sort_keys=True)
 
 
 def _one_run(lp, dp, seed, num_bin=0, force_num_bin=0, keep_pos=True):
@@ -215,14 +220,15 @@ def _one_run(lp, dp, seed, num_bin=0, force_num_bin=0, keep_pos=True):
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--full-run", action="store_true")
-    p.add_argument("--run", choices=("B", "C"), help="only the reference run of this scale config")
+    p.add_argument("--run", choices=("B", "C"), help="only the reference runs of this scale config")
+    p.add_argument("--seeds", default="0", help="placer seeds of --run, comma separated")
     a = p.parse_args()
     sys.path.insert(0, REF)
     import torch
     torch.set_num_threads(8)
     with tempfile.TemporaryDirectory() as tmp:
         if a.run:
-            config_run(tmp, a.run)
+            config_run(tmp, a.run, [int(x) for x in a.seeds.split(",")])
             return
         small_golden(tmp)
         sched_golden()
