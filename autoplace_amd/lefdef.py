@@ -1,8 +1,20 @@
-"""Minimal LEF / DEF writer and reader for the placer's inputs and its gp.def output.
+"""LEF / DEF reading and writing for the placer's inputs and its gp.def output (the behaviour of the reference's
+utils/io_parser.py:22-301: what a global placer reads from LEF / DEF, and the placed DEF it writes).
 
-Reads what a global placer needs (the fields of synth.Lef / synth.Def, as the reference's utils/io_parser.py:22-261
-reads them) and writes LEF 5.8 / DEF 5.8 text that readers of either implementation accept. The writer of the
-placed DEF rewrites only the COMPONENTS statements of the moved cells (io_parser.py:264-301 behaviour).
+Read:
+  LEF  SITE sizes; per MACRO its SIZE, CLASS, ORIGIN and the centre of every PIN (the bounding box of the pin's PORT
+       shapes, RECT and POLYGON, MASK numbers skipped). PROPERTYDEFINITIONS and OBS are skipped.
+  DEF  DESIGN, UNITS, DIEAREA, ROWs; COMPONENTS (name, master, PLACED / FIXED / COVER location and orientation);
+       PINS (location of the first LAYER shape's centre, turned by the pin's orientation); NETS (the connections
+       before the first '+'); hard PLACEMENT BLOCKAGES (SOFT / PARTIAL ones are counted, not used). Every other
+       section is skipped.
+Written:
+  write_lef / write_def   a library and a design (synth.Lef / synth.Def) as LEF 5.8 / DEF 5.8 text;
+  write_placed_def        the input DEF's text with the moved components' placement statements replaced: every other
+                          statement, and every other attribute of a moved component, is copied.
+
+Both readers work on tokens: the DEF statement by statement (text between ';'), carrying which section it is in;
+the LEF as nested blocks (SITE / MACRO / PIN / PORT ... END).
 """
 import re
 
@@ -10,7 +22,16 @@ import numpy as np
 
 from .synth import Def, Lef, Macro
 
+_COMMENT = re.compile(r"#[^\n]*")
+_TOKEN = re.compile(r"[;()]|[^\s;()]+")
+_PLACEMENT = ("PLACED", "FIXED", "COVER")
 
+# a design pin's shape point (x, y), relative to its placement point, in each orientation: (x, y) -> M (x, y)
+_PIN_TURN = {"N": ((1, 0), (0, 1)), "S": ((-1, 0), (0, -1)), "W": ((0, -1), (1, 0)), "E": ((0, 1), (-1, 0)),
+             "FN": ((-1, 0), (0, 1)), "FS": ((1, 0), (0, -1)), "FW": ((0, 1), (1, 0)), "FE": ((0, -1), (-1, 0))}
+
+
+# ------------------------------------------------------------------------------------------------------ writing
 def write_lef(lef, path, dbu=1000):
     out = ["VERSION 5.8 ;", "BUSBITCHARS \"[]\" ;", "DIVIDERCHAR \"/\" ;", "UNITS", "  DATABASE MICRONS %d ;" % dbu,
            "END UNITS", ""]
@@ -52,6 +73,11 @@ def write_def(d, path, comp_xy=None, moved=None):
         else:
             lines.append("- %s %s + UNPLACED ;" % (name, m))
     lines += ["END COMPONENTS", ""]
+    if len(d.blockages):
+        lines.append("BLOCKAGES %d ;" % len(d.blockages))
+        for x0, y0, x1, y1 in d.blockages:
+            lines.append("- PLACEMENT RECT ( %d %d ) ( %d %d ) ;" % (x0, y0, x1, y1))
+        lines += ["END BLOCKAGES", ""]
     if len(d.io_name):
         lines.append("PINS %d ;" % len(d.io_name))
         for k, name in enumerate(d.io_name):
@@ -74,157 +100,306 @@ def write_def(d, path, comp_xy=None, moved=None):
         f.write("\n".join(lines) + "\n")
 
 
+def write_placed_def(d, path, comp_index, xy, orient="N"):
+    """gp.def: the components `comp_index` PLACED at `xy` (lower-left corners, DBU) with `orient`. A design read
+    from a file keeps its text: only the moved components' placement attributes change. A design built in memory
+    (synth) is written whole."""
+    if d.text is None:
+        write_def(d, path, xy, comp_index)
+        return
+    moved = {int(i): (int(x), int(y)) for i, (x, y) in zip(comp_index, np.asarray(xy))}
+    start, end = d.comp_span
+    body, k = [], 0
+    for stmt in d.text[start:end].split(";"):
+        toks = _TOKEN.findall(stmt)
+        if len(toks) < 3 or toks[0] != "-":
+            if stmt.strip():
+                body.append(stmt + ";")
+            continue
+        if k in moved:
+            rest = _without_placement(toks[3:])
+            body.append("\n- %s %s%s + PLACED ( %d %d ) %s ;"
+                        % (toks[1], toks[2], "".join(" " + t for t in rest), moved[k][0], moved[k][1], orient))
+        else:
+            body.append(stmt + ";")
+        k += 1
+    with open(path, "w") as f:
+        f.write(d.text[:start])
+        f.write("".join(body))
+        f.write("\n")
+        f.write(d.text[end:])
+
+
+def _without_placement(attrs):
+    """A component's attributes without its '+ UNPLACED' / '+ PLACED|FIXED|COVER ( x y ) orient' group."""
+    out, i = [], 0
+    while i < len(attrs):
+        if attrs[i] == "+" and i + 1 < len(attrs) and attrs[i + 1] == "UNPLACED":
+            i += 2
+        elif attrs[i] == "+" and i + 1 < len(attrs) and attrs[i + 1] in _PLACEMENT:
+            i += 7                                   # + STATUS ( x y ) orient
+        else:
+            out.append(attrs[i])
+            i += 1
+    return out
+
+
+# -------------------------------------------------------------------------------------------------------- LEF
+def _numbers(toks):
+    """The coordinates of a shape statement's tokens (a MASK number skipped)."""
+    out, skip = [], False
+    for t in toks:
+        if skip:
+            skip = False
+        elif t == "MASK":
+            skip = True
+        elif t != "ITERATE":
+            out.append(float(t))
+    return out
+
+
+class _LefBlocks(object):
+    """Nested-block reader over one LEF file's tokens."""
+
+    def __init__(self, toks):
+        self.t, self.i = toks, 0
+
+    def statement(self):
+        """Tokens up to the next ';' (consumed), the keyword first."""
+        j = self.t.index(";", self.i)
+        out = self.t[self.i:j]
+        self.i = j + 1
+        return out
+
+    def at_end_of(self, name):
+        return self.t[self.i] == "END" and self.i + 1 < len(self.t) and self.t[self.i + 1] == name
+
+    def skip_to_end(self, name):
+        while not self.at_end_of(name):
+            self.i += 1
+        self.i += 2
+
+    def site(self, name):
+        size = None
+        while not self.at_end_of(name):
+            st = self.statement()
+            if st[0] == "SIZE":
+                size = (float(st[1]), float(st[3]))
+        self.i += 2
+        return size
+
+    def shapes(self):
+        """A PORT block: the coordinates of its shapes, up to its bare END."""
+        xs, ys = [], []
+        while self.t[self.i] != "END":
+            st = self.statement()
+            if st[0] in ("RECT", "POLYGON"):
+                v = _numbers(st[1:])
+                xs += v[0::2]
+                ys += v[1::2]
+        self.i += 1
+        return xs, ys
+
+    def pin(self, name):
+        xs, ys = [], []
+        while not self.at_end_of(name):
+            if self.t[self.i] == "PORT":
+                self.i += 1
+                px, py = self.shapes()
+                xs += px
+                ys += py
+            else:
+                self.statement()
+        self.i += 2
+        return xs, ys
+
+    def macro(self, name):
+        size, origin, cls, pins = None, (0.0, 0.0), "CORE", {}
+        while not self.at_end_of(name):
+            kw = self.t[self.i]
+            if kw == "PIN":
+                pin = self.t[self.i + 1]
+                self.i += 2
+                xs, ys = self.pin(pin)
+                if xs:
+                    pins[pin] = ((min(xs) + max(xs)) / 2 + origin[0], (min(ys) + max(ys)) / 2 + origin[1])
+            elif kw == "OBS":
+                self.i += 1
+                self.shapes()
+            else:
+                st = self.statement()
+                if kw == "SIZE":
+                    size = (float(st[1]), float(st[3]))
+                elif kw == "ORIGIN":
+                    origin = (float(st[1]), float(st[2]))
+                elif kw == "CLASS":
+                    cls = " ".join(st[1:]).upper()
+        self.i += 2
+        return None if size is None else Macro(size[0], size[1], cls, pins)
+
+
 def parse_lef(paths):
-    """-> Lef(sites, macros): site sizes; per macro size, class, pin centres (bounding box of the pin's shapes)."""
+    """Read LEF files (technology and cell libraries, any order). -> Lef(sites, macros)."""
     sites, macros = {}, {}
     for path in paths:
         with open(path) as f:
-            t = re.sub(r"#[^\n]*", "", f.read()).replace(";", " ; ").split()
-        i, n = 0, len(t)
-        while i < n:
-            if t[i] == "SITE" and i + 2 < n and t[i + 2] != ";":
-                name = t[i + 1]
-                while not (t[i] == "END" and t[i + 1] == name):
-                    if t[i] == "SIZE":
-                        sites[name] = (float(t[i + 1]), float(t[i + 3]))
-                    i += 1
-                i += 2
-                continue
-            if t[i] == "MACRO":
-                name, size, origin, cls, pins = t[i + 1], None, (0.0, 0.0), "CORE", {}
-                i += 2
-                while not (t[i] == "END" and t[i + 1] == name):
-                    if t[i] == "SIZE":
-                        size = (float(t[i + 1]), float(t[i + 3]))
-                    elif t[i] == "ORIGIN":
-                        origin = (float(t[i + 1]), float(t[i + 2]))
-                    elif t[i] == "CLASS":
-                        j = t.index(";", i)
-                        cls = " ".join(t[i + 1:j]).upper()
-                    elif t[i] == "OBS":
-                        while t[i] != "END":
-                            i += 1
-                    elif t[i] == "PIN":
-                        pin, xs, ys, depth = t[i + 1], [], [], 0
-                        i += 2
-                        while depth or not (t[i] == "END" and t[i + 1] == pin):
-                            if t[i] == "PORT":
-                                depth = 1
-                            elif t[i] == "END":
-                                depth = 0
-                            elif t[i] in ("RECT", "POLYGON"):
-                                j = t.index(";", i)
-                                v = [float(s) for s in t[i + 1:j] if s[0] in "-.0123456789"]
-                                xs += v[0::2]
-                                ys += v[1::2]
-                                i = j
-                            i += 1
-                        if xs:
-                            pins[pin] = ((min(xs) + max(xs)) / 2 + origin[0], (min(ys) + max(ys)) / 2 + origin[1])
-                        i += 1
-                    i += 1
-                if size:
-                    macros[name] = Macro(size[0], size[1], cls, pins)
-                i += 2
-                continue
-            i += 1
+            toks = _TOKEN.findall(_COMMENT.sub("", f.read()))
+        r = _LefBlocks(toks)
+        while r.i < len(toks):
+            kw = toks[r.i]
+            name = toks[r.i + 1] if r.i + 1 < len(toks) else ""
+            if kw == "PROPERTYDEFINITIONS":
+                r.i += 1
+                r.skip_to_end("PROPERTYDEFINITIONS")
+            elif kw == "SITE" and r.i + 2 < len(toks) and toks[r.i + 2] != ";":
+                r.i += 2
+                size = r.site(name)
+                if size is not None:
+                    sites[name] = size
+            elif kw == "MACRO":
+                r.i += 2
+                m = r.macro(name)
+                if m is not None:
+                    macros[name] = m
+            else:
+                r.i += 1
     return Lef(sites, macros)
 
 
-def _section(text, name):
-    head = text.find("\n%s " % name)
-    if head < 0:
-        return None
-    start = text.index(";", head) + 1
-    end = text.find("\nEND %s" % name, start)
-    return start, (end if end >= 0 else len(text))
+# -------------------------------------------------------------------------------------------------------- DEF
+class _DefBuilder(object):
+    """The fields of a Def, filled statement by statement."""
+
+    def __init__(self, d):
+        self.d = d
+        self.macro_id = {}
+        self.macro, self.fixed, self.placed, self.xy = [], [], [], []
+        self.io_xy = []
+        self.net_start, self.pin_comp, self.pin_name, self.name_id = [0], [], [], {"": 0}
+        self.comp_id = self.io_id = None
+
+    def header(self, toks):
+        d, kw = self.d, toks[0]
+        if kw == "DESIGN" and len(toks) > 1:
+            d.design = toks[1]
+        elif kw == "UNITS" and len(toks) >= 4:
+            d.dbu = int(toks[3])
+        elif kw == "DIEAREA":
+            v = [int(t) for t in toks[1:] if t not in "()"]
+            d.die = (min(v[0::2]), min(v[1::2]), max(v[0::2]), max(v[1::2]))
+        elif kw == "ROW" and len(toks) >= 6:
+            site, x, y = toks[2], int(toks[3]), int(toks[4])
+            nx = ny = 1
+            sx = sy = 0
+            if "DO" in toks:
+                k = toks.index("DO")
+                nx, ny = int(toks[k + 1]), int(toks[k + 3])
+                if "STEP" in toks:
+                    k = toks.index("STEP")
+                    sx, sy = int(toks[k + 1]), int(toks[k + 2])
+            d.rows.append((site, x, y, nx, ny, sx, sy))
+
+    def component(self, toks):
+        self.d.comp_name.append(toks[1])
+        self.macro.append(self.macro_id.setdefault(toks[2], len(self.macro_id)))
+        status, x, y, orient = None, 0, 0, "N"
+        for k in range(3, len(toks) - 6):
+            if toks[k] == "+" and toks[k + 1] in _PLACEMENT:
+                status, x, y, orient = toks[k + 1], int(toks[k + 3]), int(toks[k + 4]), toks[k + 6]
+                break
+        self.fixed.append(status in ("FIXED", "COVER"))
+        self.placed.append(status is not None)
+        self.xy.append((x, y))
+        self.d.comp_orient.append(orient)
+
+    def design_pin(self, toks):
+        shape = where = None
+        for k in range(2, len(toks)):
+            if toks[k] == "LAYER" and shape is None:
+                p = toks.index("(", k)
+                shape = (int(toks[p + 1]), int(toks[p + 2]), int(toks[p + 5]), int(toks[p + 6]))
+            elif toks[k] in _PLACEMENT and toks[k - 1] == "+" and where is None:
+                where = (int(toks[k + 2]), int(toks[k + 3]), toks[k + 5])
+        self.d.io_name.append(toks[1])
+        if where is None:
+            self.io_xy.append((np.nan, np.nan))
+            return
+        cx, cy = ((shape[0] + shape[2]) / 2.0, (shape[1] + shape[3]) / 2.0) if shape else (0.0, 0.0)
+        (a, b), (c, e) = _PIN_TURN[where[2]]
+        self.io_xy.append((where[0] + (a * cx + b * cy), where[1] + (c * cx + e * cy)))
+
+    def net(self, toks):
+        if self.comp_id is None:
+            self.comp_id = {n: i for i, n in enumerate(self.d.comp_name)}
+            self.io_id = {n: i for i, n in enumerate(self.d.io_name)}
+        k = 2
+        while k < len(toks) and toks[k] == "(":
+            close = toks.index(")", k)
+            who, what = toks[k + 1], toks[k + 2]
+            if who == "PIN":
+                self.pin_comp.append(-1 - self.io_id[what])
+                self.pin_name.append(0)
+            else:
+                self.pin_comp.append(self.comp_id[who])
+                self.pin_name.append(self.name_id.setdefault(what, len(self.name_id)))
+            k = close + 1
+        self.d.net_name.append(toks[1])
+        self.net_start.append(len(self.pin_comp))
+
+    def blockage(self, toks):
+        if toks[1] != "PLACEMENT":
+            return
+        if "SOFT" in toks or "PARTIAL" in toks:
+            self.d.soft_blockages += 1
+            return
+        for k in range(2, len(toks) - 8):
+            if toks[k] == "RECT":
+                self.d.blockages.append((int(toks[k + 2]), int(toks[k + 3]), int(toks[k + 6]), int(toks[k + 7])))
+
+    def finish(self):
+        d = self.d
+        d.macro_names = list(self.macro_id)
+        d.comp_macro_id = np.array(self.macro, dtype=np.int64)
+        d.comp_fixed = np.array(self.fixed, dtype=bool)
+        d.comp_placed = np.array(self.placed, dtype=bool)
+        d.comp_xy = np.array(self.xy, dtype=np.int64).reshape(-1, 2)
+        d.io_xy = np.array(self.io_xy, dtype=np.float64).reshape(-1, 2)
+        d.pin_names = list(self.name_id)
+        d.pin_name = np.array(self.pin_name, dtype=np.int64)
+        d.net_start = np.array(self.net_start, dtype=np.int64)
+        d.pin_comp = np.array(self.pin_comp, dtype=np.int64)
+        return d
+
+
+_SECTION_HANDLER = {"COMPONENTS": "component", "PINS": "design_pin", "NETS": "net", "BLOCKAGES": "blockage"}
 
 
 def parse_def(path):
-    """-> Def with the fields of synth.Def."""
+    """Read a DEF. -> Def (the fields of synth.Def, with the text kept for write_placed_def)."""
     d = Def()
     d.path = path
     with open(path) as f:
-        text = f.read()
-    span = _section(text, "COMPONENTS")
-    head = text[:span[0]] if span else text
-    m = re.search(r"^[ \t]*DESIGN[ \t]+(\S+)", head, re.M)
-    d.design = m.group(1) if m else ""
-    m = re.search(r"UNITS\s+DISTANCE\s+MICRONS\s+(\d+)", head)
-    d.dbu = int(m.group(1)) if m else 1000
-    m = re.search(r"^[ \t]*DIEAREA([^;]*);", head, re.M)
-    if m:
-        v = [int(s) for s in re.findall(r"-?\d+", m.group(1))]
-        d.die = (min(v[0::2]), min(v[1::2]), max(v[0::2]), max(v[1::2]))
-    for m in re.finditer(r"^[ \t]*ROW\s+\S+\s+(\S+)\s+(-?\d+)\s+(-?\d+)\s+\S+"
-                         r"(?:\s+DO\s+(\d+)\s+BY\s+(\d+)(?:\s+STEP\s+(-?\d+)\s+(-?\d+))?)?", head, re.M):
-        g = m.groups()
-        d.rows.append((g[0], int(g[1]), int(g[2]), int(g[3] or 1), int(g[4] or 1), int(g[5] or 0), int(g[6] or 0)))
-    fixed, placed, xy, macro, macro_id = [], [], [], [], {}
-    if span:
-        for stmt in text[span[0]:span[1]].split(";"):
-            t = stmt.split()
-            if len(t) < 3 or t[0] != "-":
-                continue
-            d.comp_name.append(t[1])
-            macro.append(macro_id.setdefault(t[2], len(macro_id)))
-            status, x, y, orient = "", 0, 0, "N"
-            for i in range(3, len(t) - 6):
-                if t[i] == "+" and t[i + 1] in ("PLACED", "FIXED", "COVER"):
-                    status, x, y, orient = t[i + 1], int(t[i + 3]), int(t[i + 4]), t[i + 6]
-                    break
-            fixed.append(status in ("FIXED", "COVER"))
-            placed.append(status != "")
-            xy.append((x, y))
-            d.comp_orient.append(orient)
-    d.macro_names, d.comp_macro_id = list(macro_id), np.array(macro, dtype=np.int64)
-    d.comp_fixed = np.array(fixed, dtype=bool)
-    d.comp_placed = np.array(placed, dtype=bool)
-    d.comp_xy = np.array(xy, dtype=np.int64).reshape(-1, 2)
-    span = _section(text, "PINS")
-    io_xy = []
-    if span:
-        for stmt in text[span[0]:span[1]].split(";"):
-            t = stmt.split()
-            if len(t) < 2 or t[0] != "-":
-                continue
-            rect, place = None, None
-            for i in range(2, len(t)):
-                if t[i] == "LAYER" and rect is None:
-                    j = t.index("(", i)
-                    rect = (int(t[j + 1]), int(t[j + 2]), int(t[j + 5]), int(t[j + 6]))
-                elif t[i] in ("PLACED", "FIXED", "COVER") and t[i - 1] == "+" and place is None:
-                    place = (int(t[i + 2]), int(t[i + 3]), t[i + 5])
-            d.io_name.append(t[1])
-            if place is None:
-                io_xy.append((np.nan, np.nan))
-            else:
-                cx, cy = ((rect[0] + rect[2]) / 2.0, (rect[1] + rect[3]) / 2.0) if rect else (0.0, 0.0)
-                if place[2] != "N":
-                    raise ValueError("design pin orientation %s not supported by this reader" % place[2])
-                io_xy.append((place[0] + cx, place[1] + cy))
-    d.io_xy = np.array(io_xy, dtype=np.float64).reshape(-1, 2)
-    span = _section(text, "NETS")
-    net_start, pin_comp, pin_name, name_id = [0], [], [], {"": 0}
-    pat = re.compile(r"\(\s*(\S+)\s+(\S+)[^)]*\)")
-    if span:
-        comp_id = {n: i for i, n in enumerate(d.comp_name)}
-        io_id = {n: i for i, n in enumerate(d.io_name)}
-        for stmt in text[span[0]:span[1]].split(";"):
-            t = stmt.split(None, 2)
-            if len(t) < 2 or t[0] != "-":
-                continue
-            for a, b in pat.findall(stmt):
-                if a == "PIN":
-                    pin_comp.append(-1 - io_id[b])
-                    pin_name.append(0)
+        d.text = f.read()
+    b = _DefBuilder(d)
+    section, pos, comp_start = None, 0, None
+    for stmt in d.text.split(";"):
+        toks = _TOKEN.findall(_COMMENT.sub("", stmt))
+        while len(toks) >= 2 and toks[0] == "END":      # 'END SECTION' has no ';': it opens the next statement
+            if toks[1] == "COMPONENTS" and comp_start is not None:
+                d.comp_span = (comp_start, pos + stmt.index("END"))
+            section = None
+            toks = toks[2:]
+        if toks:
+            if section is None:
+                if toks[0] == "PROPERTYDEFINITIONS" or (len(toks) == 2 and toks[1].isdigit()):
+                    section = toks[0]                   # COMPONENTS 12, NETS 40, VIAS 3, ... or PROPERTYDEFINITIONS
+                    if section == "COMPONENTS":
+                        comp_start = pos + len(stmt) + 1
                 else:
-                    pin_comp.append(comp_id[a])
-                    pin_name.append(name_id.setdefault(b, len(name_id)))
-            d.net_name.append(t[1])
-            net_start.append(len(pin_comp))
-    d.pin_names = list(name_id)
-    d.pin_name = np.array(pin_name, dtype=np.int64)
-    d.net_start = np.array(net_start, dtype=np.int64)
-    d.pin_comp = np.array(pin_comp, dtype=np.int64)
-    return d
+                    b.header(toks)
+            elif toks[0] == "-" and section in _SECTION_HANDLER and len(toks) >= 2:
+                getattr(b, _SECTION_HANDLER[section])(toks)
+        pos += len(stmt) + 1
+    if comp_start is not None and d.comp_span is None:
+        d.comp_span = (comp_start, len(d.text))
+    return b.finish()

@@ -52,6 +52,8 @@ class Def(object):
         self.pin_name = None             # int64 [pins]
         self.blockages = []
         self.soft_blockages = 0
+        self.text = None                 # the DEF's text, when read from a file (the placed DEF rewrites it)
+        self.comp_span = None            # (start, end) of the COMPONENTS statements in `text`
 
 
 CONFIGS = {
