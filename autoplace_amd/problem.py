@@ -13,7 +13,8 @@ from .resolution import force_num_bin
 
 def default_args(**kw):
     """The reference's option defaults (main.py:19-47) as a namespace; keywords override."""
-    a = SimpleNamespace(seed=0, target_density=1.0, num_bin=0, force_num_bin=0, stop_overflow=0.07, inner_iter=10000,
+    a = SimpleNamespace(lef=[], def_file="", out_dir="", eval_def="", write_def=True, gpu=0,
+                        seed=0, target_density=1.0, num_bin=0, force_num_bin=0, stop_overflow=0.07, inner_iter=10000,
                         settle_iter=30, stall_iter=200, min_iter=100, ignore_net_degree=100, start_tol=1e-3,
                         density_weight=1e-3, wa_coeff=4.0, wa_rule="accuracy", cluster_ratio=5.0, lr=0.01,
                         log_freq=100)

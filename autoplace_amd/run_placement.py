@@ -12,6 +12,7 @@ from . import gpu_loop
 from .calculator import Objective
 from .cluster import coarse_problem, expand
 from .density import ElectronicDensityLayer
+from .initializer import choose_num_bin
 from .nesterov_optimizer import NesterovOptimizer, estimate_initial_learning_rate
 from .param_scheduler import ParamScheduler
 from .problem import build_problem
@@ -162,3 +163,81 @@ def place(data, args, device="cuda", log=None, opts=None):
         raise RuntimeError("the placement is not finite (diverged before any usable point)")
     metrics.update(evaluate_placement(pos, wirelength, density, data))
     return pos, metrics
+
+
+def evaluate_def(data, placed_def, args, device="cuda"):
+    """Only evaluate: the cells where another DEF of the same netlist places them (run_placement.py:145-152,175-176):
+    exact HPWL over every net (FP64 device sum) and exact overflow (real sizes, no fillers) on the overflow grid.
+    -> (positions [num_mov, 2] on the device, metrics dict)."""
+    device = torch.device(device)
+    num_bin = choose_num_bin(data, args.num_bin)
+    wirelength = Wirelength(data.pin_cell, data.pin_off, data.pin_net, data.num_nets, data.num_mov,
+                            args.ignore_net_degree, device)
+    density = ElectronicDensityLayer(data.width, data.height, num_bin, num_bin, data.fixed_fraction(num_bin, num_bin),
+                                     args.target_density, data.size, data.num_mov, device, field=False)
+    pos = torch.tensor(data.placed_pos(placed_def), dtype=torch.float32, device=device)
+    metrics = {"design": data.name, "num_movable": data.num_mov, "num_nets": data.num_nets,
+               "num_pins": data.num_pins, "num_fillers": 0, "num_bin": num_bin, "force_num_bin": num_bin,
+               "target_density": args.target_density, "device": str(device), "evaluated": placed_def.path}
+    metrics.update(evaluate_placement(pos, wirelength, density, data))
+    return pos, metrics
+
+
+def run_placement_main(args, log=None, opts=None):
+    """The run from files (run_placement.py:120-198): read the LEF(s) and the DEF, place -- or, with args.eval_def,
+    only evaluate that placed DEF --, and with args.out_dir write gp.def (args.write_def) and metrics.json there.
+
+    args: the reference's options (main.py:19-47): lef (list), def_file, out_dir, eval_def, write_def, gpu and the
+    placement options of problem.default_args. The placement runs on the HIP device args.gpu; there is no CPU path
+    (gpu < 0 is an error). -> metrics dict."""
+    import json
+    import os
+
+    from . import lefdef
+    from .database import PlaceData
+
+    log = gpu_loop.as_log(log)
+    if getattr(args, "gpu", 0) < 0:
+        raise RuntimeError("this placer runs on a HIP device: --gpu %d (the reference's CPU path) is not part of it"
+                           % args.gpu)
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device: the placer's kernels need one")
+    start = time.time()
+    device = torch.device("cuda:%d" % getattr(args, "gpu", 0))
+    torch.cuda.set_device(device)
+    design = lefdef.parse_def(args.def_file)
+    data = PlaceData(lefdef.parse_lef(args.lef), design)
+    parse_time = time.time() - start
+    if log:
+        log("design %s: %d movable, %d fixed, %d nets, %d pins, %d rows, %d blockages (read in %.2fs)"
+            % (data.name, data.num_mov, int(data.fixed.sum()), data.num_nets, data.num_pins, len(design.rows),
+               len(design.blockages), parse_time))
+        if design.soft_blockages:
+            log("warning: %d SOFT / PARTIAL placement blockages are not used" % design.soft_blockages)
+        if data.unplaced_io_pins:
+            log("warning: %d connections to design pins without a location are not used" % data.unplaced_io_pins)
+    evaluate_only = bool(getattr(args, "eval_def", ""))
+    if evaluate_only:
+        pos, metrics = evaluate_def(data, lefdef.parse_def(args.eval_def), args, device)
+    else:
+        pos, metrics = place(data, args, device, log, opts)
+        if metrics["stop_reason"] != "converged" and log:
+            log("warning: global placement ended '%s', not converged (stop overflow %.3f)"
+                % (metrics["stop_reason"], args.stop_overflow))
+    metrics["parse_time_s"] = parse_time
+    metrics.pop("cluster_ids", None)
+    if log:
+        log("exact HPWL: %.6E sites = %.1f um, exact overflow: %.4f"
+            % (metrics["hpwl_sites"], metrics["hpwl_um"], metrics["overflow"]))
+    out_dir = getattr(args, "out_dir", "")
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        if not evaluate_only and getattr(args, "write_def", True):
+            t = time.time()
+            lefdef.write_placed_def(design, os.path.join(out_dir, "gp.def"), data.mov_index,
+                                    data.to_def_xy(pos[:data.num_mov].cpu().numpy()))
+            metrics["write_time_s"] = time.time() - t
+        metrics["total_time_s"] = time.time() - start
+        with open(os.path.join(out_dir, "metrics.json"), "w") as f:
+            json.dump({k: v for k, v in metrics.items() if isinstance(v, (int, float, str, bool, list))}, f, indent=2)
+    return metrics

@@ -79,3 +79,40 @@ def test_autotuned_run_ends_where_the_plain_run_ends(design_a, runs):
                                                                         ref["hpwl_sites"]))
     assert np.array_equal(pos.cpu().numpy(), ref_pos)
     assert m["iters"] == ref["iters"] and m["hpwl_sites"] == ref["hpwl_sites"]
+
+
+def test_run_from_files_writes_gp_def_and_evaluates_it(design_a, runs):
+    """run_placement_main from LEF / DEF files (run_placement.py:120-198): the same placement as the in-memory run,
+    gp.def and metrics.json written; evaluating gp.def (eval_def) gives the placement's exact HPWL and overflow up
+    to the DEF's rounding to whole database units, and its FP64 HPWL matches the oracle's on the read positions."""
+    import tempfile
+
+    from autoplace_amd import lefdef, synth
+    from autoplace_amd.problem import default_args
+    from autoplace_amd.run_placement import run_placement_main
+    from oracle import wa as owa
+    lef, d = synth.generate_config("A", seed=0)
+    with tempfile.TemporaryDirectory() as tmp:
+        lp, dp = os.path.join(tmp, "a.lef"), os.path.join(tmp, "a.def")
+        lefdef.write_lef(lef, lp)
+        lefdef.write_def(d, dp)
+        out = os.path.join(tmp, "out")
+        m = run_placement_main(default_args(lef=[lp], def_file=dp, out_dir=out, seed=0))
+        assert os.path.exists(os.path.join(out, "gp.def"))
+        saved = json.load(open(os.path.join(out, "metrics.json")))
+        assert saved["hpwl_sites"] == m["hpwl_sites"]
+        ref_pos, ref = runs[0]
+        assert m["hpwl_sites"] == ref["hpwl_sites"] and m["iters"] == ref["iters"]
+        e = run_placement_main(default_args(lef=[lp], def_file=dp, eval_def=os.path.join(out, "gp.def")))
+        placed = lefdef.parse_def(os.path.join(out, "gp.def"))
+        from autoplace_amd.database import PlaceData
+        data = PlaceData(lefdef.parse_lef([lp]), lefdef.parse_def(dp))
+        pos = data.placed_pos(placed)
+    print("gp.def evaluated: HPWL %.6e (placement %.6e), overflow %.4f (placement %.4f)"
+          % (e["hpwl_sites"], m["hpwl_sites"], e["overflow"], m["overflow"]))
+    assert abs(e["hpwl_sites"] / m["hpwl_sites"] - 1) < 1e-4
+    assert abs(e["overflow"] - m["overflow"]) < 5e-3
+    assert np.abs(pos - ref_pos[:data.num_mov]).max() < 0.01          # half a DBU is 0.0025 sites
+    h64 = owa.hpwl_exact(pos.astype(np.float32), data.pin_cell, data.pin_off, data.pin_net, data.num_nets,
+                         data.num_mov)
+    assert abs(e["hpwl_sites"] - h64) / h64 < 1e-6

@@ -173,3 +173,13 @@ def test_placedata_from_the_handwritten_design():
     assert np.allclose(data.pin_off[k], [(20000 + 6000) / 200.0, (4000 + 1500) / 200.0])
     # the design pin in1 (no cell): its absolute position in sites
     assert data.pin_cell[0] == data.num_mov and np.allclose(data.pin_off[0], [1.0, 45.0])
+
+
+def test_no_cpu_path():
+    """gpu = -1 asks for the reference's CPU path, which this placer does not have: a clear error, not a fallback."""
+    import pytest
+
+    from autoplace_amd.problem import default_args
+    from autoplace_amd.run_placement import run_placement_main
+    with pytest.raises(RuntimeError, match="HIP device"):
+        run_placement_main(default_args(gpu=-1, lef=["x.lef"], def_file="x.def"))
