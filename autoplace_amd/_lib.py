@@ -29,6 +29,7 @@ _SIGNATURES = {
     "ap_control": ([P, P, P, P, P, P, P, P, P], I),
     "ap_select_step": ([P, P, P, P, P, P, P, P, I, P], I),
     "ap_net_hpwl": ([P, P, P, P, I, I, P, P, P], I),
+    "ap_net_hpwl_noalloc": ([P, P, P, P, I, I, P, P], I),
     "ap_poisson_create": ([I, I, D, D, I, ctypes.POINTER(P)], I),
     "ap_poisson_destroy": ([P], I),
     "ap_poisson_method": ([P], I),

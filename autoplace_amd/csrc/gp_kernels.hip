@@ -12,6 +12,9 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <mutex>
+#include <vector>
+
 #include "gp_common.h"
 #include "gp_control.h"
 #include "gp_kernels.h"
@@ -393,6 +396,66 @@ int ap_net_hpwl(const float* pos, const int* pin_cell, const float* pin_off, con
         AP_LAUNCHED();
     }
     hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, work, nblk, out);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------ exact HPWL without an allocation
+// ap_net_hpwl_noalloc keeps its FP64 workgroup partials in this file-scope device array instead of a caller's work
+// buffer: one partial per 256 nets, so at most AP_NOALLOC_HPWL_PARTIALS * 256 nets. Each device that runs the call
+// holds its own copy of the array until the process ends. In host code `scratch` only names the symbol; the device
+// address of each device's copy comes from hipGetSymbolAddress, taken once per device on its first call.
+static __device__ __align__(256) double scratch[AP_NOALLOC_HPWL_PARTIALS];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_of_device;   // device index -> address of that device's `scratch`
+
+// The current device's address of `scratch` (taken on the device's first call, with the device current).
+int scratch_address(double** out) {
+    int dev = -1;
+    AP_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_of_device.size() <= dev) g_scratch_of_device.resize(dev + 1, nullptr);
+    if (!g_scratch_of_device[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("net_hpwl_noalloc: no device address of the scratch array on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return AP_ERR_HIP;
+        }
+        g_scratch_of_device[dev] = (double*)p;
+    }
+    *out = g_scratch_of_device[dev];
+    return AP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ap_net_hpwl_noalloc(const float* pos, const int* pin_cell, const float* pin_off, const int* net_start,
+                        int num_nets, int num_mov, double* out, void* stream) {
+    AP_REQUIRE(num_nets >= 0 && out && (num_nets == 0 || (net_start && pos && pin_cell && pin_off)),
+               "net_hpwl_noalloc: bad arguments");
+    const long long nblk = num_nets > 0 ? (long long)grid_of(num_nets, 256) : 0;
+    AP_REQUIRE(nblk <= AP_NOALLOC_HPWL_PARTIALS,
+               "net_hpwl_noalloc: %d nets need %lld workgroup partials, above its fixed scratch of %d (at most %lld "
+               "nets): use ap_net_hpwl with a work buffer",
+               num_nets, nblk, AP_NOALLOC_HPWL_PARTIALS, (long long)AP_NOALLOC_HPWL_PARTIALS * 256);
+    double* work = nullptr;
+    const int st = scratch_address(&work);
+    if (st) return st;
+    hipStream_t s = (hipStream_t)stream;
+    // every partial the sum reads is written by its workgroup first: nothing to clear between calls
+    if (nblk) {
+        hipLaunchKernelGGL(net_hpwl_kernel, dim3((unsigned)nblk), dim3(256), 0, s, (const float2*)pos, pin_cell,
+                           (const float2*)pin_off, net_start, num_nets, num_mov, work);
+        AP_LAUNCHED();
+    }
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, work, (int)nblk, out);
     AP_LAUNCHED();
     return AP_OK;
 }

@@ -75,6 +75,15 @@ AP_API int ap_select_step(const double* S, float* u, float* v, float* g, const f
 AP_API int ap_net_hpwl(const float* pos, const int* pin_cell, const float* pin_off, const int* net_start, int num_nets,
                        int num_mov, double* work, double* out, void* stream);
 
+/* ap_net_hpwl without a work buffer and without any allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Its partials live in a fixed device array of AP_NOALLOC_HPWL_PARTIALS doubles (512 KiB per device, held until the
+ * process ends), so num_nets is at most AP_NOALLOC_HPWL_PARTIALS * 256 (16,777,216); above that it returns
+ * AP_ERR_ARG. Calls on one device share that array: calls that overlap on one device (concurrent streams) race. */
+#define AP_NOALLOC_HPWL_PARTIALS 65536
+AP_API int ap_net_hpwl_noalloc(const float* pos, const int* pin_cell, const float* pin_off, const int* net_start,
+                               int num_nets, int num_mov, double* out, void* stream);
+
 /* ---------------------------------------------------------------- part 2: Poisson solve */
 
 #define AP_POISSON_AUTO 0
