@@ -64,6 +64,13 @@ _SIGNATURES = {
     "ap_batch_last_ms": ([P, ctypes.POINTER(F)], I),
     "ap_batch_autotune": ([P, I, P], I),
     "ap_batch_tile_layout": ([P, P], I),
+    "ap_cluster_create": ([I, P, P, I, P, I, P, ctypes.POINTER(P)], I),
+    "ap_cluster_round": ([P, P, F, ctypes.c_longlong, I, ctypes.POINTER(I), P], I),
+    "ap_cluster_num": ([P], I),
+    "ap_cluster_edges": ([P], I),
+    "ap_cluster_result": ([P, P, P, P], I),
+    "ap_cluster_destroy": ([P], I),
+    "ap_free_fraction": ([P, I, P, I, P, D, D, D, D, I, I, P, P], I),
 }
 
 

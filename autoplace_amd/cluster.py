@@ -1,134 +1,141 @@
-"""Two-level placement: cells merged into clusters, the clusters placed first, then every cell starts inside its
-cluster (src/cluster.py:1-144 of the reference).
+"""Two-level placement: the cells merged into clusters, the clusters placed first, every cell then started inside its
+cluster (the behaviour of src/cluster.py:1-144 of the reference).
 
-Clustering on the device, in rounds of star matching: half of the clusters, drawn from torch's CPU generator, are
-heads; every other cluster joins the head it shares the most net weight with per unit of merged area (a net of d pins
-gives 420 / (d - 1) to each pair of its cells, an integer for d <= 8), a head taking at most `per_head`. Weights and
-areas are integers, so the same input and seed give the same clusters on any device.
+Clustering runs in the library's kernels (ap_cluster_*, include/autoplace_gp.h part 5): the weighted cell graph of
+the small nets, then rounds of star matching. The reference draws each round's heads from torch's CPU generator, so
+they are drawn here the same way and handed to the device; weights and areas are integers and a score is one IEEE
+division, so the clusters are the reference's bit for bit. The coarse problem -- cluster sizes, the clusters' pins
+and netlist CSR, the free-area fraction of its grid and its density layer -- is built from device tensors, inside the
+GP clock as the reference builds it, without a round trip through the host.
 """
-import math
+import ctypes
 from types import SimpleNamespace
 
-import numpy as np
 import torch
 
+from . import _lib
 from .density import ElectronicDensityLayer
 from .resolution import force_num_bin
 from .wirelength import Wirelength
 
-
-def _merge_edges(parts, n):
-    """[(i, j, w)] -> one edge per (i, j), sorted by (i, j), weights summed; int32 out."""
-    key = torch.cat([a.long() * n + b.long() for a, b, _ in parts])
-    w = torch.cat([c for _, _, c in parts]).int()
-    key, inv = torch.unique(key, return_inverse=True)
-    w = torch.zeros(len(key), dtype=torch.int32, device=w.device).index_add_(0, inv, w)
-    return (key // n).int(), (key % n).int(), w
+AREA_SCALE = 1024          # areas as integers: 1 / 1024 of a site area, so sums are exact (cluster.py:94)
 
 
-def _top_per_group(group, score, limit):
-    """Indices of the `limit` highest scores of every group (ties: lower index first)."""
-    order = torch.argsort(score, descending=True, stable=True)
-    order = order[torch.argsort(group[order], stable=True)]
-    g = group[order]
-    at = torch.arange(len(order), device=group.device)
-    start = torch.ones_like(g, dtype=torch.bool)
-    start[1:] = g[1:] != g[:-1]
-    run_start = torch.cummax(torch.where(start, at, torch.zeros_like(at)), 0)[0]
-    return order[at - run_start < limit]
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
 
 
-def cluster_cells(pin_cell, pin_net, area, ratio, seed, max_degree=8, per_head=3, max_rounds=12):
-    """pin_cell / pin_net [P] grouped by net (cell len(area) = fixed pin); area [n] long.
-    -> (cluster of every cell [n] long, cluster areas [K] long)."""
-    n, device = len(area), area.device
-    pin_cell, pin_net = pin_cell.long(), pin_net.long()
-    degree = torch.bincount(pin_net)[pin_net]
-    small = degree <= max_degree
-    pin_cell, pin_net, degree = pin_cell[small], pin_net[small], degree[small]
-    i = j = w = None
-    for k in range(1, max_degree):
-        a, b, d = pin_cell[:-k], pin_cell[k:], degree[:-k]
-        m = (pin_net[:-k] == pin_net[k:]) & (a < n) & (b < n) & (a != b)
-        a, b, d = a[m], b[m], 420 // (d[m] - 1)
-        parts = [(a, b, d), (b, a, d)] + ([(i, j, w)] if i is not None else [])
-        i, j, w = _merge_edges(parts, n)
-    del pin_cell, pin_net, degree
-    cid, num = torch.arange(n, device=device), n
-    big = 20 * area.sum() // n
-    small_area = area[area < big]
-    # float32, as the tensor expression 3 * ratio * long_sum // count evaluates
-    cap = (torch.tensor(3 * ratio, dtype=torch.float32) * small_area.sum().cpu().float()) \
-        .div(float(2 * len(small_area)), rounding_mode="floor").to(device)
-    gen = torch.Generator().manual_seed(seed)
-    for _ in range(max_rounds):
-        if num * ratio <= n:
-            break
-        head = (torch.rand(num, generator=gen) < 0.5).to(device)
-        total = area[i] + area[j]
-        m = ~head[i] & head[j] & (total.float() <= cap) & (area[i] < big) & (area[j] < big)
-        tail, to, score = i[m], j[m], w[m].double() / total[m].double()
-        best = _top_per_group(tail, score, 1)
-        tail, to, score = tail[best], to[best], score[best]
-        taken = _top_per_group(to, score, per_head)
-        parent = torch.arange(num, device=device)
-        parent[tail[taken].long()] = to[taken].long()
-        kept, new = torch.unique(parent, return_inverse=True)
-        if len(kept) == num:
-            break
-        cid, num = new[cid], len(kept)
-        area = torch.zeros(num, dtype=torch.long, device=device).index_add_(0, new, area)
-        i, j = new[i.long()].int(), new[j.long()].int()
-        keep = i != j
-        i, j, w = _merge_edges([(i[keep], j[keep], w[keep])], num)
-    return cid, area
+def _stream(device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def merge_cap(area, big, ratio):
+    """The merged-area limit of two clusters: 3 ratio / 2 times the mean area of the cells below `big`, in float32
+    arithmetic as the reference's tensor expression 3 * ratio * area[area < big].sum() // count evaluates
+    (cluster.py:61). -> a float exactly representable in float32."""
+    below = area < big
+    total = area[below].sum().cpu().float()
+    return float((torch.tensor(3 * ratio, dtype=torch.float32) * total).div(float(2 * int(below.sum())),
+                                                                            rounding_mode="floor"))
+
+
+def cluster_cells(pin_cell, net_start, area, ratio, seed, max_degree=8, per_head=3, max_rounds=12):
+    """pin_cell int32 [P] grouped by net (index len(area) = fixed pin) and net_start int32 [nets + 1] on the device;
+    area int64 [n] on the device. -> (cluster of every cell int64 [n], cluster areas int64 [K]), on the device.
+    Rounds stop once the clusters are `ratio` times fewer than the cells, or when a round merges nothing."""
+    n, device = area.numel(), area.device
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    _lib.check(lib.ap_cluster_create(n, _p(pin_cell), _p(net_start), net_start.numel() - 1, _p(area), max_degree,
+                                     _stream(device), ctypes.byref(h)), "cluster_create")
+    try:
+        big = int(20 * area.sum() // n)                # cells above 20 x the mean area stay alone
+        cap = merge_cap(area, big, ratio)
+        gen = torch.Generator().manual_seed(seed)
+        num = n
+        for _ in range(max_rounds):
+            if num * ratio <= n:
+                break
+            heads = (torch.rand(num, generator=gen) < 0.5).to(torch.uint8).to(device)
+            after = ctypes.c_int()
+            _lib.check(lib.ap_cluster_round(h, _p(heads), cap, big, per_head, ctypes.byref(after), _stream(device)),
+                       "cluster_round")
+            if after.value == num:
+                break
+            num = after.value
+        cid = torch.empty(n, dtype=torch.int64, device=device)
+        carea = torch.empty(num, dtype=torch.int64, device=device)
+        _lib.check(lib.ap_cluster_result(h, _p(cid), _p(carea), _stream(device)), "cluster_result")
+    finally:
+        lib.ap_cluster_destroy(h)
+    return cid, carea
+
+
+def cluster_sizes(cid, carea, cell_size):
+    """A merged cluster is the square of its area; a cluster of one cell keeps the cell's size (cluster.py:97-102).
+    cell_size float32 [n, 2] on the device. -> (sizes float32 [K, 2], single-cell flags [K])."""
+    num = carea.numel()
+    side = (carea.double() / AREA_SCALE).sqrt().float()
+    size = torch.stack([side, side], 1)
+    single = torch.bincount(cid, minlength=num) == 1
+    member = torch.zeros(num, dtype=torch.int64, device=cid.device)
+    member.scatter_(0, cid, torch.arange(cid.numel(), device=cid.device))   # a single cluster's only cell
+    size[single] = cell_size[member[single]]
+    return size, single
+
+
+def cluster_pins(cid, single, pins):
+    """The clusters' netlist (cluster.py:103-116): every pin goes to its cell's cluster; a merged cluster keeps one
+    pin per net, the first in pin order, at its centre; single-cell clusters and fixed pins keep theirs.
+    pins: dict(cell, off, net) of every net (fixed pin = cell n). -> (owner [P'], offset float32 [P', 2], net [P'])."""
+    num = single.numel()
+    owner = torch.cat([cid, cid.new_tensor([num])])[pins["cell"]]
+    keeps_offset = torch.cat([single, single.new_ones(1)])[owner]
+    # the first pin of each (net, owner): the smallest pin index among the pins with that key
+    key = pins["net"] * (num + 1) + owner
+    uniq, which = torch.unique(key, return_inverse=True)
+    at = torch.arange(key.numel(), device=key.device)
+    first = torch.full((uniq.numel(),), key.numel(), dtype=torch.int64, device=key.device)
+    first.scatter_reduce_(0, which, at, "amin")
+    keep = keeps_offset | (first[which] == at)
+    off = torch.where(keeps_offset[keep].unsqueeze(1), pins["off"][keep], torch.zeros((), device=key.device))
+    return owner[keep], off, pins["net"][keep]
 
 
 def coarse_problem(data, node_size, wirelength, density, fixed_fraction, args, poisson="auto"):
-    """The problem of the clusters (cluster.py:88-134): cid, size, wirelength, density and the scheduler's totals.
-    A merged cluster is a square of its area with one pin per net at its centre; a single cell keeps its shape and
-    pins. Same fillers and blockages; grid from the cluster count (force_num_bin), between the overflow grid and the
-    cells' grid."""
+    """The clusters' problem (cluster.py:88-134) from the cell level's device data: cid, size, wirelength, density
+    and the scheduler's totals. The fillers and the blocked area are the cell level's; the grid follows the cluster
+    count as the cell grid follows the cells (resolution.force_num_bin), between the overflow grid and the cells'.
+    node_size / fixed_fraction: the cell level's host arrays (the device copies in `density` are used)."""
     n, device = data.num_mov, density.device
-    size = torch.tensor(node_size[:n], dtype=torch.float32, device=device)
-    scale = 1024
-    cid, area = cluster_cells(wirelength.all["cell"], wirelength.all["net"],
-                              (size.prod(1).double() * scale).round().long(), args.cluster_ratio, args.seed)
-    pins = wirelength.pins()
-    num = len(area)
-    side = (area.double() / scale).sqrt().float()
-    csize = torch.stack([side, side], 1)
-    single = torch.bincount(cid, minlength=num) == 1
-    member = torch.zeros(num, dtype=torch.long, device=device)
-    member[cid] = torch.arange(n, device=device)
-    csize[single] = size[member[single]]
-    owner = torch.cat([cid, cid.new_tensor([num])])[pins["cell"]]
-    keep_off = torch.cat([single, single.new_tensor([True])])[owner]
-    key = pins["net"] * (num + 1) + owner
-    order = torch.argsort(key, stable=True)
-    sorted_key = key[order]
-    repeat = torch.zeros_like(keep_off)
-    repeat[order[1:]] = (sorted_key[1:] == sorted_key[:-1]) & ~keep_off[order[1:]]
-    sel = ~repeat
-    off = torch.where(keep_off[sel].unsqueeze(1), pins["off"][sel], torch.zeros(1, 2, device=device))
-    c_owner, c_net = owner[sel], pins["net"][sel]
-    cwl = Wirelength(c_owner, off, c_net, pins["num_nets"], num, args.ignore_net_degree, device)   # CSR on the device
-    cpins = torch.bincount(c_owner, minlength=num + 1)[:num].double().cpu().numpy()
-    cnode = np.concatenate([csize.double().cpu().numpy(), node_size[n:]])
+    cell_size = density.real_size[:n].float()
+    area = (cell_size.prod(1).double() * AREA_SCALE).round().long()
+    all_pins = wirelength.all
+    cid, carea = cluster_cells(all_pins["cell"], all_pins["net_start"], area, args.cluster_ratio, args.seed)
+    num = carea.numel()
+    csize, single = cluster_sizes(cid, carea, cell_size)
+    owner, off, net = cluster_pins(cid, single, wirelength.pins())
+    cwl = Wirelength(owner, off, net, all_pins["num_nets"], num, args.ignore_net_degree, device)
+    cpins = torch.bincount(owner, minlength=num + 1)[:num].double()
+    cnode = torch.cat([csize.double(), density.real_size[n:]])
     judge_bin = density.judge.nbx if density.judge is not None else density.nbx
-    nb = min(density.nbx, force_num_bin(num, 1.0 - float(fixed_fraction.mean()), judge_bin))
-    if nb != density.nbx:
-        fixed_fraction = data.fixed_fraction(nb, nb)
-    cden = ElectronicDensityLayer(data.width, data.height, nb, nb, fixed_fraction, density.target, cnode, num, device,
-                                  np.concatenate([cpins, np.zeros(len(cnode) - num)]), poisson=poisson)
-    return SimpleNamespace(cid=cid, num=num, size=csize, wirelength=cwl, density=cden, total_pins=float(cpins.sum()),
-                           total_area=float(cnode.prod(1).sum()), num_nodes=len(cnode))
+    nb = min(density.nbx, force_num_bin(num, 1.0 - density.fixed_mean, judge_bin))
+    fraction = density.fraction_on_device() if nb == density.nbx else data.fixed_fraction_on(device, nb, nb)
+    node_pins = torch.cat([cpins, cpins.new_zeros(cnode.shape[0] - num)])
+    cden = ElectronicDensityLayer(data.width, data.height, nb, nb, fraction, density.target, cnode, num, device,
+                                  node_pins, poisson=poisson)
+    return SimpleNamespace(cid=cid, num=num, size=csize, cell_size=cell_size, wirelength=cwl, density=cden,
+                           total_pins=float(cpins.sum()), total_area=float(cnode.prod(1).sum()),
+                           num_nodes=cnode.shape[0])
 
 
 def expand(cpos, coarse, node_size, seed):
-    """Cells at uniform points of their cluster's square (cluster.py:137-144); fillers unchanged."""
-    n, device = len(coarse.cid), cpos.device
+    """Clusters + fillers -> cells + fillers (cluster.py:137-144): every cell at a uniform point (torch's CPU
+    generator, as the reference) of the room its cluster's square leaves around it; fillers unchanged."""
+    n, device = coarse.cid.numel(), cpos.device
+    cell_size = coarse.cell_size if getattr(coarse, "cell_size", None) is not None else \
+        torch.tensor(node_size[:n], dtype=torch.float32, device=device)
     gen = torch.Generator().manual_seed(seed)
-    room = (coarse.size[coarse.cid] - torch.tensor(node_size[:n], dtype=torch.float32, device=device)).clamp_(min=0)
+    room = (coarse.size[coarse.cid] - cell_size).clamp_(min=0)
     cells = cpos[:coarse.num][coarse.cid] + (torch.rand(n, 2, generator=gen) - 0.5).to(device) * room
     return torch.cat([cells, cpos[coarse.num:]])

@@ -160,16 +160,49 @@ class PlaceData(object):
         component or a blockage (database.py:103-134). Exact: the union of the rows minus the union of the blockers on
         the grid of all box edges, each grid cell's area split over the bins by the separable interval overlaps."""
         x0, y0, x1, y1 = self.region
-        boxes = np.concatenate([self.row_rects, self.block_rects])
-        xs = np.unique(np.clip(np.concatenate([boxes[:, 0], boxes[:, 2], [x0, x1]]), x0, x1))
-        ys = np.unique(np.clip(np.concatenate([boxes[:, 1], boxes[:, 3], [y0, y1]]), y0, y1))
-        usable = _mark_union(self.row_rects, xs, ys, x0, y0, x1, y1) & \
-            ~_mark_union(self.block_rects, xs, ys, x0, y0, x1, y1)
+        xs, ys, usable = self.edge_grid()
         ox = _interval_bin_overlap(xs, x0, x1, nbx)
         oy = _interval_bin_overlap(ys, y0, y1, nby)
         free = ox.T.dot(usable.astype(np.float64)).dot(oy)
         bin_area = (x1 - x0) / float(nbx) * (y1 - y0) / float(nby)
         return np.clip(1.0 - free / bin_area, 0.0, 1.0)
+
+    def edge_grid(self):
+        """(xs, ys, usable): the edges of every row / blocker box clipped to the region, and which cells of that edge
+        grid lie in a row and under no blocker. Depends on the design only: computed once."""
+        if getattr(self, "_edge_grid", None) is None:
+            x0, y0, x1, y1 = self.region
+            boxes = np.concatenate([self.row_rects, self.block_rects])
+            xs = np.unique(np.clip(np.concatenate([boxes[:, 0], boxes[:, 2], [x0, x1]]), x0, x1))
+            ys = np.unique(np.clip(np.concatenate([boxes[:, 1], boxes[:, 3], [y0, y1]]), y0, y1))
+            usable = _mark_union(self.row_rects, xs, ys, x0, y0, x1, y1) & \
+                ~_mark_union(self.block_rects, xs, ys, x0, y0, x1, y1)
+            self._edge_grid = (xs, ys, usable)
+        return self._edge_grid
+
+    def fixed_fraction_on(self, device, nbx, nby):
+        """fixed_fraction on the device (ap_free_fraction): float64 tensor [nbx, nby], equal to the host form to
+        rounding (its sums run in another order)."""
+        import ctypes
+
+        import torch
+
+        from . import _lib
+        cache = getattr(self, "_edge_grid_dev", None)
+        if cache is None or cache[0] != str(device):
+            xs, ys, usable = self.edge_grid()
+            cache = (str(device), torch.as_tensor(xs, dtype=torch.int64, device=device),
+                     torch.as_tensor(ys, dtype=torch.int64, device=device),
+                     torch.as_tensor(np.ascontiguousarray(usable, dtype=np.uint8), device=device))
+            self._edge_grid_dev = cache
+        _, xs, ys, usable = cache
+        out = torch.empty((nbx, nby), dtype=torch.float64, device=device)
+        x0, y0, x1, y1 = (float(v) for v in self.region)
+        p = lambda t: ctypes.c_void_p(t.data_ptr())   # noqa: E731
+        stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+        _lib.check(_lib.lib().ap_free_fraction(p(xs), len(xs), p(ys), len(ys), p(usable), x0, y0, x1, y1, nbx, nby,
+                                               p(out), stream), "free_fraction")
+        return out
 
     def to_def_xy(self, pos):
         """Cell centres (placer units) -> lower-left corners in DBU, rounded to the nearest integer."""

@@ -17,8 +17,10 @@ from . import gp_cuda
 class ElectronicDensityLayer(object):
     def __init__(self, width, height, nbx, nby, fixed_fraction, target_density, node_size, num_mov, device,
                  node_pins=None, field=True, poisson="auto"):
-        """node_size [nodes, 2] numpy (real sizes); fixed_fraction [nbx, nby] numpy; node_pins [nodes] numpy (for the
-        preconditioner); field=False: a grid that only measures overflows."""
+        """node_size [nodes, 2] (real sizes), fixed_fraction [nbx, nby], node_pins [nodes] (for the preconditioner):
+        numpy arrays, or float64 tensors on the device (a layer built inside the GP clock: no host round trip; the
+        float32 arrays are the same, elementwise IEEE operations either way). field=False: a grid that only measures
+        overflows."""
         device = torch.device(device)
         if device.type != "cuda":
             raise RuntimeError("the density kernels run on a HIP device (got %s)" % device)
@@ -27,23 +29,39 @@ class ElectronicDensityLayer(object):
         self.width, self.height = width, height
         self.bin_np = np.array([width / nbx, height / nby])
         self.bin_area = float(self.bin_np.prod())
-        self.mov_area = float(np.prod(node_size[:num_mov], 1).sum())
-        t = lambda a, dt=torch.float32: torch.tensor(a, dtype=dt, device=device)   # noqa: E731
+        t = lambda a, dt=torch.float32: torch.as_tensor(a, dtype=dt, device=device)   # noqa: E731
         self.bin = t(self.bin_np)
-        self.fixed_map = t(fixed_fraction * target_density)
-        real = np.asarray(node_size, dtype=np.float64)
-        smooth = np.maximum(real, math.sqrt(2) * self.bin_np)
+        on_device = torch.is_tensor(node_size)
+        real = node_size.to(device, torch.float64) if on_device else np.asarray(node_size, dtype=np.float64)
+        frac = fixed_fraction.to(device, torch.float64) if torch.is_tensor(fixed_fraction) else \
+            np.asarray(fixed_fraction, dtype=np.float64)
+        self.mov_area = float(real[:num_mov].prod(1).sum())
+        self.fixed_map = t(frac * target_density)
+        self._fraction = frac
+        self.fixed_mean = float(frac.mean())           # the blocked share of the region (cluster.py:126)
+        self.real_size = t(real, torch.float64)        # float64 real sizes, kept for a layer built from this one
+        floor = math.sqrt(2) * self.bin_np
+        smooth = torch.maximum(real, t(floor, torch.float64)) if on_device else np.maximum(real, floor)
         self.smooth = {"size": t(smooth), "ratio": t(real.prod(1) / smooth.prod(1))}
-        self.real = {"size": t(real[:num_mov]), "ratio": t(np.ones(num_mov))}
+        self.real = {"size": t(real[:num_mov]), "ratio": torch.ones(num_mov, device=device)}
         self.pos_lo = self.smooth["size"] / 2
         self.pos_hi = t([width, height]) - self.pos_lo
-        self.pins = t(np.zeros(len(real)) if node_pins is None else node_pins)
+        if node_pins is None:
+            self.pins = torch.zeros(len(real), device=device)
+        else:
+            self.pins = t(node_pins)
         self.area = t(real.prod(1))
         self.no_grad = torch.zeros(0, 2, device=device)
         self._scalars = torch.empty(2, device=device)
         self.judge = None
         self.plan = gp_cuda.PoissonPlan(nbx, nby, float(self.bin_np[0]), float(self.bin_np[1]), poisson) \
             if field else None
+
+    def fraction_on_device(self):
+        """The grid's blocked fraction as a float64 tensor [nbx, nby] on the device."""
+        if not torch.is_tensor(self._fraction):
+            self._fraction = torch.as_tensor(self._fraction, dtype=torch.float64, device=self.device)
+        return self._fraction
 
     def _density(self, pos, n, num_mov):
         """-> (total density mirrored [2 nbx, 2 nby], overflow as a 0-d float64 tensor)."""

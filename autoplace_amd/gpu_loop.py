@@ -8,6 +8,7 @@ after every pass.
 """
 import ctypes
 import math
+import time
 
 import torch
 
@@ -32,6 +33,13 @@ def as_log(log):
     if hasattr(log, "info"):
         return log.info
     raise TypeError("log must be a logging.Logger, a callable or None, not %s" % type(log).__name__)
+
+
+def mark(timeline, name, device):
+    """Instrumentation: with a timeline list, synchronise the device and append (name, wall time)."""
+    if timeline is not None:
+        torch.cuda.synchronize(device)
+        timeline.append((name, time.time()))
 
 
 def next_coef(a):
@@ -229,14 +237,16 @@ def control_state(ps, optimizer, iteration, args):
 
 
 def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every=16, use_graph=True,
-                     poisson="auto", reorder=True, hook=None, controller=None, phase=0.0, autotune=None):
+                     poisson="auto", reorder=True, hook=None, controller=None, phase=0.0, autotune=None,
+                     timeline=None):
     """Continue the level from `iteration` (density weight on) until it stops; updates optimizer, ps and
     objective.evals as the host loop would. -> (reason, iterations, hpwl, overflow) of the kept placement.
     hook(objective, u, v, g, S, kp, c), if given, sees the state the device loop starts from (instrumentation).
     controller: None (the plain rule) or a dict for the learned controller (controller.py: `attach`), whose "levels"
     list receives this level's record (passes under the controller, fallback, the logged passes).
     autotune: time the density tile layouts at the start and keep the fastest (None: for the cell level when it has
-    100k nodes or more -- the long level, where the timing pays for itself); the results do not depend on it."""
+    100k nodes or more -- the long level, where the timing pays for itself); the results do not depend on it.
+    timeline: None, or a list that receives (stage, synchronised wall time) marks (instrumentation)."""
     device, pos = optimizer.v.device, optimizer.v
     log = as_log(log)
     S, kp, c = control_state(ps, optimizer, iteration, args)
@@ -249,8 +259,10 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
         hook(objective, u, pos, g, S, kp, c)
     best = pos.clone()
     eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
+    mark(timeline, "engine set-up", device)
     if autotune or (autotune is None and phase == 0.0 and objective.density.num_nodes >= 100000):
         eng.autotune(3)
+        mark(timeline, "autotune", device)
     ctl_log = None
     if controller is not None:
         ctl_log = eng.set_controller(controller, phase)
@@ -265,6 +277,7 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
                 log("iter: %d | hpwl: %.4E overflow: %.4f density_weight: %.4E wa_coeff: %.4E" % (it, hpwl, ovf, lam, gam))
             if stop:
                 break
+        mark(timeline, "device loop", device)
         eng.finish()
         objective.bucket_spills = eng.bucket_spills()   # density entries past their bucket's room (instrumentation)
         if controller is not None:

@@ -44,10 +44,12 @@ def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None, phase=0
     hpwl = overflow = float("nan")
     for iteration in range(first_iter, args.inner_iter):
         if ps.density_weight > 0 and gpu_loop.usable(optimizer):
+            gpu_loop.mark(opts.get("timeline"), "start phase", pos.device)
             reason, iters, hpwl, overflow = gpu_loop.run_level_on_gpu(
                 optimizer, objective, ps, args, log, iteration, use_graph=opts.get("graph", True),
                 poisson=opts.get("poisson", "auto"), reorder=opts.get("reorder", True), hook=opts.get("hook"),
-                controller=opts.get("controller"), phase=phase, autotune=opts.get("autotune"))
+                controller=opts.get("controller"), phase=phase, autotune=opts.get("autotune"),
+                timeline=opts.get("timeline"))
             return SimpleNamespace(reason=reason, iters=iters, start_iters=start_iters, hpwl=hpwl, overflow=overflow)
         optimizer.step()
         if optimizer.diverged:
@@ -84,6 +86,7 @@ def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=N
         coarse = coarse_problem(data, node_size, wirelength, density, fixed_fraction, args,
                                 poisson=opts.get("poisson", "auto"))
         _sync(device)
+        gpu_loop.mark(opts.get("timeline"), "clusters", device)
         stats = {"clusters": coarse.num, "cluster_time_s": time.time() - start, "coarse_num_bin": coarse.density.nbx}
         if log:
             log("two levels: %d clusters of %d cells (%.2fs), %d x %d bins"
@@ -96,10 +99,12 @@ def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=N
         cobjective = Objective(coarse.wirelength, coarse.density, cps)
         cpos = torch.cat([pos[:coarse.num], pos[n:]])
         level = _place_level(cpos, cobjective, cps, cargs, log, 0, opts, phase=1.0)
+        gpu_loop.mark(opts.get("timeline"), "coarse level", device)
         pos.copy_(expand(cpos, coarse, node_size, args.seed)).clamp_(min=density.pos_lo, max=density.pos_hi)
         weight = cps.density_weight * objective.tension_unit(pos) / cobjective.tension_unit(cpos)
         overflow = density.forward(pos)[1].item() if args.wa_rule == "overflow" else None
         ps.resume(cps, weight, wirelength.forward(pos, ps.wa_coeff)[2].item(), overflow)
+        gpu_loop.mark(opts.get("timeline"), "expansion", device)
         if log:
             log("clusters placed after %d iterations (stop: %s, hpwl %.4E, overflow %.4f): density weight %.4E"
                 % (level.iters, level.reason, level.hpwl, level.overflow, weight))
@@ -111,6 +116,9 @@ def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=N
     level = _place_level(pos, objective, ps, args, log, first_iter, opts)
     _sync(device)
     gp_time, iters = time.time() - start, level.iters
+    if opts.get("timeline") is not None:
+        opts["timeline"].insert(0, ("start", start))
+        opts["timeline"].append(("cell level", start + gp_time))
     if log:
         log("GP stop (%s): #iters %d hpwl: %.6E overflow: %.4f GP time: %.4fs (%d gradient calls)"
             % (level.reason, iters, level.hpwl, level.overflow, gp_time, evals + objective.evals))

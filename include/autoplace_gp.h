@@ -269,6 +269,32 @@ AP_API int ap_level_set_controller(ap_level* level, const float* params, float s
  * improvement, fallback (1: the rule took over), learned passes so far. Waits for the level's stream. */
 AP_API int ap_level_controller_state(ap_level* level, double* state);
 
+/* ---------------------------------------------------------------- part 5: problem preparation on the device
+ * Clustering of the two-level placement: replaces cluster_cells (src/cluster.py:38-85), which the reference runs as
+ * torch operators inside the GP clock. The caller drives the rounds (cluster.py:64-85): for each, it draws the heads
+ * (torch's CPU generator, as the reference) and passes them as u8 [num] on the device. */
+typedef struct ap_cluster ap_cluster;
+/* The weighted cell graph of the nets of 2 .. max_degree pins (cluster.py:44-58): pin_cell i32 [P] grouped by net
+ * (num_cells = fixed pin), net_start i32 [num_nets + 1], area i64 [num_cells]. Synchronises the stream. */
+AP_API int ap_cluster_create(int num_cells, const int* pin_cell, const int* net_start, int num_nets,
+                             const long long* area, int max_degree, void* stream, ap_cluster** out);
+/* One round of star matching (cluster.py:66-84) with heads u8 [num] on the device, the merged-area cap (float32, as
+ * the reference's tensor), the big-cell limit and per_head (1 .. 4). *num_out: the clusters after the round (equal to
+ * the clusters before it when nothing merged; the state is then unchanged). Synchronises the stream. */
+AP_API int ap_cluster_round(ap_cluster* c, const unsigned char* head, float cap, long long big, int per_head,
+                            int* num_out, void* stream);
+AP_API int ap_cluster_num(const ap_cluster* c);
+AP_API int ap_cluster_edges(const ap_cluster* c);
+/* cid i64 [num_cells]: the cluster of every cell; area i64 [num]: the clusters' areas. */
+AP_API int ap_cluster_result(ap_cluster* c, long long* cid, long long* area, void* stream);
+AP_API int ap_cluster_destroy(ap_cluster* c);
+
+/* Free-area fraction of a grid (src/database.py:103-134) on the device: xs i64 [nx], ys i64 [ny] the edges of every
+ * row / blocker box clipped to the region, usable u8 [(nx - 1) x (ny - 1)] the edge-grid cells inside a row and
+ * under no blocker; out f64 [nbx x nby]. Equal to the host form to rounding (its sums run in another order). */
+AP_API int ap_free_fraction(const long long* xs, int nx, const long long* ys, int ny, const unsigned char* usable,
+                            double x0, double y0, double x1, double y1, int nbx, int nby, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

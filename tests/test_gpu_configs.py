@@ -179,6 +179,32 @@ def test_config_b_csr_and_grids(cfg_b):
     assert c.wl.num_opt_nets == c.csr["num_nets"]
 
 
+def _clusters_match_the_oracle(c):
+    """The device clustering (every round) against the oracle's numpy restatement, which the reference's own cluster
+    ids pin: the same cluster of every cell and the same areas, bit for bit."""
+    import time
+
+    from autoplace_amd.cluster import AREA_SCALE, cluster_cells
+    from oracle import cluster as ocl
+    n = c.data.num_mov
+    size = torch.tensor(c.data.size, dtype=torch.float32, device=DEV)
+    area = (size.prod(1).double() * AREA_SCALE).round().long()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    cid, carea = cluster_cells(c.wl.all["cell"], c.wl.all["net_start"], area, 5.0, 0)
+    torch.cuda.synchronize()
+    t_dev = time.time() - t0
+    ref_cid, ref_area = ocl.cluster_cells(c.data.pin_cell, c.data.pin_net, area.cpu().numpy(), 5.0, 0)
+    print("config %s clustering: %d clusters of %d cells, device %.4f s" % (c.name, len(ref_area), n, t_dev),
+          flush=True)
+    assert np.array_equal(cid.cpu().numpy(), ref_cid)
+    assert np.array_equal(carea.cpu().numpy(), ref_area)
+
+
+def test_config_b_clusters_match_the_oracle(cfg_b):
+    _clusters_match_the_oracle(cfg_b)
+
+
 def test_config_b_dropin_kernels_vs_oracle(cfg_b):
     """gp_cuda.wa / density_map (force and overflow grid) / poisson_field / density_grad at 200k cells."""
     from autoplace_amd import gp_cuda
@@ -317,6 +343,17 @@ def test_config_c_engine_objective_and_map(cfg_c):
     c = cfg_c
     assert c.prob.force_bin == 2048 and c.prob.num_bin == 512
     _check_engine_objective(c, [4.0 * 10.0, 1e-3, 1e-3, 0.0, 0.0])
+
+
+def test_config_c_clusters_match_the_oracle(cfg_c):
+    _clusters_match_the_oracle(cfg_c)
+
+
+def test_config_c_free_fraction_on_the_device(cfg_c):
+    for nb in (512, 1024, 2048):
+        host = cfg_c.data.fixed_fraction(nb, nb)
+        dev = cfg_c.data.fixed_fraction_on(DEV, nb, nb).cpu().numpy()
+        assert np.abs(dev - host).max() < 1e-12, nb
 
 
 def test_config_c_poisson_fft_and_gemm_2048(cfg_c):
