@@ -85,7 +85,8 @@ class LevelDesc(ctypes.Structure):
                 ("judge_size", P), ("judge_ratio", P), ("judge_fixed", P),
                 ("poisson_method", I),
                 ("u", P), ("v", P), ("g", P), ("best", P),
-                ("S", P), ("kp", P), ("c", P)]
+                ("S", P), ("kp", P), ("c", P),
+                ("poisson_plan", P)]
 
 
 _lib = None

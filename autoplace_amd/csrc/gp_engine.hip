@@ -18,7 +18,10 @@
 // and replays from a HIP graph.
 #include <stdio.h>
 
+#include <stdlib.h>
+
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -1036,23 +1039,39 @@ __global__ void eng_finalize_flags(int* flags) {
 // max(64, nodes / 1024) nodes could span more than T + 1 bins along an axis (those take the big-node path, which
 // every tile's workgroup scans) or while the buckets do not fit the count / fill kernels' histogram. 0: no side fits.
 constexpr int BIN_TILE_NODES = 1024;
-static int tile_side(const std::vector<float2>& size, double bx, double by, int nbx, int nby, int planes) {
-    const long long n = (long long)size.size();
+// Nodes whose reach along an axis (bins a node can touch: floor(size / bin) + 2 at most) exceeds T + 1 for the tile
+// sides T = 8, 16, 32, 64: integer counts, so the same in any order.
+__global__ __launch_bounds__(256) void eng_reach_counts(const float2* size, int n, double bx, double by,
+                                                        unsigned long long* over) {
+    __shared__ unsigned cnt[4];
+    if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const float2 s = size[i];
+        const double reach = fmin(1e9, fmax(floor((double)s.x / bx), floor((double)s.y / by)) + 2.0);
+        for (int k = 0; k < 4; k++)
+            if (reach > (double)((8 << k) + 1)) atomicAdd(&cnt[k], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && cnt[threadIdx.x]) atomicAdd(&over[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+}
+
+// The tile side of a grid: the power of two 8 .. 64 near sqrt(BIN_TILE_NODES bins / node) (about BIN_TILE_NODES
+// nodes per tile), enlarged while more than max(64, n / 1024) nodes reach past a tile's neighbours; sides whose
+// bucket count exceeds the binning's histogram are skipped. over[k]: the nodes reaching past side 8 << k.
+static int tile_side(long long n, const unsigned long long* over, int nbx, int nby, int planes) {
     const double want = std::sqrt((double)BIN_TILE_NODES * nbx * nby / (double)std::max(n, 1LL));
     int T = 8;
     while (T < 64 && 2 * T <= want) T *= 2;
-    // bins a node can touch along an axis: floor(size / bin) + 2 at most
-    std::vector<int> reach(n);
-    for (long long i = 0; i < n; i++)
-        reach[i] = (int)std::min(1e9, std::max(std::floor(size[i].x / bx), std::floor(size[i].y / by)) + 2);
     const long long allow = std::max(64LL, n / 1024);
     int fits = 0;
     for (; T <= 64; T *= 2) {
         if ((long long)planes * grid_of(nbx, T) * grid_of(nby, T) > BIN_MAX_BUCKETS) continue;
         fits = T;
-        long long big = 0;
-        for (long long i = 0; i < n; i++) big += reach[i] > T + 1;
-        if (big <= allow) return T;
+        int k = 0;
+        while ((8 << k) < T) k++;
+        if ((long long)over[k] <= allow) return T;
     }
     return fits;   // the largest side that fits; its big nodes take the slower path
 }
@@ -1079,6 +1098,7 @@ constexpr int GRAPH_CACHE = 4;
 struct Level {
     ap_level_desc d{};
     Poisson poisson;
+    bool poisson_borrowed = false;   // the plan is the caller's (ap_level_desc.poisson_plan): not destroyed here
     WaPlan wa_plan;
     float2 *u1 = nullptr, *v1 = nullptr, *g1 = nullptr, *pin_grad = nullptr;
     float2* cell_wl = nullptr;   // the cells' WA gradients (eng_cell_wl, two-stream passes)
@@ -1214,9 +1234,21 @@ int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nod
     J.nbx = nbx;
     J.nby = nby;
     J.planes = planes;
-    std::vector<float2> host_size(num_nodes);
-    AP_HIP(hipMemcpy(host_size.data(), size, (size_t)num_nodes * sizeof(float2), hipMemcpyDeviceToHost));
-    J.T = J.T0 = tile_side(host_size, bx, by, nbx, nby, planes);
+    unsigned long long over[4] = {0, 0, 0, 0};
+    {
+        unsigned long long* d_over = nullptr;
+        AP_HIP(hipMalloc(&d_over, sizeof(over)));
+        hipError_t e = hipMemset(d_over, 0, sizeof(over));
+        if (e == hipSuccess && num_nodes > 0) {
+            hipLaunchKernelGGL(eng_reach_counts, dim3(grid_of(num_nodes, 256)), dim3(256), 0, nullptr, J.size,
+                               num_nodes, bx, by, d_over);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpy(over, d_over, sizeof(over), hipMemcpyDeviceToHost);
+        (void)hipFree(d_over);
+        AP_HIP(e);
+    }
+    J.T = J.T0 = tile_side(num_nodes, over, nbx, nby, planes);
     if (!J.T) {
         set_error("level: density grid %d x %d is beyond the tile binning's %d buckets", nbx, nby, BIN_MAX_BUCKETS);
         return AP_ERR_UNSUPPORTED;
@@ -1467,7 +1499,23 @@ int Level::autotune(int reps, hipStream_t s, int* sides) {
     return AP_OK;
 }
 
+// With AP_TRACE_CREATE set in the environment, a level's set-up prints the time of each phase (device synchronised)
+// to stderr: instrumentation, off by default.
+struct CreateTrace {
+    bool on = getenv("AP_TRACE_CREATE") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void operator()(const char* phase) {
+        if (!on) return;
+        (void)hipDeviceSynchronize();
+        const auto now = std::chrono::steady_clock::now();
+        fprintf(stderr, "level create: %-20s %.3f ms\n", phase,
+                std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
+
 int Level::create(const ap_level_desc& desc) {
+    CreateTrace trace;
     d = desc;
     AP_REQUIRE(d.num_nodes > 0 && d.num_mov >= 0 && d.num_mov <= d.num_nodes && d.num_pins >= 0 && d.num_nets >= 0,
                "level: bad sizes");
@@ -1478,8 +1526,16 @@ int Level::create(const ap_level_desc& desc) {
     AP_REQUIRE(d.u && d.v && d.g && d.best && d.S && d.kp && d.c && d.size && d.ratio && d.pins && d.area && d.lo &&
                    d.hi && d.fixed,
                "level: null pointer");
-    int st = poisson.create(d.nbx, d.nby, d.bx, d.by, d.poisson_method);
-    if (st) return st;
+    const Poisson* given = d.poisson_plan ? &poisson_of(d.poisson_plan) : nullptr;
+    int st;
+    if (given && given->N == d.nbx && given->M == d.nby && given->bx == d.bx && given->by == d.by &&
+        (d.poisson_method == AP_POISSON_AUTO || d.poisson_method == given->method)) {
+        poisson = *given;                          // the caller's plan: used, not destroyed
+        poisson_borrowed = true;
+    } else if ((st = poisson.create(d.nbx, d.nby, d.bx, d.by, d.poisson_method))) {
+        return st;
+    }
+    trace("poisson plan");
     const long long N = d.num_nodes, nb = (long long)d.nbx * d.nby;
     AP_HIP(hipMalloc(&u1, N * sizeof(float2)));
     AP_HIP(hipMalloc(&v1, N * sizeof(float2)));
@@ -1503,8 +1559,10 @@ int Level::create(const ap_level_desc& desc) {
         AP_HIP(hipMalloc(&fy, nb * sizeof(float)));
     }
     nblk_n = grid_of(N, EB);
+    trace("buffers, pin slots");
     if ((st = wa_plan.create(d.net_start, d.num_nets, d.pin_cell, d.pin_off, pin_slot, d.num_pins))) return st;
     nblk_wa = wa_plan.partial_count();
+    trace("wa plan");
     if ((st = bin_job(fj, d.size, d.ratio, d.num_nodes, d.num_mov, d.bx, d.by, d.nbx, d.nby, 2, d.fixed, rho,
                       d.has_judge ? 0 : 1)))
         return st;
@@ -1512,6 +1570,7 @@ int Level::create(const ap_level_desc& desc) {
         (st = bin_job(jj, d.judge_size, d.judge_ratio, d.num_mov, d.num_mov, d.jbx, d.jby, d.njx, d.njy, 1,
                       d.judge_fixed, nullptr, 1)))
         return st;
+    trace("bin jobs");
     if ((st = set_lds())) return st;
     // one overflow partial per tile: room for the most tiles any side the autotuner may pick gives (a grid has at
     // most BIN_MAX_BUCKETS tiles)
@@ -1543,6 +1602,7 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipEventCreate(&t0));
     AP_HIP(hipEventCreate(&t1));
     AP_HIP(hipDeviceSynchronize());
+    trace("partials, streams");
     return AP_OK;
 }
 
@@ -1559,7 +1619,9 @@ void Level::destroy() {
         if (p) (void)hipFree(p);
     for (void* p : bin_mem) (void)hipFree(p);
     bin_mem.clear();
-    poisson.destroy();
+    if (!poisson_borrowed) poisson.destroy();
+    poisson = Poisson{};
+    poisson_borrowed = false;
     wa_plan.destroy();
     if (stream) (void)hipStreamDestroy(stream);
     if (side) (void)hipStreamDestroy(side);

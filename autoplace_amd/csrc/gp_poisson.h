@@ -4,6 +4,8 @@
 
 #include <vector>
 
+#include "../../include/autoplace_gp.h"
+
 namespace ap {
 
 void poisson_weights(int N, int M, double bx, double by, std::vector<float>& wx, std::vector<float>& wy);
@@ -62,5 +64,8 @@ struct Poisson {
     int fft_solve(const float* rho, int rho_ld, float* fx, float* fy, float2* f2, int fld, hipStream_t s);
     FftTab fft_tab(const float* rho, int rho_ld, float* fx, float* fy, float2* f2, int fld, const double* S) const;
 };
+
+// the plan inside a C ABI handle (gp_poisson.hip)
+const Poisson& poisson_of(const ap_poisson* plan);
 
 }  // namespace ap

@@ -321,6 +321,10 @@ struct ap_poisson {
     Poisson p;
 };
 
+namespace ap {
+const Poisson& poisson_of(const ap_poisson* plan) { return plan->p; }
+}  // namespace ap
+
 extern "C" {
 
 int ap_poisson_create(int nbx, int nby, double bx, double by, int method, ap_poisson** plan) {

@@ -10,7 +10,11 @@
 // across rows (G = 32, 64) are shuffles. After a butterfly every lane of the group holds the same bits (each step
 // adds the same two partial results, in either order), so no broadcast is needed and the results are
 // deterministic. Nets of more than 64 pins take a workgroup-wide path.
+#include <algorithm>
+#include <cstring>
 #include <vector>
+
+#include <rocprim/device/device_scan.hpp>
 
 #include "gp_common.h"
 #include "gp_kernels.h"
@@ -167,63 +171,185 @@ __global__ __launch_bounds__(256) void wa_lanes_fill(const int* lane_pin, long l
     lanes[l] = v;
 }
 
-// Nets in net order; each degree class keeps one open wavefront, closed when its 64 / G groups are taken, so the
-// wavefronts stay close to net order (and to the cells' order) within every class. The pins' data is then copied
-// into lane order once, so that a wavefront reads one contiguous 1 KB block.
+// The plan, built on the device. Nets are taken in net order; each degree class (2^k lanes per net, k = 1 .. 6)
+// keeps one open wavefront of 64 / 2^k nets, emitted when its last net joins it; at the end the classes' partial
+// wavefronts follow in class order. So the wavefronts stay close to net order (and to the cells' order) within every
+// class. Nets of more than 64 pins go to the block-wide path. Construction, without a host loop over the nets:
+//   wa_plan_class   every net's class (0: more than 64 pins) as a one-hot count; an exclusive scan gives each net
+//                   its rank within its class and the class sizes;
+//   wa_plan_rank    the nets that complete a wavefront (rank + 1 a multiple of 64 / 2^k); their exclusive scan is the
+//                   full wavefronts' order; the nets by (class, rank) and the big nets in order;
+//   wa_plan_lanes   every net's pins into the lanes of its wavefront: the wavefront of its completing net, or its
+//                   class's partial one.
+// The pins' data is then copied into lane order once, so that a wavefront reads one contiguous 1 KB block.
+struct ClassCount {
+    int c[7];                  // [0]: nets of more than 64 pins, [k]: nets of 2^k lanes
+};
+struct ClassCountPlus {
+    __host__ __device__ ClassCount operator()(const ClassCount& a, const ClassCount& b) const {
+        ClassCount r;
+        for (int k = 0; k < 7; k++) r.c[k] = a.c[k] + b.c[k];
+        return r;
+    }
+};
+
+__device__ __forceinline__ int net_class(int deg) {
+    if (deg > 64) return 0;
+    int lg = 1;
+    while ((1 << lg) < deg) lg++;
+    return lg;
+}
+
+__global__ __launch_bounds__(256) void wa_plan_class(const int* net_start, int num_nets, unsigned char* cls,
+                                                     ClassCount* one) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= num_nets) return;
+    const int k = net_class(net_start[i + 1] - net_start[i]);
+    cls[i] = (unsigned char)k;
+    ClassCount v;
+    for (int j = 0; j < 7; j++) v.c[j] = j == k;
+    one[i] = v;
+}
+
+struct ClassOffsets {
+    int off[7];                // start of each class in the nets-by-rank array
+};
+
+__global__ __launch_bounds__(256) void wa_plan_rank(const unsigned char* cls, const ClassCount* prefix, int num_nets,
+                                                    ClassOffsets o, int* rank, int* completes, int* by_rank,
+                                                    int* big) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= num_nets) return;
+    const int k = cls[i];
+    const int r = ((const int*)&prefix[i])[k];
+    rank[i] = r;
+    if (k == 0) {
+        big[r] = i;
+        completes[i] = 0;
+        return;
+    }
+    const int per_wave = 64 >> k;
+    completes[i] = (r + 1) % per_wave == 0;
+    by_rank[o.off[k] + r] = i;
+}
+
+__global__ __launch_bounds__(256) void wa_plan_lanes(const int* net_start, const unsigned char* cls, const int* rank,
+                                                     int num_nets, ClassOffsets o, ClassOffsets count,
+                                                     ClassOffsets partial_wave, const int* by_rank,
+                                                     const int* wave_of_completing, int* lane_pin,
+                                                     unsigned char* lg) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= num_nets) return;
+    const int k = cls[i];
+    if (k == 0) return;
+    const int per_wave = 64 >> k, r = rank[i];
+    const int last = (r / per_wave + 1) * per_wave - 1;      // the rank that completes this net's wavefront
+    const int wave = last < count.off[k] ? wave_of_completing[by_rank[o.off[k] + last]] : partial_wave.off[k];
+    const int base = wave * 64 + (r % per_wave) * (1 << k);
+    for (int q = net_start[i], q1 = net_start[i + 1]; q < q1; q++) lane_pin[base + (q - net_start[i])] = q;
+    if (r % per_wave == 0) lg[wave] = (unsigned char)k;
+}
+
+__global__ __launch_bounds__(256) void wa_plan_clear(int* a, long long n) {
+    const long long l = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (l < n) a[l] = -1;
+}
+
 int WaPlan::create(const int* d_net_start, int num_nets, const int* d_pin_cell, const float* d_pin_off,
                    const int* d_pin_slot, int num_pins) {
-    std::vector<int> ns(num_nets + 1);
-    AP_HIP(hipMemcpy(ns.data(), d_net_start, (num_nets + 1) * sizeof(int), hipMemcpyDeviceToHost));
-    AP_REQUIRE(ns[0] >= 0 && ns[num_nets] <= num_pins, "wa plan: net_start out of the pin range");
-    std::vector<unsigned char> lgs;
-    std::vector<int> big, lane_pin;
-    std::vector<int> open[7];
-    auto close = [&](int lg) {
-        if (open[lg].empty()) return;
-        const int G = 1 << lg;
-        lgs.push_back((unsigned char)lg);
-        const size_t base = lane_pin.size();
-        lane_pin.resize(base + 64, -1);
-        for (size_t k = 0; k < open[lg].size(); k++) {
-            const int net = open[lg][k];
-            for (int q = ns[net]; q < ns[net + 1]; q++) lane_pin[base + k * G + (q - ns[net])] = q;
-        }
-        open[lg].clear();
+    num_waves = num_big = 0;
+    if (num_nets <= 0) return AP_OK;
+    int first = 0, end = 0;
+    AP_HIP(hipMemcpy(&first, d_net_start, sizeof(int), hipMemcpyDeviceToHost));
+    AP_HIP(hipMemcpy(&end, d_net_start + num_nets, sizeof(int), hipMemcpyDeviceToHost));
+    AP_REQUIRE(first >= 0 && end <= num_pins, "wa plan: net_start out of the pin range");
+    // scratch of the construction, freed at the end
+    unsigned char* cls = nullptr;
+    ClassCount *one = nullptr, *prefix = nullptr;
+    int *rank = nullptr, *completes = nullptr, *order = nullptr, *by_rank = nullptr, *lane_pin = nullptr;
+    void* temp = nullptr;
+    std::vector<void*> scratch;
+    auto take = [&](void** p, size_t bytes) {
+        const hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 1);
+        if (e == hipSuccess) scratch.push_back(*p);
+        return e;
     };
-    for (int i = 0; i < num_nets; i++) {
-        const int deg = ns[i + 1] - ns[i];
-        if (deg > 64) {
-            big.push_back(i);
-            continue;
-        }
-        int lg = 1;
-        while ((1 << lg) < deg) lg++;
-        open[lg].push_back(i);
-        if ((int)open[lg].size() == (64 >> lg)) close(lg);
+    auto release = [&]() {
+        (void)hipDeviceSynchronize();
+        for (void* p : scratch) (void)hipFree(p);
+        scratch.clear();
+    };
+    const dim3 g(grid_of(num_nets, 256)), b(256);
+    hipError_t e = hipSuccess;
+    if ((e = take((void**)&cls, num_nets)) || (e = take((void**)&one, num_nets * sizeof(ClassCount))) ||
+        (e = take((void**)&prefix, num_nets * sizeof(ClassCount))) || (e = take((void**)&rank, num_nets * 4ull)) ||
+        (e = take((void**)&completes, num_nets * 4ull)) || (e = take((void**)&order, num_nets * 4ull)) ||
+        (e = take((void**)&by_rank, num_nets * 4ull))) {
+        release();
+        AP_HIP(e);
     }
-    for (int lg = 1; lg <= 6; lg++) close(lg);
-    num_waves = (int)lgs.size();
-    num_big = (int)big.size();
+    hipLaunchKernelGGL(wa_plan_class, g, b, 0, nullptr, d_net_start, num_nets, cls, one);
+    ClassCount zero{};
+    size_t need = 0, need2 = 0;
+    e = rocprim::exclusive_scan(nullptr, need, one, prefix, zero, (size_t)num_nets, ClassCountPlus(), nullptr);
+    if (e == hipSuccess)
+        e = rocprim::exclusive_scan(nullptr, need2, completes, order, 0, (size_t)num_nets, rocprim::plus<int>(),
+                                    nullptr);
+    if (e == hipSuccess) e = take(&temp, std::max(need, need2));
+    if (e == hipSuccess)
+        e = rocprim::exclusive_scan(temp, need, one, prefix, zero, (size_t)num_nets, ClassCountPlus(), nullptr);
+    ClassCount last_one{}, last_prefix{};
+    if (e == hipSuccess) e = hipMemcpy(&last_one, one + num_nets - 1, sizeof(ClassCount), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(&last_prefix, prefix + num_nets - 1, sizeof(ClassCount), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        release();
+        AP_HIP(e);
+    }
+    ClassOffsets off{}, count{}, partial{};
+    int acc = 0;
+    for (int k = 0; k < 7; k++) {
+        count.off[k] = last_prefix.c[k] + last_one.c[k];
+        off.off[k] = acc;
+        acc += k ? count.off[k] : 0;
+    }
+    num_big = count.off[0];
+    if (num_big && (e = hipMalloc(&d_big, num_big * sizeof(int))) != hipSuccess) {
+        release();
+        AP_HIP(e);
+    }
+    hipLaunchKernelGGL(wa_plan_rank, g, b, 0, nullptr, cls, prefix, num_nets, off, rank, completes, by_rank, d_big);
+    size_t tb = std::max(need, need2);
+    e = rocprim::exclusive_scan(temp, tb, completes, order, 0, (size_t)num_nets, rocprim::plus<int>(), nullptr);
+    int last_order = 0, last_completes = 0;
+    if (e == hipSuccess) e = hipMemcpy(&last_order, order + num_nets - 1, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(&last_completes, completes + num_nets - 1, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        release();
+        AP_HIP(e);
+    }
+    num_waves = last_order + last_completes;     // the full wavefronts, in the order their last nets come
+    for (int k = 1; k <= 6; k++) {               // then each class's partial one
+        partial.off[k] = -1;
+        if (count.off[k] % (64 >> k)) partial.off[k] = num_waves++;
+    }
     if (num_waves) {
         const long long n = (long long)num_waves * 64;
-        int* d_lane_pin = nullptr;
-        AP_HIP(hipMalloc(&d_lg, lgs.size()));
-        AP_HIP(hipMemcpy(d_lg, lgs.data(), lgs.size(), hipMemcpyHostToDevice));
-        AP_HIP(hipMalloc(&d_lanes, n * sizeof(int4)));
-        AP_HIP(hipMalloc(&d_lane_pin, n * sizeof(int)));
-        AP_HIP(hipMemcpy(d_lane_pin, lane_pin.data(), n * sizeof(int), hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(wa_lanes_fill, dim3(grid_of(n, 256)), dim3(256), 0, nullptr, d_lane_pin, n, d_pin_cell,
+        if ((e = hipMalloc(&d_lg, num_waves)) != hipSuccess || (e = hipMalloc(&d_lanes, n * sizeof(int4))) !=
+            hipSuccess || (e = take((void**)&lane_pin, n * sizeof(int))) != hipSuccess) {
+            release();
+            AP_HIP(e);
+        }
+        hipLaunchKernelGGL(wa_plan_clear, dim3(grid_of(n, 256)), b, 0, nullptr, lane_pin, n);
+        hipLaunchKernelGGL(wa_plan_lanes, g, b, 0, nullptr, d_net_start, cls, rank, num_nets, off, count, partial,
+                           by_rank, order, lane_pin, d_lg);
+        hipLaunchKernelGGL(wa_lanes_fill, dim3(grid_of(n, 256)), b, 0, nullptr, lane_pin, n, d_pin_cell,
                            (const float2*)d_pin_off, d_pin_slot, d_lanes);
-        const hipError_t e = hipGetLastError();
-        const hipError_t e2 = hipDeviceSynchronize();
-        (void)hipFree(d_lane_pin);
-        AP_HIP(e);
-        AP_HIP(e2);
     }
-    if (num_big) {
-        AP_HIP(hipMalloc(&d_big, big.size() * sizeof(int)));
-        AP_HIP(hipMemcpy(d_big, big.data(), big.size() * sizeof(int), hipMemcpyHostToDevice));
-    }
+    e = hipGetLastError();
+    release();
+    AP_HIP(e);
     return AP_OK;
 }
 

@@ -96,6 +96,10 @@ def setup_level(obj, wirelength, density, u, v, g, best, S, kp, c, poisson="auto
     desc.poisson_method = {"auto": 0, "gemm": 1, "fft": 2}[poisson]
     desc.u, desc.v, desc.g, desc.best = _p(obj.u), _p(obj.v), _p(obj.g), _p(obj.best)
     desc.S, desc.kp, desc.c = _p(S), _p(kp), _p(c)
+    # the density layer's Poisson plan (built with the layer, before the level's passes): the level uses it instead of
+    # building its own; obj keeps the plan alive
+    obj._plan = d.plan
+    desc.poisson_plan = d.plan._h if d.plan is not None else None
     obj.desc = desc
 
 

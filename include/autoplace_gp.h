@@ -151,6 +151,10 @@ typedef struct ap_level_desc {
     double* S;
     float* kp;
     const double* c;
+    /* optional: a Poisson plan of the force grid (same grid, bin sizes and a compatible method) that the level uses
+     * instead of creating its own; the caller keeps it alive while the level lives, and the level does not destroy
+     * it. Null: the level creates its own plan. */
+    const ap_poisson* poisson_plan;
 } ap_level_desc;
 
 AP_API int ap_level_create(const ap_level_desc* desc, ap_level** level);
