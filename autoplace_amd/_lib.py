@@ -16,6 +16,7 @@ D = ctypes.c_double
 
 AP_OK, AP_ERR_ARG, AP_ERR_HIP, AP_ERR_UNSUPPORTED = 0, 1, 2, 3
 AP_POISSON_AUTO, AP_POISSON_GEMM, AP_POISSON_FFT = 0, 1, 2
+AP_AUTOTUNE_ONLINE = 1
 PROF_CLASSES = ("nesterov", "wa", "wa_cells", "density_bin", "density_tiles", "overflow_tiles", "poisson", "gather",
                 "control")
 
@@ -52,7 +53,7 @@ _SIGNATURES = {
     "ap_mlp_adam": ([P, P, P, P, I, F, F, F, F, P], I),
     "ap_level_set_controller": ([P, P, F, ctypes.c_ulonglong, I, F, P, I, I], I),
     "ap_level_controller_state": ([P, ctypes.POINTER(D)], I),
-    "ap_level_autotune": ([P, I, P], I),
+    "ap_level_autotune": ([P, I, I, P], I),
     "ap_level_set_tile_layout": ([P, I, I, I, I], I),
     "ap_level_set_bucket_room": ([P, I, I], I),
     "ap_level_time_every": ([P, I], I),

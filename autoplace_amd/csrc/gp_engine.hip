@@ -1202,7 +1202,7 @@ struct Level {
     size_t state_bytes() const;
     int state_copy(char* keep, bool restore, hipStream_t s);
     int time_passes(int reps, hipStream_t s, float& ms);
-    int autotune(int reps, hipStream_t s, int* sides);
+    int autotune(int reps, hipStream_t s, int* sides, bool online = false);
     int create(const ap_level_desc& desc);
     void destroy();
     int objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev, const float2* g_prev_b,
@@ -1422,20 +1422,24 @@ int Level::time_passes(int reps, hipStream_t s, float& ms) {
 }
 
 // Autotuner of the tile layout: for each grid (force, then overflow), the admissible sides from half to twice the
-// set-up side (8 .. 64) with 1 or 2 tiles per workgroup are timed on whole passes from the current state (saved
-// once, restored before every timing and at the end) and the fastest kept. The map does not depend on either
-// (integer sums), so the choice changes the time only, never the results.
-int Level::autotune(int reps, hipStream_t s, int* sides) {
+// set-up side (8 .. 64) with 1 or 2 tiles per workgroup are timed on whole passes and the fastest kept. The map does
+// not depend on either (integer sums), so the choice changes the time only, never the results. Offline, every timing
+// starts from the current state (saved once, restored before every timing and at the end). Online, the timings are
+// the level's own next passes: they advance the level as any pass does (the control counts them), so the tuning
+// costs no extra passes -- the passes end bit for bit where untuned ones end, as the layout changes no result.
+int Level::autotune(int reps, hipStream_t s, int* sides, bool online) {
     AP_HIP(hipDeviceSynchronize());
     int st;
     char* keep = nullptr;
-    AP_HIP(hipMalloc(&keep, state_bytes()));
-    if ((st = state_copy(keep, false, s))) {
-        (void)hipFree(keep);
-        return st;
+    if (!online) {
+        AP_HIP(hipMalloc(&keep, state_bytes()));
+        if ((st = state_copy(keep, false, s))) {
+            (void)hipFree(keep);
+            return st;
+        }
     }
     auto timed = [&](int n, float& ms) -> int {
-        int e = state_copy(keep, true, s);
+        int e = keep ? state_copy(keep, true, s) : AP_OK;
         return e ? e : time_passes(n, s, ms);
     };
     // coordinate descent: the force grid, the overflow grid, then the force grid again with the overflow grid's
@@ -1471,10 +1475,10 @@ int Level::autotune(int reps, hipStream_t s, int* sides) {
         set_side(*J, best, best_g);
         update_lds();
     }
-    // the state the timing passes changed, back; then the keep buffer goes
-    if (!st) st = state_copy(keep, true, s);
+    // offline: the state the timing passes changed, back; then the keep buffer goes
+    if (!st && keep) st = state_copy(keep, true, s);
     (void)hipStreamSynchronize(s);
-    (void)hipFree(keep);
+    if (keep) (void)hipFree(keep);
     if (st) return st;
     // the timing runs leave the fill form's counts behind: back to zero, the state the counting-sort form (the
     // batch's levels) starts from; the fill form counts again at the new layout anyway (rooms_stale)
@@ -2428,10 +2432,10 @@ int ap_batch_last_ms(ap_batch* batch, float* ms) {
     return AP_OK;
 }
 
-int ap_level_autotune(ap_level* level, int reps, int* sides) {
-    AP_REQUIRE(level && reps > 0, "level_autotune: bad arguments");
+int ap_level_autotune(ap_level* level, int reps, int flags, int* sides) {
+    AP_REQUIRE(level && reps > 0 && (flags & ~AP_AUTOTUNE_ONLINE) == 0, "level_autotune: bad arguments");
     Level& L = level->L;
-    return L.autotune(reps, L.stream, sides);
+    return L.autotune(reps, L.stream, sides, (flags & AP_AUTOTUNE_ONLINE) != 0);
 }
 
 int ap_level_set_tile_layout(ap_level* level, int force_side, int force_group, int overflow_side,

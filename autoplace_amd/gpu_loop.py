@@ -120,11 +120,13 @@ class Engine(object):
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         _lib.check(_lib.lib().ap_level_run(self._h, passes, int(use_graph), stream), "level_run")
 
-    def autotune(self, reps=5):
-        """Time the density tile layouts around the set-up ones (side, tiles per workgroup) at the current point and
-        keep the fastest (results unchanged). -> ((force side, group), (overflow side, group) or (0, 0))."""
+    def autotune(self, reps=5, online=False):
+        """Time the density tile layouts around the set-up ones (side, tiles per workgroup) and keep the fastest
+        (results unchanged). Offline: from the current point, restored afterwards. online: on the level's own next
+        passes, which advance it (no extra passes). -> ((force side, group), (overflow side, group) or (0, 0))."""
         lay = (ctypes.c_int * 4)()
-        _lib.check(_lib.lib().ap_level_autotune(self._h, int(reps), lay), "level_autotune")
+        flags = _lib.AP_AUTOTUNE_ONLINE if online else 0
+        _lib.check(_lib.lib().ap_level_autotune(self._h, int(reps), flags, lay), "level_autotune")
         return (lay[0], lay[1]), (lay[2], lay[3])
 
     def set_tile_layout(self, force, overflow=(0, 0)):
@@ -264,12 +266,12 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
     best = pos.clone()
     eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
     mark(timeline, "engine set-up", device)
-    if autotune or (autotune is None and phase == 0.0 and objective.density.num_nodes >= 100000):
-        eng.autotune(3)
-        mark(timeline, "autotune", device)
     ctl_log = None
     if controller is not None:
         ctl_log = eng.set_controller(controller, phase)
+    if autotune or (autotune is None and phase == 0.0 and objective.density.num_nodes >= 100000):
+        eng.autotune(3, online=True)               # timed on the level's first passes: no extra passes
+        mark(timeline, "autotune", device)
     evals_before, logged = objective.evals, iteration // args.log_freq
     watch = torch.tensor([K["stop"], K["loop_it"], K["lam"], K["gam"], K["res_hpwl"], K["res_ovf"]], device=device)
     try:

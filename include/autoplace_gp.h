@@ -201,11 +201,14 @@ AP_API int ap_level_density(ap_level* level, float* rho, void* stream);
 
 /* Autotuner of the density tile layout: the admissible sides from half to twice the set-up side (8 .. 64 bins) with
  * 1 or 2 tiles per workgroup of the tile kernel, for the force grid, then the overflow grid, then the force grid
- * again, timed on whole passes (reps each) from the level's current state, which is saved first and restored after
- * every timing; the fastest is kept. layout i32[4] (optional): force side, force group, overflow
- * side, overflow group (0, 0: no overflow grid). The density map depends on neither, so the results of later passes
- * are the same whatever the choice. Drops the level's captured graphs. */
-AP_API int ap_level_autotune(ap_level* level, int reps, int* layout);
+ * again, timed on whole passes (reps each); the fastest is kept. flags 0: timed from the level's current state,
+ * which is saved first and restored after every timing (no pass of the level runs). flags AP_AUTOTUNE_ONLINE: timed
+ * on the level's own next passes, which advance it as ap_level_run's passes would (the control counts them): no extra
+ * passes. layout i32[4] (optional): force side, force group, overflow side, overflow group (0, 0: no overflow grid).
+ * The density map depends on neither, so the results are the same whatever the choice. Drops the level's captured
+ * graphs. */
+#define AP_AUTOTUNE_ONLINE 1
+AP_API int ap_level_autotune(ap_level* level, int reps, int flags, int* layout);
 /* Set the density tile layout directly (an admissible side and group of each grid, group 0: the default for the
  * side; the overflow pair is ignored without an overflow grid): the autotuner's choice, replayed. Drops the level's
  * captured graphs. */
