@@ -25,6 +25,8 @@ _SIGNATURES = {
     "ap_version": ([], I),
     "ap_wa_wirelength": ([P, P, P, P, P, P, I, I, P, P, P, P, I, P], I),
     "ap_density_map": ([P, P, P, I, I, F, F, I, I, P, P, F, P, P, P], I),
+    "ap_density_map_work_bytes": ([I, I, I, I], ctypes.c_longlong),
+    "ap_density_map_ws": ([P, P, P, I, I, F, F, I, I, P, F, P, P, P, ctypes.c_longlong, P], I),
     "ap_nesterov_step": ([P, P, P, P, P, P, P, I, P, P, P], I),
     "ap_density_grad": ([P, P, P, P, P, P, P, I, I, P, P, F, F, I, I, P, P], I),
     "ap_control": ([P, P, P, P, P, P, P, P, P], I),

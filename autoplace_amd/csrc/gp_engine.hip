@@ -159,6 +159,8 @@ struct BinJob {
     unsigned long long* spill; // [planes][nbx][nby] fill form: spilled contributions (zero between passes)
     int* spill_tile;           // [tiles] fill form: 1 when the tile has spilled contributions (zero between passes)
     unsigned long long* spill_n;  // [1] fill form: entries spilled since the level was created
+    float* rho_mirror;         // drop-in map (null: none): the total density mirrored in both axes, [2 nbx][2 nby]
+    float* over_map;           // drop-in map (null: none): max(cells + blocked - target, 0) per bin, [nbx][nby]
 };
 
 __device__ __forceinline__ unsigned long long contribution(float wx, const Span& sy, int by) {
@@ -660,6 +662,14 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
         const float r = J.planes == 2 ? mov + (float)((double)acc[TT + e] / kDensityScale) : mov;
         if (J.rho) J.rho[bin] = r;
         if (J.count_over) o += (double)fmaxf(mov - target, 0.f);
+        if (J.rho_mirror) {                       // the reference's FFT input: the bin and its three mirror images
+            const long long w = 2LL * J.nby, mx = 2LL * J.nbx - 1 - bx, my = w - 1 - by;
+            J.rho_mirror[bx * w + by] = r;
+            J.rho_mirror[bx * w + my] = r;
+            J.rho_mirror[mx * w + by] = r;
+            J.rho_mirror[mx * w + my] = r;
+        }
+        if (J.over_map) J.over_map[bin] = fmaxf(mov - target, 0.f);
     };
 #pragma unroll
     for (int i = 0; i < TILE_PF; i++) {
@@ -714,6 +724,8 @@ struct GatherArgs {
     float inv_bx, inv_by;
     int nbx, nby;
     int use_density;          // 0: grad = wl / max(pins, 1) (the weight-0 objective)
+    const float* weight_p;    // drop-in gather: the weight and the preconditioner's weight as separate scalars
+    const float* precond_p;
 };
 
 // a cell's WA gradient: the sum of its pins' gradients, which the WA kernel wrote in cell order (contiguous)
@@ -744,7 +756,10 @@ __global__ __launch_bounds__(EB) void eng_cell_wl(GatherArgs a, float2* cell_wl,
 
 // part[b]: |g' - g|^2 of workgroup b; part_wl[b], part_dn[b]: squared norms of the wirelength part and of the
 // weighted density part of the (unpreconditioned) gradient, the learned controller's gradient-ratio feature
-// (part_wl is eng_cell_wl's when a.cell_wl is set)
+// (part_wl is eng_cell_wl's when a.cell_wl is set).
+// DROPIN: the drop-in density_grad -- cell_wl is the caller's wirelength gradient of the first num_mov nodes (no
+// extra row), the weights come from weight_p / precond_p, and no partials are written.
+template <bool DROPIN = false>
 __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, double* part_wl, double* part_dn, int wg,
                                             int nwg) {
     __shared__ double red[16 * 3];
@@ -758,12 +773,16 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
         const float pins = a.pins[i];
         const float area = a.use_density ? a.area[i] : 0.f;
         float2 w_cell = make_float2(0.f, 0.f);
-        if (a.cell_wl) w_cell = a.cell_wl[min(i, a.num_mov)];   // num_mov + 1 entries
+        if (DROPIN) {
+            if (a.cell_wl && i < a.num_mov) w_cell = a.cell_wl[i];
+        } else if (a.cell_wl) {
+            w_cell = a.cell_wl[min(i, a.num_mov)];   // num_mov + 1 entries
+        }
         float2 gp = make_float2(0.f, 0.f);
         if (a.g_prev) gp = ((a.flags && cur == 1) ? a.g_prev_b : a.g_prev)[i];
         float gx = 0, gy = 0;
         if (a.use_density) {
-            const float weight = a.kp[1];
+            const float weight = DROPIN ? *a.weight_p : a.kp[1];
             const float2 p = pos[i], sz = a.size[i];
             const float r = a.ratio[i];
             Span sx = make_span(p.x, sz.x, a.inv_bx, a.nbx), sy = make_span(p.y, sz.y, a.inv_by, a.nby);
@@ -819,7 +838,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
         }
         float scale;
         if (a.use_density) {
-            const float precond = a.kp[2];
+            const float precond = DROPIN ? *a.precond_p : a.kp[2];
             scale = precond < 0 ? 1.f : 1.f / fmaxf(pins + precond * area, 1.f);
         } else {
             scale = 1.f / fmaxf(pins, 1.f);
@@ -832,6 +851,7 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
             t2[0] = dx * dx + dy * dy;
         }
     }
+    if (DROPIN) return;
     block_sum<3>(t2, red);
     if (threadIdx.x == 0) {
         part[wg] = t2[0];
@@ -2138,6 +2158,155 @@ int Batch::graph_for(int passes, int& slot) {
     return AP_OK;
 }
 
+// ------------------------------------------------------------------------------ drop-in density map and gather
+// The reference's gp_cuda.density_map and density_grad (gp_cuda.cpp:41-65) on the engine's own machinery, so that a
+// caller of the six drop-in functions (the reference's gpu_loop.py) gets the same design as the engine: the map by
+// the tile binning in its counting-sort form (no global float or 64-bit atomics on the map: LDS integer sums per
+// tile, the reference's fixed-point contributions, bit-exact), the gradient by the engine's gather. Stateless: one
+// call's buckets live in a workspace (its size from dropin_map_plan) and every launch goes to the given stream.
+__global__ __launch_bounds__(EB) void dm_clear(BinJob J) {
+    const int n = J.planes * J.tiles;
+    for (int k = blockIdx.x * EB + threadIdx.x; k < n; k += gridDim.x * EB) J.count[k] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *J.big_count = 0;
+}
+__global__ __launch_bounds__(EB) void dm_count(Pick pos, BinJob J) { bin_count_body(pos, J, J, blockIdx.x); }
+__global__ __launch_bounds__(SCAN_T) void dm_scan(BinJob J) { scan_buckets<SCAN_T>(J); }
+__global__ __launch_bounds__(EB) void dm_scatter(Pick pos, BinJob J) { bin_scatter_body(pos, J, J, blockIdx.x); }
+__global__ __launch_bounds__(TB_LEVEL, 8) void dm_tiles(Pick pos, BinJob J, float target, double* part) {
+    bin_tiles_body<TB_LEVEL>(pos, J, target, part, blockIdx.x);
+}
+__global__ __launch_bounds__(EB) void dm_gather(GatherArgs a) {
+    gather_body<true>(a, nullptr, nullptr, nullptr, blockIdx.x, gridDim.x);
+}
+
+// The drop-in map's tile side and workspace layout: the power of two 8 .. 64 at or below the side that gives
+// DROPIN_TILE_NODES nodes per tile (no reach count: a node over 2 x 2 tiles takes the big-node path, which every
+// tile clips), one tile per workgroup, each array on a 256-byte boundary. A stateless call has no autotuner; on the
+// 200k-cell design (force grid 1024 x 1024, overflow grid 512 x 512) sides 16 and 16 measured fastest of every side
+// and group (a whole call 100 and 62 us; 113 and 96 us at sides 32 and 32, which 1024 nodes per tile would give),
+// and more tiles per workgroup were slower at every side.
+constexpr int DROPIN_TILE_NODES = 256;
+struct DropMap {
+    int T = 8, planes = 1;
+    size_t count = 0, start = 0, big_count = 0, big_n = 0, off = 0, big = 0, geo = 0, gr = 0, part = 0, bytes = 0;
+    long long room = 0;
+};
+
+static DropMap dropin_map_plan(int num_nodes, int num_mov, int nbx, int nby) {
+    DropMap m;
+    m.planes = num_mov < num_nodes ? 2 : 1;
+    const double want = std::sqrt((double)DROPIN_TILE_NODES * nbx * nby / (double)std::max(num_nodes, 1));
+    while (m.T < 64 && 2 * m.T <= want) m.T *= 2;
+    while (m.T < 64 && (long long)m.planes * grid_of(nbx, m.T) * grid_of(nby, m.T) > BIN_MAX_BUCKETS) m.T *= 2;
+    m.room = 4LL * num_nodes + 1;                  // a small node has at most 2 x 2 entries
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = at;
+        at += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    m.count = take(BIN_MAX_BUCKETS * sizeof(int));
+    m.start = take((BIN_MAX_BUCKETS + 1) * sizeof(int));
+    m.big_count = take(sizeof(int));
+    m.big_n = take(sizeof(int));
+    m.off = take((size_t)num_nodes * sizeof(int4));
+    m.big = take((size_t)num_nodes * sizeof(int));
+    m.geo = take((size_t)m.room * sizeof(float4));
+    m.gr = take((size_t)m.room * sizeof(float));
+    m.part = take(BIN_MAX_BUCKETS * sizeof(double));
+    m.bytes = at;
+    return m;
+}
+
+static int dropin_map(const float* pos, const float* size, const float* ratio, int num_nodes, int num_mov, float bx,
+                      float by, int nbx, int nby, const float* fixed, float target, float* rho, float* over,
+                      void* work, long long work_bytes, hipStream_t s) {
+    const DropMap m = dropin_map_plan(num_nodes, num_mov, nbx, nby);
+    AP_REQUIRE((long long)m.planes * grid_of(nbx, m.T) * grid_of(nby, m.T) <= BIN_MAX_BUCKETS,
+               "density_map: grid %d x %d has more tiles than the binning's %d buckets", nbx, nby, BIN_MAX_BUCKETS);
+    AP_REQUIRE(work && work_bytes >= (long long)m.bytes, "density_map: workspace of %lld bytes, %zu needed",
+               work_bytes, m.bytes);
+    char* w = (char*)work;
+    BinJob J{};
+    J.size = (const float2*)size;
+    J.ratio = ratio;
+    J.num_nodes = num_nodes;
+    J.num_mov = num_mov;
+    J.inv_bx = 1.f / bx;          // the same float32 reciprocal as the reference's host code
+    J.inv_by = 1.f / by;
+    J.nbx = nbx;
+    J.nby = nby;
+    J.T = J.T0 = m.T;
+    J.tiles_y = (int)grid_of(nby, m.T);
+    J.tiles = (int)grid_of(nbx, m.T) * J.tiles_y;
+    J.G = 1;
+    J.wgs = J.tiles;
+    J.planes = m.planes;
+    J.blocks = (int)grid_of(num_nodes, BIN_NODES);
+    J.count = (int*)(w + m.count);
+    J.start = (int*)(w + m.start);
+    J.big_count = (int*)(w + m.big_count);
+    J.big_n = (int*)(w + m.big_n);
+    J.off = (int4*)(w + m.off);
+    J.big = (int*)(w + m.big);
+    J.geo = (float4*)(w + m.geo);
+    J.gr = (float*)(w + m.gr);
+    J.fixed = fixed;
+    J.room = (int)std::min<long long>(m.room, 0x7fffffff);
+    J.rho_mirror = rho;
+    J.over_map = over;
+    const Pick p{(const float2*)pos, nullptr, nullptr};
+    const size_t hist = (size_t)J.planes * J.tiles * sizeof(int);
+    const size_t acc = (size_t)J.planes * J.T * J.T * sizeof(unsigned long long);
+    AP_HIP(hipFuncSetAttribute((const void*)dm_count, hipFuncAttributeMaxDynamicSharedMemorySize, (int)hist));
+    AP_HIP(hipFuncSetAttribute((const void*)dm_tiles, hipFuncAttributeMaxDynamicSharedMemorySize, (int)acc));
+    hipLaunchKernelGGL(dm_clear, dim3(std::max(1, (int)grid_of((long long)J.planes * J.tiles, EB))), dim3(EB), 0, s,
+                       J);
+    AP_LAUNCHED();
+    if (num_nodes > 0) {
+        hipLaunchKernelGGL(dm_count, dim3(J.blocks), dim3(EB), hist, s, p, J);
+        AP_LAUNCHED();
+    }
+    hipLaunchKernelGGL(dm_scan, dim3(1), dim3(SCAN_T), 0, s, J);
+    AP_LAUNCHED();
+    if (num_nodes > 0) {
+        hipLaunchKernelGGL(dm_scatter, dim3(J.blocks), dim3(EB), 0, s, p, J);
+        AP_LAUNCHED();
+    }
+    hipLaunchKernelGGL(dm_tiles, dim3(J.wgs), dim3(TB_LEVEL), acc, s, p, J, target, (double*)(w + m.part));
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
+static int dropin_grad(const float* pos, const float* size, const float* ratio, const float* field,
+                       const float* wl_grad, const float* pins, const float* area, int num_nodes, int num_wl,
+                       const float* weight, const float* precond, float bx, float by, int nbx, int nby, float* grad,
+                       hipStream_t s) {
+    GatherArgs a{};
+    a.pos = Pick{(const float2*)pos, nullptr, nullptr};
+    a.g_out_a = (float2*)grad;
+    a.size = (const float2*)size;
+    a.ratio = ratio;
+    a.fx = field;                             // the reference's field [2][2 nbx][2 nby]: the [:nbx, :nby] quadrants
+    a.fy = field + 4LL * nbx * nby;
+    a.ld = 2 * nby;
+    a.cell_wl = num_wl > 0 ? (const float2*)wl_grad : nullptr;
+    a.pins = pins;
+    a.area = area;
+    a.num_nodes = num_nodes;
+    a.num_mov = num_wl;
+    a.inv_bx = 1.f / bx;
+    a.inv_by = 1.f / by;
+    a.nbx = nbx;
+    a.nby = nby;
+    a.use_density = 1;
+    a.weight_p = weight;
+    a.precond_p = precond;
+    hipLaunchKernelGGL(dm_gather, dim3(grid_of(num_nodes, EB)), dim3(EB), 0, s, a);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
 }  // namespace ap
 
 using namespace ap;
@@ -2518,6 +2687,56 @@ int ap_level_controller_state(ap_level* level, double* x) {
     AP_HIP(hipStreamSynchronize(L.stream));
     AP_HIP(hipMemcpy(x, L.ctl_x, NUM_X * sizeof(double), hipMemcpyDeviceToHost));
     return AP_OK;
+}
+
+// ---------------------------------------------------------------------- drop-in density map and gather (part 1)
+
+long long ap_density_map_work_bytes(int num_nodes, int num_mov, int nbx, int nby) {
+    if (num_nodes < 0 || num_mov < 0 || nbx <= 0 || nby <= 0) return -1;
+    return (long long)dropin_map_plan(num_nodes, num_mov, nbx, nby).bytes;
+}
+
+int ap_density_map_ws(const float* pos, const float* size, const float* ratio, int num_nodes, int num_mov, float bx,
+                      float by, int nbx, int nby, const float* fixed, float target, float* rho, float* over,
+                      void* work, long long work_bytes, void* stream) {
+    AP_REQUIRE(nbx > 0 && nby > 0 && num_nodes >= 0 && num_mov >= 0 && num_mov <= num_nodes,
+               "density_map: bad grid %d x %d", nbx, nby);
+    AP_REQUIRE(fixed && rho && over && (num_nodes == 0 || (pos && size && ratio)), "density_map: null pointer");
+    AP_REQUIRE(((uintptr_t)work & 15) == 0, "density_map: workspace not 16-byte aligned");
+    return dropin_map(pos, size, ratio, num_nodes, num_mov, bx, by, nbx, nby, fixed, target, rho, over, work,
+                      work_bytes, (hipStream_t)stream);
+}
+
+int ap_density_map(const float* pos, const float* size, const float* ratio, int num_nodes, int num_mov, float bx,
+                   float by, int nbx, int nby, unsigned long long* map, const float* fixed, float target, float* rho,
+                   float* over, void* stream) {
+    AP_REQUIRE(nbx > 0 && nby > 0 && num_nodes >= 0 && num_mov >= 0 && num_mov <= num_nodes,
+               "density_map: bad grid %d x %d", nbx, nby);
+    hipStream_t s = (hipStream_t)stream;
+    const long long need = (long long)dropin_map_plan(num_nodes, num_mov, nbx, nby).bytes;
+    const long long have = map ? 16LL * nbx * nby : 0;       // the caller's int64 [2][nbx][nby] map
+    if (have >= need)
+        return ap_density_map_ws(pos, size, ratio, num_nodes, num_mov, bx, by, nbx, nby, fixed, target, rho, over,
+                                 map, have, stream);
+    void* work = nullptr;                                    // stream-ordered, from the device's pool
+    AP_HIP(hipMallocAsync(&work, (size_t)need, s));
+    const int st = ap_density_map_ws(pos, size, ratio, num_nodes, num_mov, bx, by, nbx, nby, fixed, target, rho,
+                                     over, work, need, stream);
+    const hipError_t e = hipFreeAsync(work, s);
+    if (st) return st;
+    AP_HIP(e);
+    return AP_OK;
+}
+
+int ap_density_grad(const float* pos, const float* size, const float* ratio, const float* field, const float* wl_grad,
+                    const float* pins, const float* area, int num_nodes, int num_wl, const float* weight,
+                    const float* precond, float bx, float by, int nbx, int nby, float* grad, void* stream) {
+    AP_REQUIRE(nbx > 0 && nby > 0 && num_nodes >= 0 && num_wl >= 0 && num_wl <= num_nodes, "density_grad: bad sizes");
+    if (num_nodes == 0) return AP_OK;
+    AP_REQUIRE(pos && size && ratio && field && pins && area && weight && precond && grad && (num_wl == 0 || wl_grad),
+               "density_grad: null pointer");
+    return dropin_grad(pos, size, ratio, field, wl_grad, pins, area, num_nodes, num_wl, weight, precond, bx, by, nbx,
+                       nby, grad, (hipStream_t)stream);
 }
 
 }  // extern "C"

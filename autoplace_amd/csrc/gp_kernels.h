@@ -27,24 +27,6 @@ struct WaArgs {
 // block_net[b] = first net whose first pin is at or after b * WA_SPAN, b in [0, blocks]
 int launch_wa_block_table(const int* net_start, int num_nets, int blocks, int* block_net, hipStream_t s);
 
-struct DgradArgs {
-    const float2* pos;
-    const float2* size;
-    const float* ratio;
-    const float* field_x;
-    const float* field_y;
-    long long ld;       // row stride of the field planes
-    const float2* wl_grad;
-    const float* pins;
-    const float* area;
-    int num_nodes, num_wl;
-    const float* weight;
-    const float* precond;
-    float inv_bx, inv_by;
-    int nbx, nby;
-    float2* grad;
-};
-
 int launch_wa(const WaArgs& a, int num_pins, hipStream_t s);
 
 }  // namespace ap

@@ -4,14 +4,14 @@
 //             (cell, offset, position gather) are one per lane, coalesced; the pins are staged in LDS; one lane per
 //             net reduces max / min and the exponential sums over its pins in pin order (the same order as a
 //             sequential pass, so the per-net values are deterministic); one lane per pin writes its gradient.
-//             Block partial sums of (WA, HPWL) in FP64. wa_cell_kernel: per-cell sum over the cell CSR.
-//   density   integer fixed-point scatter (2^32 per bin area) with 64-bit atomics: order-free, hence deterministic
-//             and bit-identical to any other order; post kernels turn the maps into densities.
-//   gather    one lane per node: field at the node's bins, weight, WA gradient, preconditioner.
+//             Block partial sums of (WA, HPWL) in FP64. wa_cell_sum_kernel: per-cell sum over the cell CSR.
+//   step      Nesterov step and select: one node (two coordinates) per lane, 8-byte loads and stores.
 //   control   one lane, double precision, the decision order of the reference's control kernel.
+// The drop-in density map and gather are the GP engine's tile binning and gather (gp_engine.hip).
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <initializer_list>
 #include <mutex>
 #include <vector>
 
@@ -140,18 +140,20 @@ __global__ __launch_bounds__(WA_BLOCK) void wa_net_kernel(WaArgs a) {
     }
 }
 
-// grad[c] = sum of the pin gradients of cell c, in cell_pin order
-__global__ __launch_bounds__(256) void wa_cell_kernel(const float2* pin_grad, const int* cell_start,
-                                                      const int* cell_pin, int num_mov, float2* grad) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+// A cell's WA gradient: the gradients of its pins (pin order in pin_grad, the cell's pins listed by the cell CSR),
+// summed in CSR order, one cell per lane.
+__global__ __launch_bounds__(256) void wa_cell_sum_kernel(const float2* pin_grad, const int* cell_start,
+                                                          const int* cell_pin, int num_mov, float2* grad) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= num_mov) return;
-    float gx = 0, gy = 0;
-    for (int k = cell_start[c], e = cell_start[c + 1]; k < e; k++) {
-        float2 g = pin_grad[cell_pin[k]];
-        gx += g.x;
-        gy += g.y;
+    const int k0 = cell_start[c], k1 = cell_start[c + 1];
+    float2 sum = make_float2(0.f, 0.f);
+    for (int k = k0; k < k1; k++) {
+        const float2 q = pin_grad[cell_pin[k]];
+        sum.x += q.x;
+        sum.y += q.y;
     }
-    grad[c] = make_float2(gx, gy);
+    grad[c] = sum;
 }
 
 __global__ __launch_bounds__(256) void wa_block_table_kernel(const int* net_start, int num_nets, int blocks,
@@ -177,97 +179,71 @@ int launch_wa(const WaArgs& a, int num_pins, hipStream_t s) {
     return AP_OK;
 }
 
-// ------------------------------------------------------------------------------------------- density map
-
-__global__ __launch_bounds__(256) void density_map_kernel(const float2* pos, const float2* size, const float* ratio,
-                                                          int num_nodes, int num_mov, float inv_bx, float inv_by,
-                                                          int nbx, int nby, unsigned long long* map) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= num_nodes) return;
-    const float2 p = pos[i], sz = size[i];
-    const float r = ratio[i];
-    Span sx = make_span(p.x, sz.x, inv_bx, nbx), sy = make_span(p.y, sz.y, inv_by, nby);
-    unsigned long long* m = map + (i < num_mov ? 0 : (long long)nbx * nby);
-    for (int bx = sx.b0; bx <= sx.b1; bx++) {
-        float wx = r * span_overlap(sx, bx);
-        unsigned long long* row = m + (long long)bx * nby;
-        for (int by = sy.b0; by <= sy.b1; by++) {
-            unsigned long long q = (unsigned long long)((double)(wx * span_overlap(sy, by)) * kDensityScale);
-            if (q) atomicAdd(row + by, q);
-        }
-    }
-}
-
-// maps -> mirrored total density and the overflow map (drop-in layout)
-__global__ __launch_bounds__(256) void density_post_mirror_kernel(const unsigned long long* map, const float* fixed,
-                                                                  float target, int nbx, int nby, float* rho,
-                                                                  float* over) {
-    long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x, nb = (long long)nbx * nby;
-    if (b >= nb) return;
-    int bx = (int)(b / nby), by = (int)(b - (long long)bx * nby);
-    long long w = 2LL * nby;
-    int mx = 2 * nbx - 1 - bx, my = 2 * nby - 1 - by;
-    float mov = (float)((double)map[b] / kDensityScale) + fixed[b];
-    float r = mov + (float)((double)map[nb + b] / kDensityScale);
-    over[b] = fmaxf(mov - target, 0.f);
-    rho[bx * w + by] = r;
-    rho[bx * w + my] = r;
-    rho[mx * w + by] = r;
-    rho[mx * w + my] = r;
-}
-
-// ------------------------------------------------------------------------------------------- gather
-
-__global__ __launch_bounds__(256) void density_grad_kernel(DgradArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.num_nodes) return;
-    const float weight = *a.weight, precond = *a.precond;
-    const float2 p = a.pos[i], sz = a.size[i];
-    const float r = a.ratio[i];
-    Span sx = make_span(p.x, sz.x, a.inv_bx, a.nbx), sy = make_span(p.y, sz.y, a.inv_by, a.nby);
-    float gx = 0, gy = 0;
-    for (int bx = sx.b0; bx <= sx.b1; bx++) {
-        float wx = r * span_overlap(sx, bx);
-        const float* fxr = a.field_x + (long long)bx * a.ld;
-        const float* fyr = a.field_y + (long long)bx * a.ld;
-        for (int by = sy.b0; by <= sy.b1; by++) {
-            float w = wx * span_overlap(sy, by);
-            gx -= w * fxr[by];
-            gy -= w * fyr[by];
-        }
-    }
-    gx *= weight;
-    gy *= weight;
-    if (i < a.num_wl) {
-        float2 wl = a.wl_grad[i];
-        gx += wl.x;
-        gy += wl.y;
-    }
-    float scale = precond < 0 ? 1.f : 1.f / fmaxf(a.pins[i] + precond * a.area[i], 1.f);
-    a.grad[i] = make_float2(gx * scale, gy * scale);
-}
-
 // ------------------------------------------------------------------------------------------- Nesterov, select
+// (the density map and the gather of the drop-in entry points run on the engine's tile binning and gather:
+// gp_engine.hip, "drop-in density map and gather")
 
-__global__ __launch_bounds__(256) void nesterov_step_kernel(const float* u, const float* v, const float* g,
+// u' = v - alpha g, v' = clamp(u' + coef (u' - u), lo, hi). PAIRS: one node (its x and y) per lane with 8-byte
+// loads, an odd trailing coordinate by the first lane; else (a pointer not 8-byte aligned) one coordinate per lane.
+template <bool PAIRS>
+__global__ __launch_bounds__(256) void nesterov_pair_kernel(const float* u, const float* v, const float* g,
                                                             const float* lo, const float* hi, const float* alpha_p,
-                                                            const float* coef_p, int n, float* u_next, float* v_next) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+                                                            const float* coef_p, int n, float* u_next,
+                                                            float* v_next) {
     const float alpha = *alpha_p, coef = *coef_p;
-    float un = v[i] - alpha * g[i];
-    u_next[i] = un;
-    v_next[i] = fminf(fmaxf(un + coef * (un - u[i]), lo[i]), hi[i]);
+    auto step = [&](float uu, float vv, float gg, float l, float h, float& un, float& vn) {
+        un = vv - alpha * gg;
+        vn = fminf(fmaxf(un + coef * (un - uu), l), h);
+    };
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (!PAIRS) {
+        if (k < n) step(u[k], v[k], g[k], lo[k], hi[k], u_next[k], v_next[k]);
+        return;
+    }
+    if (k < n / 2) {
+        const float2 uu = ((const float2*)u)[k], vv = ((const float2*)v)[k], gg = ((const float2*)g)[k];
+        const float2 l = ((const float2*)lo)[k], h = ((const float2*)hi)[k];
+        float2 un, vn;
+        step(uu.x, vv.x, gg.x, l.x, h.x, un.x, vn.x);
+        step(uu.y, vv.y, gg.y, l.y, h.y, un.y, vn.y);
+        ((float2*)u_next)[k] = un;
+        ((float2*)v_next)[k] = vn;
+    }
+    if ((n & 1) && k == 0) step(u[n - 1], v[n - 1], g[n - 1], lo[n - 1], hi[n - 1], u_next[n - 1], v_next[n - 1]);
 }
 
-__global__ __launch_bounds__(256) void select_kernel(const double* S, float* u, float* v, float* g, const float* u_next,
-                                                     const float* v_next, const float* g_next, float* best, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || S[ACCEPT] == 0) return;
-    u[i] = u_next[i];
-    v[i] = v_next[i];
-    g[i] = g_next[i];
-    if (S[IS_BEST] != 0) best[i] = v_next[i];
+// an accepted step moves u, v, g to the pass's point; a best one is also kept (PAIRS as above)
+template <bool PAIRS>
+__global__ __launch_bounds__(256) void select_pair_kernel(const double* S, float* u, float* v, float* g,
+                                                          const float* u_next, const float* v_next,
+                                                          const float* g_next, float* best, int n) {
+    if (S[ACCEPT] == 0) return;                      // uniform: the whole launch returns
+    const bool keep = S[IS_BEST] != 0;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    auto one = [&](int i) {
+        u[i] = u_next[i];
+        v[i] = v_next[i];
+        g[i] = g_next[i];
+        if (keep) best[i] = v_next[i];
+    };
+    if (!PAIRS) {
+        if (k < n) one(k);
+        return;
+    }
+    if (k < n / 2) {
+        const float2 vn = ((const float2*)v_next)[k];
+        ((float2*)u)[k] = ((const float2*)u_next)[k];
+        ((float2*)v)[k] = vn;
+        ((float2*)g)[k] = ((const float2*)g_next)[k];
+        if (keep) ((float2*)best)[k] = vn;
+    }
+    if ((n & 1) && k == 0) one(n - 1);
+}
+
+static bool aligned8(std::initializer_list<const void*> ps) {
+    for (const void* p : ps)
+        if ((uintptr_t)p & 7) return false;
+    return true;
 }
 
 // ------------------------------------------------------------------------------------------- control
@@ -331,26 +307,10 @@ int ap_wa_wirelength(const float* pos, const int* pin_cell, const float* pin_off
     int st = launch_wa(a, num_pins, s);
     if (st) return st;
     if (num_mov > 0) {
-        hipLaunchKernelGGL(wa_cell_kernel, dim3(grid_of(num_mov, 256)), dim3(256), 0, s, (const float2*)pin_grad,
+        hipLaunchKernelGGL(wa_cell_sum_kernel, dim3(grid_of(num_mov, 256)), dim3(256), 0, s, (const float2*)pin_grad,
                            cell_start, cell_pin, num_mov, (float2*)grad);
         AP_LAUNCHED();
     }
-    return AP_OK;
-}
-
-int ap_density_map(const float* pos, const float* size, const float* ratio, int num_nodes, int num_mov, float bx,
-                   float by, int nbx, int nby, unsigned long long* map, const float* fixed, float target, float* rho,
-                   float* over, void* stream) {
-    AP_REQUIRE(nbx > 0 && nby > 0 && num_nodes >= 0, "density_map: bad grid %d x %d", nbx, nby);
-    hipStream_t s = (hipStream_t)stream;
-    if (num_nodes > 0) {
-        hipLaunchKernelGGL(density_map_kernel, dim3(grid_of(num_nodes, 256)), dim3(256), 0, s, (const float2*)pos,
-                           (const float2*)size, ratio, num_nodes, num_mov, 1.f / bx, 1.f / by, nbx, nby, map);
-        AP_LAUNCHED();
-    }
-    hipLaunchKernelGGL(density_post_mirror_kernel, dim3(grid_of((long long)nbx * nby, 256)), dim3(256), 0, s, map,
-                       fixed, target, nbx, nby, rho, over);
-    AP_LAUNCHED();
     return AP_OK;
 }
 
@@ -358,21 +318,13 @@ int ap_nesterov_step(const float* u, const float* v, const float* g, const float
                      const float* alpha, const float* coef, int n, float* u_next, float* v_next, void* stream) {
     AP_REQUIRE(n >= 0, "nesterov_step: negative size");
     if (n == 0) return AP_OK;
-    hipLaunchKernelGGL(nesterov_step_kernel, dim3(grid_of(n, 256)), dim3(256), 0, (hipStream_t)stream, u, v, g, lo, hi,
-                       alpha, coef, n, u_next, v_next);
-    AP_LAUNCHED();
-    return AP_OK;
-}
-
-int ap_density_grad(const float* pos, const float* size, const float* ratio, const float* field, const float* wl_grad,
-                    const float* pins, const float* area, int num_nodes, int num_wl, const float* weight,
-                    const float* precond, float bx, float by, int nbx, int nby, float* grad, void* stream) {
-    AP_REQUIRE(nbx > 0 && nby > 0 && num_nodes >= 0 && num_wl >= 0 && num_wl <= num_nodes, "density_grad: bad sizes");
-    if (num_nodes == 0) return AP_OK;
-    DgradArgs a{(const float2*)pos, (const float2*)size, ratio, field, field + 4LL * nbx * nby, 2LL * nby,
-                (const float2*)wl_grad, pins, area, num_nodes, num_wl, weight, precond, 1.f / bx, 1.f / by, nbx, nby,
-                (float2*)grad};
-    hipLaunchKernelGGL(density_grad_kernel, dim3(grid_of(num_nodes, 256)), dim3(256), 0, (hipStream_t)stream, a);
+    AP_REQUIRE(u && v && g && lo && hi && alpha && coef && u_next && v_next, "nesterov_step: null pointer");
+    if (aligned8({u, v, g, lo, hi, u_next, v_next}))
+        hipLaunchKernelGGL(nesterov_pair_kernel<true>, dim3(grid_of(n / 2 + 1, 256)), dim3(256), 0,
+                           (hipStream_t)stream, u, v, g, lo, hi, alpha, coef, n, u_next, v_next);
+    else
+        hipLaunchKernelGGL(nesterov_pair_kernel<false>, dim3(grid_of(n, 256)), dim3(256), 0, (hipStream_t)stream, u,
+                           v, g, lo, hi, alpha, coef, n, u_next, v_next);
     AP_LAUNCHED();
     return AP_OK;
 }
@@ -464,8 +416,13 @@ int ap_select_step(const double* S, float* u, float* v, float* g, const float* u
                    const float* g_next, float* best, int n, void* stream) {
     AP_REQUIRE(n >= 0, "select: negative size");
     if (n == 0) return AP_OK;
-    hipLaunchKernelGGL(select_kernel, dim3(grid_of(n, 256)), dim3(256), 0, (hipStream_t)stream, S, u, v, g, u_next,
-                       v_next, g_next, best, n);
+    AP_REQUIRE(S && u && v && g && u_next && v_next && g_next && best, "select: null pointer");
+    if (aligned8({u, v, g, u_next, v_next, g_next, best}))
+        hipLaunchKernelGGL(select_pair_kernel<true>, dim3(grid_of(n / 2 + 1, 256)), dim3(256), 0,
+                           (hipStream_t)stream, S, u, v, g, u_next, v_next, g_next, best, n);
+    else
+        hipLaunchKernelGGL(select_pair_kernel<false>, dim3(grid_of(n, 256)), dim3(256), 0, (hipStream_t)stream, S, u,
+                           v, g, u_next, v_next, g_next, best, n);
     AP_LAUNCHED();
     return AP_OK;
 }

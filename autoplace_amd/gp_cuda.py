@@ -45,12 +45,17 @@ def wa(pos, pin_cell, pin_off, net_start, cell_start, cell_pin, num_mov, gamma):
 def density_map(pos, size, ratio, num_mov, bx, by, nbx, nby, fixed, target):
     """-> (total density mirrored [2 nbx, 2 nby], density of cells + blocked area above the target [nbx, nby])
     (gp_cuda.cpp:41-48)."""
-    m = torch.zeros((2, nbx, nby), dtype=torch.int64, device=pos.device)
+    L = _lib.lib()
+    nbytes = L.ap_density_map_work_bytes(pos.size(0), num_mov, nbx, nby)
+    if nbytes < 0:
+        raise ValueError("density_map: bad sizes (%d nodes, %d cells, grid %d x %d)"
+                         % (pos.size(0), num_mov, nbx, nby))
+    work = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=pos.device)   # torch's caching allocator
     rho = pos.new_empty((2 * nbx, 2 * nby))
     over = pos.new_empty((nbx, nby))
-    _lib.check(_lib.lib().ap_density_map(F(pos, "pos"), F(size, "size"), F(ratio, "ratio"), pos.size(0), num_mov,
-                                         float(bx), float(by), nbx, nby, ctypes.c_void_p(m.data_ptr()),
-                                         F(fixed, "fixed"), float(target), F(rho), F(over), _stream()), "density_map")
+    _lib.check(L.ap_density_map_ws(F(pos, "pos"), F(size, "size"), F(ratio, "ratio"), pos.size(0), num_mov, float(bx),
+                                   float(by), nbx, nby, F(fixed, "fixed"), float(target), F(rho), F(over),
+                                   ctypes.c_void_p(work.data_ptr()), nbytes, _stream()), "density_map")
     return rho, over
 
 

@@ -44,18 +44,31 @@ AP_API int ap_wa_wirelength(const float* pos, const int* pin_cell, const float* 
                      const int* cell_start, const int* cell_pin, int num_nets, int num_mov, const float* gamma,
                      float* pin_grad, float* out, float* grad, int num_pins, void* stream);
 
-/* Density map (gp_cuda.cpp:6-7 `density_map`; kernels gp_cuda_kernel.cu:84-95,130-146).
- * map u64[2][nbx][nby] must be zero on entry (as the reference's binding allocates it); fixed f32[nbx][nby].
- * Writes rho f32[2 nbx][2 nby] (total density mirrored in both axes) and over f32[nbx][nby]. */
+/* Density map (gp_cuda.cpp:6-7 `density_map`; kernels gp_cuda_kernel.cu:84-95,130-146), computed by the GP
+ * engine's tile binning (per-tile LDS sums of the reference's 2^32 fixed-point contributions: bit-exact).
+ * map: the caller's u64[2][nbx][nby] (the reference's binding allocates it zeroed), here only a workspace: when it
+ * holds ap_density_map_work_bytes() bytes it is used as such (its contents on return are unspecified), else the call
+ * takes a stream-ordered workspace from the device's memory pool (hipMallocAsync) and frees it on the same stream;
+ * fixed f32[nbx][nby]. Writes rho f32[2 nbx][2 nby] (total density mirrored in both axes) and over f32[nbx][nby]. */
 AP_API int ap_density_map(const float* pos, const float* size, const float* ratio, int num_nodes, int num_mov, float bx,
                    float by, int nbx, int nby, unsigned long long* map, const float* fixed, float target, float* rho,
                    float* over, void* stream);
+
+/* Workspace bytes of ap_density_map_ws for these sizes (-1: bad sizes). */
+AP_API long long ap_density_map_work_bytes(int num_nodes, int num_mov, int nbx, int nby);
+
+/* ap_density_map with the caller's workspace (work_bytes >= ap_density_map_work_bytes(); any contents, 16-byte
+ * aligned): allocates nothing. Calls that share a workspace must not overlap. */
+AP_API int ap_density_map_ws(const float* pos, const float* size, const float* ratio, int num_nodes, int num_mov,
+                      float bx, float by, int nbx, int nby, const float* fixed, float target, float* rho, float* over,
+                      void* work, long long work_bytes, void* stream);
 
 /* Nesterov step (gp_cuda.cpp:8-9 `nesterov_step`; kernel gp_cuda_kernel.cu:157-170); n = coordinates. */
 AP_API int ap_nesterov_step(const float* u, const float* v, const float* g, const float* lo, const float* hi,
                      const float* alpha, const float* coef, int n, float* u_next, float* v_next, void* stream);
 
-/* Preconditioned gradient from the field (gp_cuda.cpp:10-11 `density_grad`; kernel gp_cuda_kernel.cu:102-125,148-153).
+/* Preconditioned gradient from the field (gp_cuda.cpp:10-11 `density_grad`; kernel gp_cuda_kernel.cu:102-125,148-153),
+ * computed by the GP engine's gather.
  * field f32[2][2 nbx][2 nby] (only [:, :nbx, :nby] is read); wl_grad f32[num_wl][2]; weight, precond f32[1]. */
 AP_API int ap_density_grad(const float* pos, const float* size, const float* ratio, const float* field, const float* wl_grad,
                     const float* pins, const float* area, int num_nodes, int num_wl, const float* weight,

@@ -86,6 +86,82 @@ def test_density_map_bit_exact(small):
     assert np.array_equal(over_dev.cpu().numpy(), over)
 
 
+def _odd_nodes(seed, n, nbx, nby, bx, by, big=12):
+    """Nodes that test the tile binning's edge cases: small ones, ones over tile corners, a few far wider than a
+    tile (the big-node path), ones hanging over the region's edges, zero-size ones."""
+    rng = np.random.default_rng(seed)
+    W, H = nbx * bx, nby * by
+    size = np.stack([rng.uniform(0.2, 1.8, n) * bx, rng.uniform(0.2, 1.8, n) * by], 1).astype(np.float32)
+    size[:big] = np.stack([rng.uniform(20, 90, big) * bx, rng.uniform(20, 90, big) * by], 1)
+    size[big:2 * big] = 0
+    pos = np.stack([rng.uniform(-2 * bx, W + 2 * bx, n), rng.uniform(-2 * by, H + 2 * by, n)], 1).astype(np.float32)
+    ratio = rng.uniform(0.3, 1.0, n).astype(np.float32)
+    return pos, size, ratio
+
+
+@pytest.mark.parametrize("case", [(20000, 15000, 96, 80, 2.0, 3.0), (20000, 20000, 64, 64, 1.0, 1.0),
+                                  (3000, 1000, 7, 300, 5.0, 0.5), (1, 1, 3, 3, 1.0, 1.0), (0, 0, 16, 16, 1.0, 1.0)])
+def test_density_map_edge_cases_bit_exact(case):
+    """The map of the tile binning against the oracle's per-node scatter, bit for bit: big nodes, nodes outside the
+    region, zero-size nodes, fillers (num_mov < num_nodes) or none, ragged grids, one node, no node."""
+    from autoplace_amd import gp_cuda
+    n, num_mov, nbx, nby, bx, by = case
+    pos, size, ratio = _odd_nodes(n + nbx, n, nbx, nby, bx, by, big=min(12, n // 3))
+    fixed = np.random.default_rng(1).uniform(0, 0.5, (nbx, nby)).astype(np.float32)
+    target = 0.9
+    m0, m1 = oden.density_maps(pos, size, ratio, num_mov, bx, by, nbx, nby)
+    rho, over = oden.density_post(m0, m1, fixed, target, nbx, nby)
+    for _ in range(2):                       # the second call reuses torch's cached workspace (stale contents)
+        rho_dev, over_dev = gp_cuda.density_map(_t(pos), _t(size), _t(ratio), num_mov, bx, by, nbx, nby, _t(fixed),
+                                                target)
+        assert np.array_equal(rho_dev.cpu().numpy(), oden.mirror(rho))
+        assert np.array_equal(over_dev.cpu().numpy(), over)
+
+
+def test_density_map_caller_map_as_workspace(small):
+    """ap_density_map (the reference's signature, with its u64 map): the map buffer serves as the workspace when it is
+    large enough, else a stream-ordered one is taken; both give the oracle's map."""
+    import ctypes
+    from autoplace_amd import _lib
+    data, p, pos, smooth, ratio, bx, by = small
+    nb = p.force_bin
+    fixed = (p.fixed_fraction * p.target).astype(np.float32)
+    m0, m1 = oden.density_maps(pos, smooth, ratio, data.num_mov, bx, by, nb, nb)
+    rho, over = oden.density_post(m0, m1, fixed, p.target, nb, nb)
+    need = _lib.lib().ap_density_map_work_bytes(len(pos), data.num_mov, nb, nb)
+    assert need > 0
+    V = lambda t: ctypes.c_void_p(t.data_ptr())       # noqa: E731
+    tp, ts, tr, tf = _t(pos), _t(smooth), _t(ratio), _t(fixed)      # held: their memory must outlive the call
+    for words in (2 * nb * nb, (need + 7) // 8):       # the reference's map size, and one that holds the workspace
+        m = torch.full((words,), -1, dtype=torch.int64, device=DEV)
+        r = torch.empty((2 * nb, 2 * nb), device=DEV)
+        o = torch.empty((nb, nb), device=DEV)
+        st = _lib.lib().ap_density_map(V(tp), V(ts), V(tr), len(pos), data.num_mov, bx, by, nb, nb, V(m), V(tf),
+                                       p.target, V(r), V(o), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+        _lib.check(st, "density_map")
+        assert np.array_equal(r.cpu().numpy(), oden.mirror(rho)) and np.array_equal(o.cpu().numpy(), over)
+
+
+def test_density_grad_edge_cases():
+    """The gather of the drop-in density_grad on nodes wider than 3 bins, outside the region, zero-size, and
+    num_wl < num_nodes (fillers get no wirelength term)."""
+    from autoplace_amd import gp_cuda
+    n, nbx, nby, bx, by = 5000, 40, 56, 2.0, 1.5
+    pos, size, ratio = _odd_nodes(7, n, nbx, nby, bx, by)
+    rng = np.random.default_rng(8)
+    fx, fy = rng.normal(size=(2, nbx, nby)).astype(np.float32)
+    field = np.zeros((2, 2 * nbx, 2 * nby), np.float32)
+    field[0, :nbx, :nby], field[1, :nbx, :nby] = fx, fy
+    num_wl = 3000
+    wl = rng.normal(size=(num_wl, 2)).astype(np.float32)
+    pins = rng.integers(0, 6, n).astype(np.float32)
+    area = size.prod(1).astype(np.float32)
+    ref = oden.density_grad(pos, size, ratio, fx, fy, wl, pins, area, 0.7, 0.01, bx, by, nbx, nby)
+    g = gp_cuda.density_grad(_t(pos), _t(size), _t(ratio), _t(field), _t(wl), _t(pins), _t(area), _t([0.7]),
+                             _t([0.01]), bx, by, nbx, nby).cpu().numpy()
+    assert norm_err(g, ref) < 1e-5
+
+
 @pytest.mark.parametrize("shape", [(16, 16, 3.0, 3.0), (64, 64, 12.5, 12.5), (128, 128, 6.3, 6.3), (48, 40, 7.0, 5.5),
                                    (512, 512, 1.7, 1.7)])
 def test_poisson_gemm(shape):
@@ -189,6 +265,32 @@ def test_nesterov_and_select():
     S[octl.K["is_best"]] = 1
     gp_cuda.select(S, tu, tv, tg, du, dv, gn, best)
     assert torch.equal(tu, du) and torch.equal(tv, dv) and torch.equal(tg, gn) and torch.equal(best, dv)
+
+
+@pytest.mark.parametrize("offset", [0, 1])
+def test_nesterov_and_select_odd_and_unaligned(offset):
+    """An odd count of coordinates, and tensors that start 4 bytes past an 8-byte boundary (the one-coordinate-per-
+    lane kernels): the step equals the oracle's to the bit, select moves every coordinate."""
+    from autoplace_amd import gp_cuda
+    rng = np.random.default_rng(12)
+    n = 20001
+
+    def dev(a):
+        buf = torch.empty(n + offset, dtype=torch.float32, device=DEV)
+        buf[offset:] = torch.as_tensor(a, dtype=torch.float32)
+        return buf[offset:]
+    u, v, g = (rng.normal(size=n).astype(np.float32) for _ in range(3))
+    lo, hi = np.full(n, -1.2, np.float32), np.full(n, 1.3, np.float32)
+    un, vn = octl.nesterov_step(u, v, g, lo, hi, 0.37, 0.61)
+    du, dv = gp_cuda.nesterov_step(dev(u), dev(v), dev(g), dev(lo), dev(hi), _t([0.37]), _t([0.61]))
+    assert rel_err(du.cpu().numpy(), un) < 1e-6 and rel_err(dv.cpu().numpy(), vn) < 1e-6
+    S = torch.zeros(21, dtype=torch.float64, device=DEV)
+    S[octl.K["accept"]] = 1
+    S[octl.K["is_best"]] = 1
+    tu, tv, tg, best, gn = dev(u), dev(v), dev(g), dev(np.zeros(n)), dev(g * 2)
+    dun, dvn = dev(un), dev(vn)
+    gp_cuda.select(S, tu, tv, tg, dun, dvn, gn, best)
+    assert torch.equal(tu, dun) and torch.equal(tv, dvn) and torch.equal(tg, gn) and torch.equal(best, dvn)
 
 
 def test_control_matches_oracle_fp64():
