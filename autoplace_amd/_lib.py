@@ -69,6 +69,9 @@ _SIGNATURES = {
     "ap_batch_tile_layout": ([P, P], I),
     "ap_cluster_create": ([I, P, P, I, P, I, P, ctypes.POINTER(P)], I),
     "ap_cluster_round": ([P, P, F, ctypes.c_longlong, I, ctypes.POINTER(I), P], I),
+    "ap_cluster_seed_heads": ([P, ctypes.c_uint], I),
+    "ap_cluster_round_seeded": ([P, F, ctypes.c_longlong, I, ctypes.POINTER(I), P], I),
+    "ap_head_draws": ([ctypes.c_uint, I, P], I),
     "ap_cluster_num": ([P], I),
     "ap_cluster_edges": ([P], I),
     "ap_cluster_result": ([P, P, P, P], I),

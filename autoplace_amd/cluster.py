@@ -3,8 +3,9 @@ cluster (the behaviour of src/cluster.py:1-144 of the reference).
 
 Clustering runs in the library's kernels (ap_cluster_*, include/autoplace_gp.h part 5): the weighted cell graph of
 the small nets, then rounds of star matching. The reference draws each round's heads from torch's CPU generator; the
-same draws are made here on the host (head_draws) and handed to the device. Weights and areas are integers and a
-score is one IEEE division, so the clusters are the reference's bit for bit. The coarse problem -- cluster sizes, the clusters' pins
+library makes the same draws (MT19937 seeded as torch.manual_seed, on the host, the next round's while the device
+runs a round) and hands them to the device. Weights and areas are integers and a score is one IEEE division, so the
+clusters are the reference's bit for bit. The coarse problem -- cluster sizes, the clusters' pins
 and netlist CSR, the free-area fraction of its grid and its density layer -- is built from device tensors, inside the
 GP clock as the reference builds it, without a round trip through the host.
 """
@@ -40,18 +41,6 @@ def merge_cap(area, big, ratio):
                                                                             rounding_mode="floor"))
 
 
-def head_draws(seed):
-    """The round-by-round head draws of the reference, torch.rand(num, generator=manual_seed(seed)) < 0.5
-    (cluster.py:65), as a generator of bool arrays: torch's CPU generator is MT19937 seeded as numpy's legacy
-    RandomState, and torch.rand keeps the low 24 bits of each 32-bit output, so a draw is below 0.5 exactly when bit
-    23 of the output is clear. numpy yields the same outputs several times faster. -> send(num) for each round."""
-    rs = np.random.RandomState(seed)
-    num = yield None
-    while True:
-        raw = rs.randint(0, 2 ** 32, size=num, dtype=np.uint32)
-        num = yield (raw & np.uint32(1 << 23)) == 0
-
-
 def cluster_cells(pin_cell, net_start, area, ratio, seed, max_degree=8, per_head=3, max_rounds=12):
     """pin_cell int32 [P] grouped by net (index len(area) = fixed pin) and net_start int32 [nets + 1] on the device;
     area int64 [n] on the device. -> (cluster of every cell int64 [n], cluster areas int64 [K]), on the device.
@@ -64,16 +53,14 @@ def cluster_cells(pin_cell, net_start, area, ratio, seed, max_degree=8, per_head
     try:
         big = int(20 * area.sum() // n)                # cells above 20 x the mean area stay alone
         cap = merge_cap(area, big, ratio)
-        draws = head_draws(seed)
-        next(draws)
+        _lib.check(lib.ap_cluster_seed_heads(h, seed), "cluster_seed_heads")
         num = n
         for _ in range(max_rounds):
             if num * ratio <= n:
                 break
-            heads = torch.from_numpy(draws.send(num).view(np.uint8)).to(device)
             after = ctypes.c_int()
-            _lib.check(lib.ap_cluster_round(h, _p(heads), cap, big, per_head, ctypes.byref(after), _stream(device)),
-                       "cluster_round")
+            _lib.check(lib.ap_cluster_round_seeded(h, cap, big, per_head, ctypes.byref(after), _stream(device)),
+                       "cluster_round_seeded")
             if after.value == num:
                 break
             num = after.value

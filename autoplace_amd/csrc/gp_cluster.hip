@@ -13,6 +13,8 @@
 // Weights and areas are integers and the scores one IEEE division each, so the clusters are those of the
 // reference's torch operators on any device, bit for bit. Each step is a thread per cluster or per edge; the
 // sorts and scans are rocPRIM's.
+#include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -220,6 +222,82 @@ inline unsigned bits_for(unsigned long long v) {   // bits that hold every value
     return b;
 }
 
+// The heads' draws: torch.rand(num, generator=...) < 0.5 of the reference (cluster.py:65). torch's CPU generator is
+// MT19937 (Matsumoto and Nishimura, 1998) seeded by its standard initialisation, and a float32 draw keeps the low
+// 24 bits of one 32-bit output as k / 2^24, so a draw is below 0.5 exactly when bit 23 of the output is clear. One
+// stream serves every round in order. The flags are made ahead: a round needs as many as there are clusters before
+// it, at most the count of the round before, so while the device runs a round the host draws that many for the next
+// (the surplus carries over to the round after).
+struct HeadStream {
+    uint32_t mt[624];
+    int idx = 624;
+    std::vector<unsigned char> ahead;   // drawn flags not yet handed to a round: ahead[used, size)
+    size_t used = 0;
+    unsigned char* pinned = nullptr;    // staging for the copy to the device (num_cells bytes)
+    unsigned char* dev = nullptr;       // the round's heads on the device
+    bool seeded = false;
+
+    void seed(uint32_t s) {
+        mt[0] = s;
+        for (uint32_t i = 1; i < 624; i++) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + i;
+        idx = 624;
+        ahead.clear();
+        used = 0;
+        seeded = true;
+    }
+    static uint32_t mix(uint32_t a, uint32_t b, uint32_t c) {     // word i from words i, i + 1 and i + 397
+        const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+        return c ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
+    }
+    void twist() {       // the next 624 words, in three loops without wrap-around indices (each vectorises)
+        int i = 0;
+        for (; i < 624 - 397; i++) mt[i] = mix(mt[i], mt[i + 1], mt[i + 397]);
+        for (; i < 623; i++) mt[i] = mix(mt[i], mt[i + 1], mt[i + 397 - 624]);
+        mt[623] = mix(mt[623], mt[0], mt[396]);
+        idx = 0;
+    }
+    // `count` more flags (1: head) at the end of `ahead`
+    void draw(size_t count) {
+        const size_t at = ahead.size();
+        ahead.resize(at + count);
+        unsigned char* out = ahead.data() + at;
+        size_t k = 0;
+        while (k < count) {
+            if (idx >= 624) twist();
+            const size_t m = std::min<size_t>(624 - idx, count - k);
+            for (size_t j = 0; j < m; j++) {
+                uint32_t y = mt[idx + j];
+                y ^= y >> 11;
+                y ^= (y << 7) & 0x9d2c5680u;
+                y ^= (y << 15) & 0xefc60000u;
+                y ^= y >> 18;
+                out[k + j] = (unsigned char)(((y >> 23) & 1u) ^ 1u);
+            }
+            idx += (int)m;
+            k += m;
+        }
+    }
+    // hand `count` flags to a round: drawn now if not ahead, staged and copied to `dev` on the stream
+    int take(size_t count, hipStream_t s) {
+        if (ahead.size() - used < count) draw(count - (ahead.size() - used));
+        std::memcpy(pinned, ahead.data() + used, count);
+        used += count;
+        if (used == ahead.size()) {
+            ahead.clear();
+            used = 0;
+        } else if (used > (ahead.size() >> 1)) {     // keep the carried surplus at the front
+            ahead.erase(ahead.begin(), ahead.begin() + (long)used);
+            used = 0;
+        }
+        AP_HIP(hipMemcpyAsync(dev, pinned, count, hipMemcpyHostToDevice, s));
+        return AP_OK;
+    }
+    // draw ahead up to `count` unused flags (the next round's need is at most this)
+    void prepare(size_t count) {
+        if (ahead.size() - used < count) draw(count - (ahead.size() - used));
+    }
+};
+
 struct Cluster {
     int n = 0, num = 0;
     unsigned long long cap_edges = 0;   // pair entries at the start: no later edge list is longer
@@ -234,6 +312,7 @@ struct Cluster {
     void* temp = nullptr;
     size_t temp_bytes = 0;
     std::vector<void*> mem;
+    HeadStream heads;
 
     template <typename T>
     int alloc(T** p, unsigned long long count) {
@@ -246,11 +325,15 @@ struct Cluster {
         mem.clear();
         if (temp) (void)hipFree(temp);
         temp = nullptr;
+        if (heads.pinned) (void)hipHostFree(heads.pinned);
+        heads.pinned = nullptr;
     }
     int sort_and_merge(unsigned long long entries, unsigned long long num_keys, hipStream_t s);
     int create(int cells, const int* pin_cell, const int* net_start, int num_nets, const long long* area_in,
                int max_degree, hipStream_t s);
-    int round(const unsigned char* head, float cap, long long big, int per_head, hipStream_t s);
+    // while_busy: host work run once the round's first kernels are queued, before the host waits for them
+    int round(const unsigned char* head, float cap, long long big, int per_head, hipStream_t s,
+              HeadStream* while_busy = nullptr);
 };
 
 // sort `entries` (key, weight) pairs by key, sum equal keys into the edge list, drop the self-loop key num_keys^2
@@ -349,7 +432,8 @@ int Cluster::create(int cells, const int* pin_cell, const int* net_start, int nu
     return sort_and_merge(total, (unsigned long long)n, s);
 }
 
-int Cluster::round(const unsigned char* head, float cap, long long big, int per_head, hipStream_t s) {
+int Cluster::round(const unsigned char* head, float cap, long long big, int per_head, hipStream_t s,
+                   HeadStream* while_busy) {
     const dim3 g(grid_of(num, CLU_BLOCK)), b(CLU_BLOCK);
     hipLaunchKernelGGL(clu_best_head, g, b, 0, s, row_start, ej, ew, num, head, area, big, cap, best_to, best_score);
     AP_LAUNCHED();
@@ -377,6 +461,7 @@ int Cluster::round(const unsigned char* head, float cap, long long big, int per_
     int last_rank = 0, last_kept = 0;
     AP_HIP(hipMemcpyAsync(&last_rank, rank + num - 1, sizeof(int), hipMemcpyDeviceToHost, s));
     AP_HIP(hipMemcpyAsync(&last_kept, kept + num - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (while_busy) while_busy->prepare((size_t)num);         // the next round's heads, beside this round
     AP_HIP(hipStreamSynchronize(s));
     const int num_new = last_rank + last_kept;
     if (num_new == num) return AP_OK;                             // nothing merged: the state stays
@@ -435,6 +520,41 @@ int ap_cluster_round(ap_cluster* c, const unsigned char* head, float cap, long l
     const int st = c->C.round(head, cap, big, per_head, (hipStream_t)stream);
     if (st) return st;
     *num_out = c->C.num;
+    return AP_OK;
+}
+
+int ap_cluster_seed_heads(ap_cluster* c, unsigned int seed) {
+    AP_REQUIRE(c, "cluster_seed_heads: null argument");
+    Cluster& C = c->C;
+    if (!C.heads.pinned) AP_HIP(hipHostMalloc((void**)&C.heads.pinned, (size_t)C.n, hipHostMallocDefault));
+    if (!C.heads.dev) {
+        const int st = C.alloc(&C.heads.dev, (unsigned long long)C.n);
+        if (st) return st;
+    }
+    C.heads.seed(seed);
+    return AP_OK;
+}
+
+int ap_cluster_round_seeded(ap_cluster* c, float cap, long long big, int per_head, int* num_out, void* stream) {
+    AP_REQUIRE(c && num_out, "cluster_round_seeded: null argument");
+    AP_REQUIRE(c->C.heads.seeded, "cluster_round_seeded: no head stream (ap_cluster_seed_heads)");
+    AP_REQUIRE(per_head >= 1 && per_head <= 4, "cluster_round_seeded: per_head %d outside 1 .. 4", per_head);
+    Cluster& C = c->C;
+    hipStream_t s = (hipStream_t)stream;
+    int st = C.heads.take((size_t)C.num, s);
+    if (st) return st;
+    st = C.round(C.heads.dev, cap, big, per_head, s, &C.heads);
+    if (st) return st;
+    *num_out = C.num;
+    return AP_OK;
+}
+
+int ap_head_draws(unsigned int seed, int count, unsigned char* out) {
+    AP_REQUIRE(count >= 0 && (count == 0 || out), "head_draws: bad arguments");
+    HeadStream h;
+    h.seed(seed);
+    h.draw((size_t)count);
+    if (count) std::memcpy(out, h.ahead.data(), (size_t)count);
     return AP_OK;
 }
 

@@ -38,9 +38,12 @@ def build_problem(data, args, log=None):
     node_size = np.concatenate([data.size, fillers])
     node_pins = np.concatenate([data.cell_pins, np.zeros(len(node_size) - data.num_mov)])
     pos0 = init_positions(data, node_size, args.seed).numpy()
+    # the scheduler's totals (node areas summed in float64 as one contiguous array)
+    total_area = float((node_size[:, 0] * node_size[:, 1]).sum())
     return SimpleNamespace(num_bin=num_bin, force_bin=force_bin, judge_fraction=judge_fraction,
                            fixed_fraction=fixed_fraction, node_size=node_size, node_pins=node_pins,
                            target=args.target_density, pos0=pos0, num_mov=data.num_mov, num_fillers=len(fillers),
                            width=data.width, height=data.height, mov_area=data.mov_area, cell_pins=data.cell_pins,
                            pin_cell=data.pin_cell, pin_off=data.pin_off, pin_net=data.pin_net, num_nets=data.num_nets,
-                           fraction_of=lambda nb: data.fixed_fraction(nb, nb), data=data)
+                           fraction_of=lambda nb: data.fixed_fraction(nb, nb), data=data,
+                           total_area=total_area, total_pins=float(data.cell_pins.sum()))

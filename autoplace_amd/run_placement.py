@@ -78,8 +78,7 @@ def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=N
     log = gpu_loop.as_log(log)
     _sync(device)
     start = time.time()
-    ps = _scheduler(density, n, data.mov_area, float(data.cell_pins.sum()), float(node_size.prod(1).sum()),
-                    len(node_size), args)
+    ps = _scheduler(density, n, data.mov_area, prob.total_pins, prob.total_area, len(node_size), args)
     objective = Objective(wirelength, density, ps)
     stats, evals, first_iter, start_iters = {}, 0, 0, 0
     if args.cluster_ratio > 1:

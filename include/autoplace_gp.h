@@ -303,6 +303,16 @@ AP_API int ap_cluster_create(int num_cells, const int* pin_cell, const int* net_
  * the clusters before it when nothing merged; the state is then unchanged). Synchronises the stream. */
 AP_API int ap_cluster_round(ap_cluster* c, const unsigned char* head, float cap, long long big, int per_head,
                             int* num_out, void* stream);
+/* The heads drawn by the library instead (the same draws): ap_cluster_seed_heads starts the stream of
+ * torch.manual_seed(seed) (MT19937 with its standard seeding; a head is a float32 draw below 0.5, i.e. bit 23 of the
+ * 32-bit output clear), and every ap_cluster_round_seeded takes the next `num` draws of it, `num` the clusters before
+ * the round, as ap_cluster_round with those heads would. The host draws the next round's heads while the device runs
+ * a round. Synchronises the stream. */
+AP_API int ap_cluster_seed_heads(ap_cluster* c, unsigned int seed);
+AP_API int ap_cluster_round_seeded(ap_cluster* c, float cap, long long big, int per_head, int* num_out,
+                                   void* stream);
+/* Host only: the first `count` head flags of the stream of `seed` (1: head), u8 [count] in host memory. */
+AP_API int ap_head_draws(unsigned int seed, int count, unsigned char* out);
 AP_API int ap_cluster_num(const ap_cluster* c);
 AP_API int ap_cluster_edges(const ap_cluster* c);
 /* cid i64 [num_cells]: the cluster of every cell; area i64 [num]: the clusters' areas. */

@@ -1,4 +1,4 @@
-"""Host logic of the device clustering: the head draws equal the reference's torch.rand(num) < 0.5 from a
+"""Host logic of the device clustering: the library's head draws equal the reference's torch.rand(num) < 0.5 from a
 manual_seed generator round after round (cluster.py:65), and the merge cap is the reference's float32 expression
 (cluster.py:61) as the oracle restates it (oracle/cluster.py, whose cluster ids the reference's pin)."""
 import numpy as np
@@ -6,16 +6,22 @@ import pytest
 import torch
 
 
-@pytest.mark.parametrize("seed", [0, 1, 5, 4242])
+@pytest.mark.parametrize("seed", [0, 1, 5, 4242, 2 ** 32 - 1])
 def test_head_draws_are_torch_rand_below_half(seed):
-    from autoplace_amd.cluster import head_draws
+    """The library's head draws (host code, ap_head_draws): the stream of torch.manual_seed(seed), consumed round
+    after round in pieces of shrinking size, gives torch.rand(num) < 0.5 of every round -- across MT19937's 624-word
+    blocks and at the ends of pieces."""
+    import ctypes
+    from autoplace_amd import _lib
     gen = torch.Generator().manual_seed(seed)
-    draws = head_draws(seed)
-    next(draws)
-    for num in (200000, 126671, 87234, 87234, 50000, 3, 1):      # shrinking, as the rounds go
+    pieces = (200000, 126671, 87234, 87234, 624, 623, 50000, 3, 1, 0)     # shrinking, as the rounds go
+    stream = np.empty(sum(pieces), np.uint8)
+    _lib.check(_lib.lib().ap_head_draws(seed, len(stream), stream.ctypes.data_as(ctypes.c_void_p)), "head_draws")
+    at = 0
+    for num in pieces:
         ref = (torch.rand(num, generator=gen) < 0.5).numpy()
-        mine = draws.send(num)
-        assert mine.dtype == bool and np.array_equal(mine, ref), num
+        assert np.array_equal(stream[at:at + num].astype(bool), ref), num
+        at += num
 
 
 def test_merge_cap_is_the_reference_float32_expression():
