@@ -729,12 +729,32 @@ struct GatherArgs {
 };
 
 // a cell's WA gradient: the sum of its pins' gradients, which the WA kernel wrote in cell order (contiguous)
+// (four pins' loads issued before the first of them is added, then added in pin order: the sum of the one-at-a-time
+// loop, bit for bit, with a quarter of its dependent memory waits)
 __device__ __forceinline__ float2 cell_pin_sum(const GatherArgs& a, int i) {
     float wx = 0, wy = 0;
-    for (int k = a.cell_start[i], e = a.cell_start[i + 1]; k < e; k++) {
-        const float2 q = a.pin_grad[k];
-        wx += q.x;
-        wy += q.y;
+    int k = a.cell_start[i];
+    const int e = a.cell_start[i + 1];
+    for (; k + 4 <= e; k += 4) {
+        float2 q[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) q[j] = a.pin_grad[k + j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            wx += q[j].x;
+            wy += q[j].y;
+        }
+    }
+    if (k < e) {                                    // the last 1 .. 3 pins, their loads also issued together
+        float2 q[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) q[j] = k + j < e ? a.pin_grad[k + j] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            if (k + j < e) {
+                wx += q[j].x;
+                wy += q[j].y;
+            }
     }
     return make_float2(wx, wy);
 }

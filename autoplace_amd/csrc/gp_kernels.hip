@@ -141,17 +141,28 @@ __global__ __launch_bounds__(WA_BLOCK) void wa_net_kernel(WaArgs a) {
 }
 
 // A cell's WA gradient: the gradients of its pins (pin order in pin_grad, the cell's pins listed by the cell CSR),
-// summed in CSR order, one cell per lane.
+// summed in CSR order, one cell per lane; four pins' index and gradient loads are issued before the first of them is
+// added (the same sums as one pin at a time, with a quarter of the dependent memory waits).
 __global__ __launch_bounds__(256) void wa_cell_sum_kernel(const float2* pin_grad, const int* cell_start,
                                                           const int* cell_pin, int num_mov, float2* grad) {
     const int c = blockIdx.x * 256 + threadIdx.x;
     if (c >= num_mov) return;
-    const int k0 = cell_start[c], k1 = cell_start[c + 1];
+    const int k1 = cell_start[c + 1];
+    int k = cell_start[c];
     float2 sum = make_float2(0.f, 0.f);
-    for (int k = k0; k < k1; k++) {
-        const float2 q = pin_grad[cell_pin[k]];
-        sum.x += q.x;
-        sum.y += q.y;
+    for (; k < k1; k += 4) {
+        int p[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) p[j] = k + j < k1 ? cell_pin[k + j] : -1;
+        float2 q[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) q[j] = p[j] >= 0 ? pin_grad[p[j]] : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (p[j] >= 0) {
+                sum.x += q[j].x;
+                sum.y += q[j].y;
+            }
     }
     grad[c] = sum;
 }

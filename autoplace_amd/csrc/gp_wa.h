@@ -6,6 +6,8 @@
 
 namespace ap {
 
+constexpr int WA_PLAN_PER_WAVE = 2;   // wavefronts of the plan run by one launched wavefront
+
 struct WaveArgs {
     const float2* pos;
     const float2* pos_other;
@@ -32,7 +34,7 @@ struct WaPlan {
     int create(const int* d_net_start, int num_nets, const int* d_pin_cell, const float* d_pin_off,
                const int* d_pin_slot, int num_pins);
     void destroy();
-    int wave_blocks() const { return (num_waves + 3) / 4; }
+    int wave_blocks() const { return (num_waves + 4 * WA_PLAN_PER_WAVE - 1) / (4 * WA_PLAN_PER_WAVE); }
     int partial_count() const { return wave_blocks() + num_big; }   // (WA, HPWL) pairs written per launch
     int launch(const WaArgs& a, double* partial, hipStream_t s) const;
     WaveArgs wave_args(const WaArgs& a, double* partial) const;

@@ -58,19 +58,16 @@ __device__ __forceinline__ float group_reduce(float v, Op op) {
     return v;
 }
 
+// q: the position of the pin's cell (zero for a fixed pin or an empty lane), loaded by the caller
 template <int LG>
-__device__ __forceinline__ void wa_group(const WaveArgs& a, const float2* pos, const int4& ln, int lane,
-                                         double (&acc)[2]) {
+__device__ __forceinline__ void wa_group(const WaveArgs& a, float2 q, const int4& ln, int lane, double (&acc)[2]) {
     constexpr int G = 1 << LG;
     const int sub = lane & (G - 1);
     const int c = ln.x, slot = ln.y;
     const bool live = c >= 0;
     float2 x = make_float2(__int_as_float(ln.z), __int_as_float(ln.w));
-    if (live && c < a.num_mov) {
-        const float2 q = pos[c];
-        x.x += q.x;
-        x.y += q.y;
-    }
+    x.x += q.x;
+    x.y += q.y;
     const float ig = 1.0f / *a.gamma;
     const float NEG = -3.402823466e38f, POS = 3.402823466e38f;
     const float hx = group_reduce<LG>(live ? x.x : NEG, MaxOp()), hy = group_reduce<LG>(live ? x.y : NEG, MaxOp());
@@ -90,27 +87,42 @@ __device__ __forceinline__ void wa_group(const WaveArgs& a, const float2* pos, c
         gr.y = epy * (1 + (x.y - xpy) * ig) * rpy - eny * (1 - (x.y - xny) * ig) * rny;
         if (slot >= 0) a.pin_grad[slot] = gr;
         if (sub == 0) {
-            acc[0] = (double)((xpx - xnx) + (xpy - xny));
-            acc[1] = (double)((hx - lx) + (hy - ly));
+            acc[0] += (double)((xpx - xnx) + (xpy - xny));
+            acc[1] += (double)((hx - lx) + (hy - ly));
         }
     }
 }
 
+// Each wavefront of the launch runs WA_PLAN_PER_WAVE consecutive wavefronts of the plan, their lane data and cell
+// positions loaded before the first is reduced (their memory waits overlap).
 __device__ __forceinline__ void wa_wave_body(const WaveArgs& a, int wg, int nwg) {
     __shared__ double red[16 * 2];
     const int lane = threadIdx.x & 63;
-    const int w = xcd_remap(wg, nwg) * 4 + (threadIdx.x >> 6);
+    const int w0 = (xcd_remap(wg, nwg) * 4 + (threadIdx.x >> 6)) * WA_PLAN_PER_WAVE;
     const float2* pos = a.flag ? (*a.flag == 0 ? a.pos_other : a.pos) : a.pos;
     double acc[2] = {0.0, 0.0};
-    if (w < a.num_waves) {
-        const int4 ln = a.lanes[(long long)w * 64 + lane];
-        switch (a.lg[w]) {   // uniform over the wavefront
-            case 1: wa_group<1>(a, pos, ln, lane, acc); break;
-            case 2: wa_group<2>(a, pos, ln, lane, acc); break;
-            case 3: wa_group<3>(a, pos, ln, lane, acc); break;
-            case 4: wa_group<4>(a, pos, ln, lane, acc); break;
-            case 5: wa_group<5>(a, pos, ln, lane, acc); break;
-            default: wa_group<6>(a, pos, ln, lane, acc); break;
+    int4 ln[WA_PLAN_PER_WAVE];
+    int lg[WA_PLAN_PER_WAVE];
+#pragma unroll
+    for (int k = 0; k < WA_PLAN_PER_WAVE; k++) {
+        const bool in = w0 + k < a.num_waves;       // uniform over the wavefront
+        ln[k] = in ? a.lanes[(long long)(w0 + k) * 64 + lane] : make_int4(-1, -1, 0, 0);
+        lg[k] = in ? a.lg[w0 + k] : 0;
+    }
+    float2 q[WA_PLAN_PER_WAVE];
+#pragma unroll
+    for (int k = 0; k < WA_PLAN_PER_WAVE; k++)
+        q[k] = ln[k].x >= 0 && ln[k].x < a.num_mov ? pos[ln[k].x] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < WA_PLAN_PER_WAVE; k++) {
+        switch (lg[k]) {   // uniform over the wavefront
+            case 0: break;
+            case 1: wa_group<1>(a, q[k], ln[k], lane, acc); break;
+            case 2: wa_group<2>(a, q[k], ln[k], lane, acc); break;
+            case 3: wa_group<3>(a, q[k], ln[k], lane, acc); break;
+            case 4: wa_group<4>(a, q[k], ln[k], lane, acc); break;
+            case 5: wa_group<5>(a, q[k], ln[k], lane, acc); break;
+            default: wa_group<6>(a, q[k], ln[k], lane, acc); break;
         }
     }
     block_sum<2>(acc, red);
