@@ -190,13 +190,21 @@ __device__ __forceinline__ void bin_count_body(const Pick& pos_p, const BinJob& 
     __syncthreads();
     const float2* pos = pos_p.get(true);
     int rank[BIN_PER_LANE][4], b0[BIN_PER_LANE];
+    float4 nd[BIN_PER_LANE];    // every node's centre and size loaded before the first is used (one memory wait)
+#pragma unroll
+    for (int q = 0; q < BIN_PER_LANE; q++) {
+        const int i = base + q * EB + threadIdx.x;
+        const int k = i < J.num_nodes ? i : 0;
+        const float2 p = pos[k], sz = J.size[k];
+        nd[q] = make_float4(p.x, p.y, sz.x, sz.y);
+    }
 #pragma unroll
     for (int q = 0; q < BIN_PER_LANE; q++) {
         rank[q][0] = rank[q][1] = rank[q][2] = rank[q][3] = -1;
         b0[q] = 0;
         const int i = base + q * EB + threadIdx.x;
         if (i >= J.num_nodes) continue;
-        const float2 p = pos[i], sz = J.size[i];
+        const float2 p = make_float2(nd[q].x, nd[q].y), sz = make_float2(nd[q].z, nd[q].w);
         const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
         if (sx.b0 > sx.b1 || sy.b0 > sy.b1) continue;
         int t0, wx, wy;
@@ -460,15 +468,24 @@ __device__ __forceinline__ void bin_fill_body(const Pick& pos_p, const BinJob& f
     unsigned rk[BIN_PER_LANE][2];
     float4 nd[BIN_PER_LANE];   // the nodes' centre and size, kept for the second phase (LDS, not registers, bounds the
                                // force grid's occupancy)
+    float nr[BIN_PER_LANE];    // and their ratio
+    // every node's loads issued before the first is used (one memory wait for the eight, not eight in a row; a lane
+    // past the grid's nodes reads node 0 and drops it)
+#pragma unroll
+    for (int q = 0; q < BIN_PER_LANE; q++) {
+        const int i = base + q * EB + threadIdx.x;
+        const int k = i < J.num_nodes ? i : 0;
+        const float2 p = pos[k], sz = J.size[k];
+        nd[q] = make_float4(p.x, p.y, sz.x, sz.y);
+        nr[q] = J.ratio[k];
+    }
 #pragma unroll
     for (int q = 0; q < BIN_PER_LANE; q++) {
         rk[q][0] = rk[q][1] = 0;
-        nd[q] = make_float4(0.f, 0.f, 0.f, 0.f);
         const int i = base + q * EB + threadIdx.x;
         if (i >= J.num_nodes) continue;
-        const float2 p = pos[i], sz = J.size[i];
-        nd[q] = make_float4(p.x, p.y, sz.x, sz.y);
-        const Span sx = make_span(p.x, sz.x, J.inv_bx, J.nbx), sy = make_span(p.y, sz.y, J.inv_by, J.nby);
+        const float4 g = nd[q];
+        const Span sx = make_span(g.x, g.z, J.inv_bx, J.nbx), sy = make_span(g.y, g.w, J.inv_by, J.nby);
         if (sx.b0 > sx.b1 || sy.b0 > sy.b1) continue;
         int t0, wx, wy;
         if (!small_node(sx, sy, J.T, J.tiles_y, t0, wx, wy)) {
@@ -495,7 +512,7 @@ __device__ __forceinline__ void bin_fill_body(const Pick& pos_p, const BinJob& f
         if (!rk[q][0]) continue;                                // no entries: out of range, no bins or a big node
         const int i = base + q * EB + threadIdx.x;
         const float4 g = nd[q];
-        const float r = J.ratio[i];
+        const float r = nr[q];
         const Span sx = make_span(g.x, g.z, J.inv_bx, J.nbx), sy = make_span(g.y, g.w, J.inv_by, J.nby);
         int t0, wx, wy;
         small_node(sx, sy, J.T, J.tiles_y, t0, wx, wy);
@@ -560,11 +577,12 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
     const int X0 = (t / J.tiles_y) * T, Y0 = (t % J.tiles_y) * T;
     const int X1 = min(X0 + T, J.nbx) - 1, Y1 = min(Y0 + T, J.nby) - 1;
     unsigned long long* acc = acc_all + sub * J.planes * TT;
-    // issued before anything waits on memory: the bucket bounds of both planes (fill form: with their counts and the
-    // tile's spill flag) and this lane's share of the fixed map (the first TILE_PF bins of its stride), so that a tile
-    // pays two dependent memory latencies, not four
-    int lo0 = 0, n0 = 0, lo1 = 0, n1 = 0, c0 = 0, c1 = 0, spilled = 0;
+    // issued before anything waits on memory: the grid's big-node count, the bucket bounds of both planes (fill form:
+    // with their counts and the tile's spill flag) and this lane's share of the fixed map (the first TILE_PF bins of
+    // its stride), so that a tile pays two dependent memory latencies, not four
+    int lo0 = 0, n0 = 0, lo1 = 0, n1 = 0, c0 = 0, c1 = 0, spilled = 0, nbig = 0;
     if (live) {
+        nbig = J.fill ? *J.big_count : *J.big_n;
         lo0 = J.start[t];
         n0 = J.start[t + 1];
         if (J.planes == 2) {
@@ -619,7 +637,6 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
                             make_span(g[k].y, g[k].w, J.inv_by, J.nby), r[k]);
     }
     // big nodes: one at a time, their bins in this tile spread over the tile's lanes
-    const int nbig = live ? (J.fill ? *J.big_count : *J.big_n) : 0;
     if (nbig) {
         const float2* pos = pos_p.get(true);
         for (int k = 0; k < nbig; k++) {
@@ -909,28 +926,41 @@ __device__ __forceinline__ void slice(int n, int wg, int& lo, int& hi) {
     hi = min(n, lo + per);
 }
 
+// (a slice is mostly at most EB long: each lane's first element of every segment is loaded before any is added --
+// one memory wait, not one per segment -- and any further ones follow in the loops; the per-lane order is the same)
 __device__ __forceinline__ void reduce_body(const Segs& s, double* out, int wg) {
     __shared__ double red[16 * RED_VALS];
-    double v[RED_VALS] = {0, 0, 0, 0, 0, 0, 0};
-    int lo, hi;
-    slice(s.n_moved, wg, lo, hi);
-    for (int k = lo + threadIdx.x; k < hi; k += EB) v[0] += s.moved[k];
-    slice(s.n_turned, wg, lo, hi);
-    for (int k = lo + threadIdx.x; k < hi; k += EB) v[1] += s.turned[k];
-    slice(s.n_wa, wg, lo, hi);
-    for (int k = lo + threadIdx.x; k < hi; k += EB) {
+    const int t = threadIdx.x;
+    int lo[5], hi[5];
+    slice(s.n_moved, wg, lo[0], hi[0]);
+    slice(s.n_turned, wg, lo[1], hi[1]);
+    slice(s.n_wa, wg, lo[2], hi[2]);
+    slice(s.n_over, wg, lo[3], hi[3]);
+    slice(s.n_norm, wg, lo[4], hi[4]);
+    bool in[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) in[q] = lo[q] + t < hi[q];
+    double v[RED_VALS];
+    v[0] = in[0] ? s.moved[lo[0] + t] : 0.0;
+    v[1] = in[1] ? s.turned[lo[1] + t] : 0.0;
+    v[2] = in[2] ? s.wa[2 * (lo[2] + t)] : 0.0;
+    v[3] = in[2] ? s.wa[2 * (lo[2] + t) + 1] : 0.0;
+    v[4] = in[3] ? s.over[lo[3] + t] : 0.0;
+    v[5] = in[4] ? s.wl2[lo[4] + t] : 0.0;
+    v[6] = in[4] ? s.dn2[lo[4] + t] : 0.0;
+    for (int k = lo[0] + t + EB; k < hi[0]; k += EB) v[0] += s.moved[k];
+    for (int k = lo[1] + t + EB; k < hi[1]; k += EB) v[1] += s.turned[k];
+    for (int k = lo[2] + t + EB; k < hi[2]; k += EB) {
         v[2] += s.wa[2 * k];
         v[3] += s.wa[2 * k + 1];
     }
-    slice(s.n_over, wg, lo, hi);
-    for (int k = lo + threadIdx.x; k < hi; k += EB) v[4] += s.over[k];
-    slice(s.n_norm, wg, lo, hi);
-    for (int k = lo + threadIdx.x; k < hi; k += EB) {
+    for (int k = lo[3] + t + EB; k < hi[3]; k += EB) v[4] += s.over[k];
+    for (int k = lo[4] + t + EB; k < hi[4]; k += EB) {
         v[5] += s.wl2[k];
         v[6] += s.dn2[k];
     }
     block_sum<RED_VALS>(v, red);
-    if (threadIdx.x == 0)
+    if (t == 0)
         for (int k = 0; k < RED_VALS; k++) out[wg * RED_VALS + k] = v[k];
 }
 __global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) { reduce_body(s, out, blockIdx.x); }
@@ -959,10 +989,13 @@ __device__ __forceinline__ void control_body(const double* red, double over_scal
     float* kp = kp_l;
     const double* c = c_l;
     __syncthreads();
+    if (threadIdx.x < RED_VALS) {          // the second stage: sum k by lane k, in block order
+        double a = 0.0;
+        for (int b = 0; b < RED_BLOCKS; b++) a += rb[b * RED_VALS + threadIdx.x];
+        r[threadIdx.x] = a;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 0; k < RED_VALS; k++) r[k] = 0;
-        for (int b = 0; b < RED_BLOCKS; b++)
-            for (int k = 0; k < RED_VALS; k++) r[k] += rb[b * RED_VALS + k];
         learn = 0;
         if (S && ctl.params && S[STOP] == 0) {
             const double moved = sqrt(r[0]), turned = sqrt(r[1]), wa = r[2], hpwl = r[3], ovf = r[4] * over_scale;
