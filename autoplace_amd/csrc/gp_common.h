@@ -10,6 +10,18 @@ namespace ap {
 // fixed point of the density maps: 2^32 per bin area (integer sums do not depend on their order)
 constexpr double kDensityScale = 4294967296.0;
 
+// floor(w 2^32) of a float w, as (unsigned long long)((double)w * 2^32) gives it, in single-precision operations
+// without branches: f = w 2^32 is exact (a power-of-two scale), h = trunc(f 2^-32) its high word, and f - h 2^32 is
+// exact (both are multiples of f's unit in the last place, the difference below 2^32), so its truncation is the low
+// word. For every w in [0, 2^31); w <= 0, w >= 2^31 and NaN give 0.
+__host__ __device__ __forceinline__ unsigned long long fixed_point32(float w) {
+    const bool ok = w > 0.f && w < 2147483648.f;
+    const float f = (ok ? w : 0.f) * 4294967296.f;
+    const float h = truncf(f * 2.3283064365386963e-10f);
+    const unsigned hi = (unsigned)h, lo = (unsigned)(f - h * 4294967296.f);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // control state layout (S, float64) and constants (c, float64)
 enum { LAM, GAM, ALPHA, A, ITER, HPWL_PREV, MOTION, WW, MU, SETTLED, BEST_HPWL, BEST_OVF, IS_BEST, IMPROVED, STOP,
        ACCEPT, LOOP_IT, EVALS, RES_HPWL, RES_OVF, HAS_BEST, NUM_S };

@@ -164,7 +164,7 @@ struct BinJob {
 };
 
 __device__ __forceinline__ unsigned long long contribution(float wx, const Span& sy, int by) {
-    return (unsigned long long)((double)(wx * span_overlap(sy, by)) * kDensityScale);
+    return fixed_point32(wx * span_overlap(sy, by));
 }
 
 // first tile of a node and whether its bins stay within 2 x 2 tiles
@@ -427,7 +427,7 @@ __device__ __forceinline__ void add_clipped(unsigned long long* a, int T, int X0
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const unsigned long long c = (unsigned long long)((double)(wx[i] * wy[j]) * kDensityScale);
+                const unsigned long long c = fixed_point32(wx[i] * wy[j]);
                 if (c) atomicAdd(p + i * T + j, c);
             }
         return;
