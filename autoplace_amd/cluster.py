@@ -10,6 +10,7 @@ and netlist CSR, the free-area fraction of its grid and its density layer -- is 
 GP clock as the reference builds it, without a round trip through the host.
 """
 import ctypes
+import threading
 from types import SimpleNamespace
 
 import numpy as np
@@ -130,13 +131,33 @@ def coarse_problem(data, node_size, wirelength, density, fixed_fraction, args, p
                            num_nodes=cnode.shape[0])
 
 
-def expand(cpos, coarse, node_size, seed):
+class ExpansionDraws(object):
+    """The expansion's uniform draws, torch.rand(n, 2) of a generator seeded with `seed` (as the reference draws them,
+    cluster.py:137-144), made on a host thread into pinned memory while the device places the clusters: they depend on
+    nothing else. get() waits for them."""
+
+    def __init__(self, n, seed):
+        self._out = torch.empty((n, 2), dtype=torch.float32, pin_memory=torch.cuda.is_available())
+        self._seed = seed
+        self._thread = threading.Thread(target=self._draw, daemon=True)
+        self._thread.start()
+
+    def _draw(self):
+        torch.rand(self._out.shape, generator=torch.Generator().manual_seed(self._seed), out=self._out)
+
+    def get(self):
+        self._thread.join()
+        return self._out
+
+
+def expand(cpos, coarse, node_size, seed, draws=None):
     """Clusters + fillers -> cells + fillers (cluster.py:137-144): every cell at a uniform point (torch's CPU
-    generator, as the reference) of the room its cluster's square leaves around it; fillers unchanged."""
+    generator, as the reference) of the room its cluster's square leaves around it; fillers unchanged.
+    draws: an ExpansionDraws of the same n and seed made beforehand (else drawn here)."""
     n, device = coarse.cid.numel(), cpos.device
     cell_size = coarse.cell_size if getattr(coarse, "cell_size", None) is not None else \
         torch.tensor(node_size[:n], dtype=torch.float32, device=device)
-    gen = torch.Generator().manual_seed(seed)
+    u = draws.get() if draws is not None else torch.rand(n, 2, generator=torch.Generator().manual_seed(seed))
     room = (coarse.size[coarse.cid] - cell_size).clamp_(min=0)
-    cells = cpos[:coarse.num][coarse.cid] + (torch.rand(n, 2, generator=gen) - 0.5).to(device) * room
+    cells = cpos[:coarse.num][coarse.cid] + (u.to(device, non_blocking=True) - 0.5) * room
     return torch.cat([cells, cpos[coarse.num:]])

@@ -10,7 +10,7 @@ import torch
 
 from . import gpu_loop
 from .calculator import Objective
-from .cluster import coarse_problem, expand
+from .cluster import ExpansionDraws, coarse_problem, expand
 from .density import ElectronicDensityLayer
 from .initializer import choose_num_bin
 from .nesterov_optimizer import NesterovOptimizer, estimate_initial_learning_rate
@@ -97,9 +97,10 @@ def global_placement_main(prob, wirelength, density, pos, args, log=None, opts=N
         cps.handover_gamma = ps.wa_coeff
         cobjective = Objective(coarse.wirelength, coarse.density, cps)
         cpos = torch.cat([pos[:coarse.num], pos[n:]])
+        draws = ExpansionDraws(n, args.seed)             # drawn on the host beside the clusters' placement
         level = _place_level(cpos, cobjective, cps, cargs, log, 0, opts, phase=1.0)
         gpu_loop.mark(opts.get("timeline"), "coarse level", device)
-        pos.copy_(expand(cpos, coarse, node_size, args.seed)).clamp_(min=density.pos_lo, max=density.pos_hi)
+        pos.copy_(expand(cpos, coarse, node_size, args.seed, draws)).clamp_(min=density.pos_lo, max=density.pos_hi)
         weight = cps.density_weight * objective.tension_unit(pos) / cobjective.tension_unit(cpos)
         overflow = density.forward(pos)[1].item() if args.wa_rule == "overflow" else None
         ps.resume(cps, weight, wirelength.forward(pos, ps.wa_coeff)[2].item(), overflow)

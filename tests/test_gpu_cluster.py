@@ -51,6 +51,10 @@ def test_coarse_problem_matches_the_reference(small, golden_small):
     pr = torch.tensor(g["pos_rand"], device=DEV)
     cpos = torch.cat([pr[:coarse.num], pr[n:]])
     assert np.array_equal(expand(cpos, coarse, prob.node_size, 0).cpu().numpy(), g["expand_pr"])
+    # the same with the draws made beforehand on a host thread, as the flow makes them
+    from autoplace_amd.cluster import ExpansionDraws
+    draws = ExpansionDraws(n, 0)
+    assert np.array_equal(expand(cpos, coarse, prob.node_size, 0, draws).cpu().numpy(), g["expand_pr"])
 
 
 def test_device_density_layer_equals_the_host_built_one(small):
