@@ -42,6 +42,9 @@ _SIGNATURES = {
     "ap_level_destroy": ([P], I),
     "ap_level_run": ([P, I, I, P], I),
     "ap_level_prepare": ([P, I], I),
+    "ap_level_start_begin": ([P, D, I, I], I),
+    "ap_level_start_run": ([P, I, I, P], I),
+    "ap_level_start_state": ([P, ctypes.POINTER(D), ctypes.POINTER(D), I], I),
     "ap_level_last_ms": ([P, ctypes.POINTER(F)], I),
     "ap_level_profile": ([P, I, ctypes.POINTER(D), P], I),
     "ap_level_time_class": ([P, I], I),

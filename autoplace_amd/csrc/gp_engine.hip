@@ -1072,6 +1072,47 @@ __global__ __launch_bounds__(64) void eng_control(const double* red, double over
     control_body(red, over_scale, S, kp, c, flags, stats, ctl);
 }
 
+// The wirelength-only start's control (src/nesterov_optimizer.py:34-53 and param_scheduler.py:38-48, as
+// run_placement.py:41-76 drives them before the density weight is on): every step is taken unless a norm or the HPWL
+// is not finite (diverged: the point stays); the BB step alpha = moved / turned (kept when turned is 0); the
+// momentum a' = (1 + sqrt(4 a^2 + 1)) / 2 and the next pass's alpha and coef (a' - 1) / a'' in kp; the pass's (WA,
+// HPWL) appended to hist; the phase ends when WA fell by at most tol x window x WA over the last `window` passes
+// (done 1), or at the pass limit (done 3). A done state freezes the point for the rest of the run.
+enum { SS_DONE, SS_COUNT, SS_TOL, SS_WINDOW, SS_MAX, START_STATE = 8 };
+__global__ __launch_bounds__(64) void eng_start_control(const double* red, double* S, float* kp, int* flags,
+                                                        double* st, double* hist) {
+    __shared__ double r[RED_VALS];
+    __shared__ double rb[RED_BLOCKS * RED_VALS];
+    for (int i = threadIdx.x; i < RED_BLOCKS * RED_VALS; i += blockDim.x) rb[i] = red[i];
+    __syncthreads();
+    if (threadIdx.x < RED_VALS) {          // the second stage: sum k by lane k, in block order
+        double a = 0.0;
+        for (int b = 0; b < RED_BLOCKS; b++) a += rb[b * RED_VALS + threadIdx.x];
+        r[threadIdx.x] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0 || st[SS_DONE] != 0) return;
+    const double moved = sqrt(r[0]), turned = sqrt(r[1]), wa = r[2], hpwl = r[3];
+    if (!isfinite(moved) || !isfinite(turned) || !isfinite(hpwl)) {
+        st[SS_DONE] = 2;
+        return;
+    }
+    flags[0] = 1 - flags[0];                                  // the step is taken: the pass's point is current
+    const double alpha = turned > 0 ? moved / turned : S[ALPHA];
+    const double a = S[A], a1 = (1.0 + sqrt(4.0 * a * a + 1.0)) / 2.0, a2 = (1.0 + sqrt(4.0 * a1 * a1 + 1.0)) / 2.0;
+    S[ALPHA] = alpha;
+    S[A] = a1;
+    kp[3] = (float)alpha;
+    kp[4] = (float)((a1 - 1.0) / a2);
+    const int n = (int)st[SS_COUNT], window = (int)st[SS_WINDOW];
+    hist[2 * n] = wa;
+    hist[2 * n + 1] = hpwl;
+    st[SS_COUNT] = n + 1;
+    S[EVALS] += 1;
+    if (n + 1 > window && !(hist[2 * (n - window)] - wa > st[SS_TOL] * window * wa)) st[SS_DONE] = 1;
+    else if (n + 1 >= (int)st[SS_MAX]) st[SS_DONE] = 3;
+}
+
 // level set-up: pin -> its position in the cell CSR, so the WA kernel writes pin gradients in cell order
 __global__ __launch_bounds__(EB) void eng_pin_slot_clear(int* slot, int num_pins) {
     const int p = blockIdx.x * EB + threadIdx.x;
@@ -1152,6 +1193,7 @@ static int tile_side(long long n, const unsigned long long* over, int nbx, int n
 struct GraphEntry {            // a captured run of `passes` passes
     int passes = 0;
     int timing = -1;           // kernel class bracketed by events (-1: none)
+    bool start = false;        // passes of the wirelength-only start
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> ev;   // timing: 2 per timed pass (before the class, after it)
@@ -1164,6 +1206,7 @@ struct GraphEntry {            // a captured run of `passes` passes
         graph = nullptr;
         passes = 0;
         timing = -1;
+        start = false;
     }
 };
 constexpr int GRAPH_CACHE = 4;
@@ -1203,6 +1246,12 @@ struct Level {
     bool capturing = false;
     GraphEntry graphs[GRAPH_CACHE];
     int graph_next = 0;
+    // the wirelength-only start (ap_level_start_*): passes without the density stages whose control applies the
+    // start rule; its state st f64[START_STATE] and the (WA, HPWL) of every start pass, hist f64[2 start_cap]
+    bool start_mode = false;
+    double* start_st = nullptr;
+    double* start_hist = nullptr;
+    int start_cap = 0;
     // event timing of one kernel class: events before and after it in every pass of a run (graph or direct)
     int timing_cls = -1;
     int time_every = 1;             // events in passes 0, every, 2 every, ... of a run only
@@ -1691,7 +1740,7 @@ void Level::destroy() {
     for (hipEvent_t& ev : pev)
         if (ev) (void)hipEventDestroy(ev);
     void* ptrs[] = {u1, v1, g1, pin_grad, cell_wl, rho, fx, fy, f2, part, stats, flags, red, ctl_x, block_net,
-                    pin_slot};
+                    pin_slot, start_st, start_hist};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (void* p : bin_mem) (void)hipFree(p);
@@ -1837,8 +1886,11 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     AP_LAUNCHED();
     Ctl ctl_pass = ctl;
     if (!S) ctl_pass.params = nullptr;                       // a plain objective evaluation: no controller
-    hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red, use_density ? over_scale() : 0.0, S, kp_out, d.c,
-                       flags, stats, ctl_pass);
+    if (start_mode && S)
+        hipLaunchKernelGGL(eng_start_control, dim3(1), dim3(64), 0, s, red, S, kp_out, flags, start_st, start_hist);
+    else
+        hipLaunchKernelGGL(eng_control, dim3(1), dim3(64), 0, s, red, use_density ? over_scale() : 0.0, S, kp_out,
+                           d.c, flags, stats, ctl_pass);
     AP_LAUNCHED();
     tend(AP_PROF_CONTROL, s);
     mark(AP_PROF_CLASSES, s);
@@ -1848,11 +1900,13 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
 int Level::pass(hipStream_t s) {
     mark(AP_PROF_NESTEROV, s);
     tbegin(AP_PROF_NESTEROV, s);
-    const int rooms = jj.count ? 2 : 1;                      // the rooms of the pass's fill, beside the step
+    // the rooms of the pass's fill, beside the step (none for a start pass: no binning)
+    const int rooms = start_mode ? 0 : jj.count ? 2 : 1;
     hipLaunchKernelGGL(eng_nesterov, dim3(rooms + nblk_n), dim3(EB), 0, s, nest_args(), fj, jj, rooms);
     AP_LAUNCHED();
     tend(AP_PROF_NESTEROV, s);
-    return objective(pass_pos(), d.kp, true, (const float2*)d.g, g1, (float2*)d.g, g1, s, d.S, d.kp, 1, true);
+    return objective(pass_pos(), d.kp, true, (const float2*)d.g, g1, (float2*)d.g, g1, s, d.S, d.kp,
+                     start_mode ? 0 : 1, true);
 }
 
 int Level::finalize(hipStream_t s) {
@@ -1866,7 +1920,8 @@ int Level::finalize(hipStream_t s) {
 // the captured graph of `passes` passes with the current timing class, captured now when not cached
 int Level::graph_for(int passes, int& slot) {
     for (int k = 0; k < GRAPH_CACHE; k++)
-        if (graphs[k].exec && graphs[k].passes == passes && graphs[k].timing == timing_cls) {
+        if (graphs[k].exec && graphs[k].passes == passes && graphs[k].timing == timing_cls &&
+            graphs[k].start == start_mode) {
             slot = k;
             return AP_OK;
         }
@@ -1912,6 +1967,7 @@ int Level::graph_for(int passes, int& slot) {
     AP_HIP(hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0));
     G.passes = passes;
     G.timing = timing_cls;
+    G.start = start_mode;
     return AP_OK;
 }
 
@@ -2398,15 +2454,63 @@ int ap_level_prepare(ap_level* level, int passes) {
     return level->L.graph_for(passes, slot);
 }
 
+static int run_passes(Level& L, int passes, int use_graph, void* stream);
+
 int ap_level_run(ap_level* level, int passes, int use_graph, void* stream) {
     AP_REQUIRE(level && passes >= 0, "level_run: bad arguments");
+    level->L.start_mode = false;
+    return run_passes(level->L, passes, use_graph, stream);
+}
+
+int ap_level_start_begin(ap_level* level, double tol, int window, int max_passes) {
+    AP_REQUIRE(level && tol >= 0 && window >= 1 && max_passes >= 1,
+               "level_start_begin: tol %g, window %d, max_passes %d", tol, window, max_passes);
     Level& L = level->L;
+    AP_HIP(hipStreamSynchronize(L.stream));
+    if (!L.start_st) AP_HIP(hipMalloc((void**)&L.start_st, START_STATE * sizeof(double)));
+    if (L.start_cap < max_passes) {
+        if (L.start_hist) AP_HIP(hipFree(L.start_hist));
+        L.start_hist = nullptr;
+        L.start_cap = 0;
+        AP_HIP(hipMalloc((void**)&L.start_hist, 2 * (size_t)max_passes * sizeof(double)));
+        L.start_cap = max_passes;
+    }
+    const double st[START_STATE] = {0.0, 0.0, tol, (double)window, (double)max_passes, 0.0, 0.0, 0.0};
+    AP_HIP(hipMemcpy(L.start_st, st, sizeof(st), hipMemcpyHostToDevice));
+    return AP_OK;
+}
+
+int ap_level_start_run(ap_level* level, int passes, int use_graph, void* stream) {
+    AP_REQUIRE(level && passes >= 0, "level_start_run: bad arguments");
+    Level& L = level->L;
+    AP_REQUIRE(L.start_st, "level_start_run: no start state (ap_level_start_begin)");
+    L.start_mode = true;
+    const int st = run_passes(L, passes, use_graph, stream);
+    L.start_mode = false;
+    L.rooms_stale = true;           // the points moved with no binning: the next density pass counts afresh
+    return st;
+}
+
+int ap_level_start_state(ap_level* level, double* state, double* history, int n) {
+    AP_REQUIRE(level && state && n >= 0 && (n == 0 || history), "level_start_state: bad arguments");
+    Level& L = level->L;
+    AP_REQUIRE(L.start_st, "level_start_state: no start state (ap_level_start_begin)");
+    AP_HIP(hipStreamSynchronize(L.stream));
+    AP_HIP(hipMemcpy(state, L.start_st, START_STATE * sizeof(double), hipMemcpyDeviceToHost));
+    const int count = (int)state[SS_COUNT];
+    AP_REQUIRE(n <= L.start_cap, "level_start_state: %d passes asked, %d kept", n, L.start_cap);
+    if (n) AP_HIP(hipMemcpy(history, L.start_hist, 2 * (size_t)(n < count ? n : count) * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    return AP_OK;
+}
+
+static int run_passes(Level& L, int passes, int use_graph, void* stream) {
     hipStream_t caller = (hipStream_t)stream;
     int st = AP_OK, slot = -1;
     if (use_graph && passes > 0 && (st = L.graph_for(passes, slot))) return st;
     AP_HIP(hipEventRecord(L.ev_in, caller));
     AP_HIP(hipStreamWaitEvent(L.stream, L.ev_in, 0));
-    if ((st = L.init_rooms(L.cur_pos(), L.stream))) return st;
+    if (!L.start_mode && (st = L.init_rooms(L.cur_pos(), L.stream))) return st;
     AP_HIP(hipEventRecord(L.t0, L.stream));
     if (slot >= 0) {
         GraphEntry& G = L.graphs[slot];

@@ -67,6 +67,7 @@ def setup_level(obj, wirelength, density, u, v, g, best, S, kp, c, poisson="auto
     pm = (lambda t: t[perm].contiguous()) if reorder else (lambda t: t)           # noqa: E731
     pc = (lambda t: t[perm[:n]].contiguous()) if reorder else (lambda t: t)       # noqa: E731
     obj.perm = perm
+    obj.S, obj.kp, obj.c = S, kp, c
     obj.u, obj.v, obj.g, obj.best = pm(u), pm(v), pm(g), pm(best)
     arrays = dict(size=pm(d.smooth["size"]), ratio=pm(d.smooth["ratio"]), pins=pm(d.pins), area=pm(d.area),
                   lo=pm(d.pos_lo), hi=pm(d.pos_hi))
@@ -151,6 +152,37 @@ class Engine(object):
         for mine, theirs in zip((self.u, self.v, self.g, self.best), self.user):
             theirs[self.perm] = mine
 
+    def start_begin(self, tol, window, max_passes):
+        """Reset the wirelength-only start (ap_level_start_begin): the rule's tolerance and window, the pass limit."""
+        _lib.check(_lib.lib().ap_level_start_begin(self._h, float(tol), int(window), int(max_passes)),
+                   "level_start_begin")
+
+    def start_run(self, passes, use_graph=True):
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _lib.check(_lib.lib().ap_level_start_run(self._h, passes, int(use_graph), stream), "level_start_run")
+
+    def start_state(self, history=False):
+        """-> (done: 0 running, 1 the start rule, 2 diverged, 3 the pass limit; passes taken; [(WA, HPWL)] of every
+        pass taken when history)."""
+        st = (ctypes.c_double * 8)()
+        _lib.check(_lib.lib().ap_level_start_state(self._h, st, None, 0), "level_start_state")
+        done, count = int(st[0]), int(st[1])
+        if not history or count == 0:
+            return done, count, []
+        h = (ctypes.c_double * (2 * count))()
+        _lib.check(_lib.lib().ap_level_start_state(self._h, st, h, count), "level_start_state")
+        return done, count, [(h[2 * k], h[2 * k + 1]) for k in range(count)]
+
+    def rebind(self, u, v, g, best, S, kp, c):
+        """Continue with the level's current point as the start of the next phase: the caller's tensors that
+        finish() writes, the control state's new values (copied into the level's S / kp / c), best = the current
+        point."""
+        self.user = (u, v, g, best)
+        self.S.copy_(S)
+        self.kp.copy_(kp)
+        self.c.copy_(c)
+        self.best.copy_(self.v)
+
     def prepare(self, passes):
         """Capture the graph of `passes` passes now (outside any timed region)."""
         _lib.check(_lib.lib().ap_level_prepare(self._h, passes), "level_prepare")
@@ -227,10 +259,12 @@ class Engine(object):
 
 
 def control_state(ps, optimizer, iteration, args):
-    """(S f64[21], kp f32[5], c f64[13]) from the host state (gpu_loop.py:39-53)."""
+    """(S f64[21], kp f32[5], c f64[13]) from the host state (gpu_loop.py:39-53). Before a level's first step (the
+    start phase's state) there is no previous HPWL: hpwl_prev is then +inf."""
     S = [0.0] * len(FIELDS)
     S[K["lam"]], S[K["gam"]], S[K["alpha"]], S[K["a"]] = ps.density_weight, ps.wa_coeff, optimizer.alpha, optimizer.a
-    S[K["iter"]], S[K["hpwl_prev"]], S[K["motion"]], S[K["ww"]] = ps.iter, ps.hpwl[-1], ps.motion, ps.weighted_weight
+    S[K["iter"]], S[K["motion"]], S[K["ww"]] = ps.iter, ps.motion, ps.weighted_weight
+    S[K["hpwl_prev"]] = ps.hpwl[-1] if ps.hpwl else float("inf")
     S[K["mu"]], S[K["settled"]], S[K["improved"]] = ps.mu, ps.settled, ps.improved_iter
     S[K["best_hpwl"]], S[K["best_ovf"]] = ps.best_hpwl, ps.best_overflow
     S[K["loop_it"]] = iteration
@@ -244,7 +278,7 @@ def control_state(ps, optimizer, iteration, args):
 
 def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every=16, use_graph=True,
                      poisson="auto", reorder=True, hook=None, controller=None, phase=0.0, autotune=None,
-                     timeline=None):
+                     timeline=None, engine=None):
     """Continue the level from `iteration` (density weight on) until it stops; updates optimizer, ps and
     objective.evals as the host loop would. -> (reason, iterations, hpwl, overflow) of the kept placement.
     hook(objective, u, v, g, S, kp, c), if given, sees the state the device loop starts from (instrumentation).
@@ -252,7 +286,9 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
     list receives this level's record (passes under the controller, fallback, the logged passes).
     autotune: time the density tile layouts at the start and keep the fastest (None: for the cell level when it has
     100k nodes or more -- the long level, where the timing pays for itself); the results do not depend on it.
-    timeline: None, or a list that receives (stage, synchronised wall time) marks (instrumentation)."""
+    timeline: None, or a list that receives (stage, synchronised wall time) marks (instrumentation).
+    engine: the level's Engine left by run_start_on_gpu (its point is the optimizer's), continued instead of a new
+    one."""
     device, pos = optimizer.v.device, optimizer.v
     log = as_log(log)
     S, kp, c = control_state(ps, optimizer, iteration, args)
@@ -264,8 +300,13 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
     if hook is not None:
         hook(objective, u, pos, g, S, kp, c)
     best = pos.clone()
-    eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
-    mark(timeline, "engine set-up", device)
+    if engine is not None:
+        eng = engine
+        eng.rebind(u, pos, g, best, S, kp, c)
+        S, kp, c = eng.S, eng.kp, eng.c          # the level reads and writes its own control tensors
+    else:
+        eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
+        mark(timeline, "engine set-up", device)
     ctl_log = None
     if controller is not None:
         ctl_log = eng.set_controller(controller, phase)
@@ -274,8 +315,12 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
         mark(timeline, "autotune", device)
     evals_before, logged = objective.evals, iteration // args.log_freq
     watch = torch.tensor([K["stop"], K["loop_it"], K["lam"], K["gam"], K["res_hpwl"], K["res_ovf"]], device=device)
+    runs_left = (args.inner_iter - iteration) // check_every + 2     # the control stops at inner_iter
     try:
         while True:
+            if runs_left == 0:
+                raise RuntimeError("device loop: no stop after %d iterations" % args.inner_iter)
+            runs_left -= 1
             eng.run(check_every, use_graph)
             stop, it, lam, gam, hpwl, ovf = S[watch].tolist()
             if log and int(it) // args.log_freq > logged:
@@ -306,3 +351,65 @@ def run_level_on_gpu(optimizer, objective, ps, args, log, iteration, check_every
     if get("has_best"):
         pos.copy_(best)
     return REASON[int(get("stop"))], int(get("loop_it")), get("res_hpwl"), get("res_ovf")
+
+
+START_WINDOW = 5      # the start rule's window (param_scheduler.py:38-43)
+
+
+def run_start_on_gpu(optimizer, objective, ps, args, log, iteration, check_every=16, use_graph=True,
+                     poisson="auto", reorder=True, timeline=None):
+    """The level's wirelength-only start (run_placement.py:41-76 while the density weight is 0) as engine passes
+    (ap_level_start_*): the same steps, the BB step from FP64 norms, the start rule of param_scheduler.py:38-48 on
+    the device, `check_every` passes per graph replay and one synchronisation per replay instead of one per step.
+    Updates optimizer (u, v, g, alpha, a), ps (iter, the hpwl / wa history) and objective.evals as the host loop
+    would, and logs every log_freq-th iteration as it does. -> (engine, done, passes): the level's Engine, holding the
+    current point, for run_level_on_gpu to continue; done 1 (the rule fired), 2 (diverged) or 3 (the iteration
+    limit)."""
+    device, pos = optimizer.v.device, optimizer.v
+    log = as_log(log)
+    if ps.wa or ps.hpwl:
+        raise ValueError("run_start_on_gpu starts a level's wirelength-only phase: the scheduler has history")
+    S, kp, c = control_state(ps, optimizer, iteration, args)
+    S = torch.tensor(S, dtype=torch.float64, device=device)
+    kp = torch.tensor(kp, dtype=torch.float32, device=device)
+    c = torch.tensor(c, dtype=torch.float64, device=device)
+    u = optimizer.u.contiguous()
+    g = optimizer.g.contiguous()
+    best = pos.clone()
+    eng = Engine(objective.wirelength, objective.density, u, pos, g, best, S, kp, c, poisson, reorder)
+    mark(timeline, "start engine set-up", device)
+    try:
+        eng.start_begin(args.start_tol, START_WINDOW, args.inner_iter - iteration)
+        runs_left = (args.inner_iter - iteration) // check_every + 2     # the pass limit ends it before
+        while True:
+            if runs_left == 0:
+                raise RuntimeError("start phase: no end after %d passes" % (args.inner_iter - iteration))
+            runs_left -= 1
+            eng.start_run(check_every, use_graph)
+            done, count, _ = eng.start_state()
+            if done:
+                break
+        done, count, hist = eng.start_state(history=True)
+        eng.finish()
+    except BaseException:
+        torch.cuda.synchronize(device)
+        eng.close()
+        raise
+    mark(timeline, "start passes", device)
+    St = S.tolist()
+    optimizer.u, optimizer.g = u, g
+    optimizer.alpha, optimizer.a = St[K["alpha"]], St[K["a"]]
+    objective.evals += count
+    for k, (wa, hpwl) in enumerate(hist):
+        ps.hpwl.append(hpwl)
+        ps.wa.append(wa)
+        ps.iter += 1
+        if log and (iteration + k) % args.log_freq == 0:
+            log("iter: %d | hpwl: %.4E overflow: %.4f density_weight: %.4E wa_coeff: %.4E"
+                % (iteration + k, hpwl, float("nan"), ps.density_weight, ps.wa_coeff))
+    if hist:
+        objective.wa = torch.tensor(hist[-1][0], dtype=torch.float64, device=device)
+        objective.hpwl = torch.tensor(hist[-1][1], dtype=torch.float64, device=device)
+    if done == 2:
+        optimizer.diverged = True
+    return eng, done, count

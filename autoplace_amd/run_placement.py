@@ -42,6 +42,28 @@ def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None, phase=0
                                                                                objective.wa))
     reason, iteration, start_iters = "max_iter", first_iter, 0
     hpwl = overflow = float("nan")
+    engine = None
+    if ps.density_weight == 0 and gpu_loop.usable(optimizer) and opts.get("device_start", True):
+        # the wirelength-only start as engine passes; the level's engine then goes on with the density phase
+        gpu_loop.mark(opts.get("timeline"), "start set-up", pos.device)
+        engine, done, count = gpu_loop.run_start_on_gpu(
+            optimizer, objective, ps, args, log, first_iter, use_graph=opts.get("graph", True),
+            poisson=opts.get("poisson", "auto"), reorder=opts.get("reorder", True), timeline=opts.get("timeline"))
+        first_iter += count
+        iteration = first_iter - 1
+        if count:
+            hpwl, wa = ps.hpwl[-1], ps.wa[-1]
+        if done == 2:
+            engine.close()
+            return SimpleNamespace(reason="diverged", iters=first_iter, start_iters=0, hpwl=hpwl, overflow=overflow)
+        if done == 1:
+            ps.density_weight, start_iters = args.density_weight * objective.tension_unit(pos), first_iter
+            if log:
+                log("wirelength-only start over after %d iterations (hpwl %.4E): density weight %.4E"
+                    % (start_iters, hpwl, ps.density_weight))
+        else:
+            engine.close()
+            engine = None
     for iteration in range(first_iter, args.inner_iter):
         if ps.density_weight > 0 and gpu_loop.usable(optimizer):
             gpu_loop.mark(opts.get("timeline"), "start phase", pos.device)
@@ -49,7 +71,7 @@ def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None, phase=0
                 optimizer, objective, ps, args, log, iteration, use_graph=opts.get("graph", True),
                 poisson=opts.get("poisson", "auto"), reorder=opts.get("reorder", True), hook=opts.get("hook"),
                 controller=opts.get("controller"), phase=phase, autotune=opts.get("autotune"),
-                timeline=opts.get("timeline"))
+                timeline=opts.get("timeline"), engine=engine)
             return SimpleNamespace(reason=reason, iters=iters, start_iters=start_iters, hpwl=hpwl, overflow=overflow)
         optimizer.step()
         if optimizer.diverged:

@@ -179,6 +179,19 @@ AP_API int ap_level_run(ap_level* level, int passes, int use_graph, void* stream
 /* Capture (and instantiate) the graph of `passes` passes now, so that the first ap_level_run with that count only
  * launches it. Optional. */
 AP_API int ap_level_prepare(ap_level* level, int passes);
+/* The wirelength-only start of a level (src/run_placement.py:41-76 while the density weight is 0, with the steps of
+ * nesterov_optimizer.py:34-53 and the rule of param_scheduler.py:38-48) as engine passes: each pass is the Nesterov
+ * step with kp's alpha and coef, WA, the cells' WA gradient divided by max(1, pins) (fillers 0), the FP64 norms, and
+ * a control that takes every step (unless a norm or the HPWL is not finite: diverged), sets the BB step and the
+ * momentum (S[ALPHA], S[A], kp[3], kp[4]) and records the pass's (WA, HPWL); the phase ends once WA fell by at most
+ * tol x window x WA over the last `window` passes, or after max_passes passes. A run's passes after the end leave
+ * the point unchanged. begin: resets the start state (synchronises the level's stream). start_run: `passes` such
+ * passes (use_graph as ap_level_run). start_state: state f64[8] = (done: 0 running, 1 the rule, 2 diverged, 3 the
+ * pass limit; passes taken; tol; window; max_passes) and history f64[n][2] = (WA, HPWL) of the first n passes taken
+ * (waits for the level's stream). */
+AP_API int ap_level_start_begin(ap_level* level, double tol, int window, int max_passes);
+AP_API int ap_level_start_run(ap_level* level, int passes, int use_graph, void* stream);
+AP_API int ap_level_start_state(ap_level* level, double* state, double* history, int n);
 /* Kernel classes of a pass, in launch order, for ap_level_profile and ap_level_time_class. */
 #define AP_PROF_NESTEROV 0
 #define AP_PROF_WA 1

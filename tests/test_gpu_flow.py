@@ -116,3 +116,58 @@ def test_run_from_files_writes_gp_def_and_evaluates_it(design_a, runs):
     h64 = owa.hpwl_exact(pos.astype(np.float32), data.pin_cell, data.pin_off, data.pin_net, data.num_nets,
                          data.num_mov)
     assert abs(e["hpwl_sites"] - h64) / h64 < 1e-6
+
+
+@pytest.mark.parametrize("seed", [0, 3])
+def test_device_start_phase_equals_the_host_loop(design_a, seed):
+    """The wirelength-only start as engine passes (run_start_on_gpu) against the host loop it replaces
+    (run_placement.py:41-76 with density weight 0) from the same state: the start rule fires at the same iteration,
+    the HPWL history agrees to 1e-5, the WA history to 1e-6 of the HPWL, the BB step to 1e-3, the points to 1e-5 of
+    the region and the last gradient to 2e-3."""
+    from types import SimpleNamespace
+
+    from autoplace_amd import gpu_loop
+    from autoplace_amd.calculator import Objective
+    from autoplace_amd.nesterov_optimizer import NesterovOptimizer, estimate_initial_learning_rate
+    from autoplace_amd.problem import build_problem, default_args
+    from autoplace_amd.run_placement import _scheduler, build_operators
+    data = design_a
+    args = default_args(seed=seed)
+    prob = build_problem(data, args)
+    wl, den = build_operators(prob, args, "cuda:0")
+    pos0 = torch.tensor(prob.pos0, device="cuda:0")
+
+    def fresh():
+        ps = _scheduler(den, data.num_mov, data.mov_area, prob.total_pins, prob.total_area, len(prob.node_size), args)
+        obj = Objective(wl, den, ps)
+        pos = pos0.clone()
+        bounds = (den.pos_lo, den.pos_hi)
+        lr = estimate_initial_learning_rate(pos, obj, args.lr, bounds)
+        opt = NesterovOptimizer(pos, obj, lr, bounds, stats=lambda: (obj.hpwl, obj.overflow, obj.wa))
+        return SimpleNamespace(ps=ps, obj=obj, pos=pos, opt=opt)
+
+    host = fresh()
+    n_host = 0
+    for it in range(args.inner_iter):
+        host.opt.step()
+        hpwl, _, wa = host.opt.fetched
+        n_host += 1
+        if host.ps.start_step(hpwl, wa):
+            break
+    dev = fresh()
+    eng, done, n_dev = gpu_loop.run_start_on_gpu(dev.opt, dev.obj, dev.ps, args, None, 0)
+    eng.close()
+    print("start phase: host %d iterations, device %d (done %d); alpha %.6e / %.6e; last HPWL %.6e / %.6e"
+          % (n_host, n_dev, done, host.opt.alpha, dev.opt.alpha, host.ps.hpwl[-1], dev.ps.hpwl[-1]))
+    assert done == 1 and n_dev == n_host
+    assert dev.ps.iter == host.ps.iter and dev.obj.evals == host.obj.evals
+    # the WA total is a sum of small per-net differences (the smoothed spans minus their means): its FP32 rounding
+    # is compared on the HPWL's scale
+    assert np.allclose(dev.ps.hpwl, host.ps.hpwl, rtol=1e-5)
+    assert np.allclose(dev.ps.wa, host.ps.wa, rtol=0, atol=1e-6 * host.ps.hpwl[0])
+    assert abs(dev.opt.alpha / host.opt.alpha - 1) < 1e-3 and abs(dev.opt.a / host.opt.a - 1) < 1e-12
+    extent = max(data.width, data.height)
+    for a, b in ((dev.opt.v, host.opt.v), (dev.opt.u, host.opt.u)):
+        assert float((a - b).abs().max()) < 1e-5 * extent
+    # the gradient at the end of the start is small (the wirelength nearly settled): the points' drift shows in it
+    assert float((dev.opt.g - host.opt.g).norm() / host.opt.g.norm()) < 2e-3
