@@ -24,6 +24,9 @@ _SIGNATURES = {
     "ap_last_error": ([], ctypes.c_char_p),
     "ap_version": ([], I),
     "ap_wa_wirelength": ([P, P, P, P, P, P, I, I, P, P, P, P, I, P], I),
+    "ap_wa_plan_create": ([P, P, P, I, I, ctypes.POINTER(P)], I),
+    "ap_wa_plan_destroy": ([P], I),
+    "ap_wa_wirelength_planned": ([P, P, P, P, P, P, P, I, I, P, P, P, P, I, P], I),
     "ap_density_map": ([P, P, P, I, I, F, F, I, I, P, P, F, P, P, P], I),
     "ap_density_map_work_bytes": ([I, I, I, I], ctypes.c_longlong),
     "ap_density_map_ws": ([P, P, P, I, I, F, F, I, I, P, F, P, P, P, ctypes.c_longlong, P], I),

@@ -28,5 +28,8 @@ struct WaArgs {
 int launch_wa_block_table(const int* net_start, int num_nets, int blocks, int* block_net, hipStream_t s);
 
 int launch_wa(const WaArgs& a, int num_pins, hipStream_t s);
+// grad[c] = the sum of the gradients of cell c's pins (pin_grad in pin order, the cell's pins in cell_pin)
+int launch_wa_cell_sum(const float2* pin_grad, const int* cell_start, const int* cell_pin, int num_mov, float2* grad,
+                       hipStream_t s);
 
 }  // namespace ap

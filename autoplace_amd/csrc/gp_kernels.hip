@@ -182,6 +182,15 @@ int launch_wa_block_table(const int* net_start, int num_nets, int blocks, int* b
     return AP_OK;
 }
 
+int launch_wa_cell_sum(const float2* pin_grad, const int* cell_start, const int* cell_pin, int num_mov, float2* grad,
+                       hipStream_t s) {
+    if (num_mov <= 0) return AP_OK;
+    hipLaunchKernelGGL(wa_cell_sum_kernel, dim3(grid_of(num_mov, 256)), dim3(256), 0, s, pin_grad, cell_start,
+                       cell_pin, num_mov, grad);
+    AP_LAUNCHED();
+    return AP_OK;
+}
+
 int launch_wa(const WaArgs& a, int num_pins, hipStream_t s) {
     if (a.num_nets <= 0) return AP_OK;
     unsigned grid = grid_of((long long)num_pins + 1, WA_SPAN);   // +1: a trailing empty net still has an owner
@@ -317,12 +326,7 @@ int ap_wa_wirelength(const float* pos, const int* pin_cell, const float* pin_off
              (float2*)pin_grad, (float2*)out, nullptr, nullptr, nullptr, nullptr, nullptr};
     int st = launch_wa(a, num_pins, s);
     if (st) return st;
-    if (num_mov > 0) {
-        hipLaunchKernelGGL(wa_cell_sum_kernel, dim3(grid_of(num_mov, 256)), dim3(256), 0, s, (const float2*)pin_grad,
-                           cell_start, cell_pin, num_mov, (float2*)grad);
-        AP_LAUNCHED();
-    }
-    return AP_OK;
+    return launch_wa_cell_sum((const float2*)pin_grad, cell_start, cell_pin, num_mov, (float2*)grad, s);
 }
 
 int ap_nesterov_step(const float* u, const float* v, const float* g, const float* lo, const float* hi,

@@ -19,6 +19,8 @@ struct WaveArgs {
     const float* gamma;
     float2* pin_grad;
     double* partial;           // per workgroup (WA, HPWL)
+    float2* out;               // per net (WA, HPWL) (the drop-in's planned WA; null in the engine)
+    const int* lane_net;       // with out: per lane, the net of the lane's group (-1: none)
 };
 
 // grouped launch of many levels' WA (tables on the device; see wa_wave_grouped)
@@ -30,12 +32,14 @@ struct WaPlan {
     int4* d_lanes = nullptr;         // per lane, in wavefront order: pin cell (-1: no pin), gradient slot (-1: none),
                                      // pin offset x, y (float bits)
     int* d_big = nullptr;            // nets of more than 64 pins
-    // pin_slot: where pin p's gradient goes (null: p)
+    int* d_net = nullptr;            // with_nets: per lane, the net of the lane's group (-1: none)
+    // pin_slot: where pin p's gradient goes (null: p); with_nets: the per-lane nets, for per-net outputs
     int create(const int* d_net_start, int num_nets, const int* d_pin_cell, const float* d_pin_off,
-               const int* d_pin_slot, int num_pins);
+               const int* d_pin_slot, int num_pins, bool with_nets = false);
     void destroy();
     int wave_blocks() const { return (num_waves + 4 * WA_PLAN_PER_WAVE - 1) / (4 * WA_PLAN_PER_WAVE); }
     int partial_count() const { return wave_blocks() + num_big; }   // (WA, HPWL) pairs written per launch
+    // a.out set: also every net's (WA, HPWL) (a plan made with_nets)
     int launch(const WaArgs& a, double* partial, hipStream_t s) const;
     WaveArgs wave_args(const WaArgs& a, double* partial) const;
 };

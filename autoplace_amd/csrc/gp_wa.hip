@@ -58,9 +58,11 @@ __device__ __forceinline__ float group_reduce(float v, Op op) {
     return v;
 }
 
-// q: the position of the pin's cell (zero for a fixed pin or an empty lane), loaded by the caller
-template <int LG>
-__device__ __forceinline__ void wa_group(const WaveArgs& a, float2 q, const int4& ln, int lane, double (&acc)[2]) {
+// q: the position of the pin's cell (zero for a fixed pin or an empty lane), loaded by the caller. OUT: the group's
+// first lane also writes the net's (WA, HPWL) to a.out[net] ((0, 0) for a net without pins)
+template <int LG, bool OUT = false>
+__device__ __forceinline__ void wa_group(const WaveArgs& a, float2 q, const int4& ln, int lane, double (&acc)[2],
+                                         int net = -1) {
     constexpr int G = 1 << LG;
     const int sub = lane & (G - 1);
     const int c = ln.x, slot = ln.y;
@@ -91,10 +93,13 @@ __device__ __forceinline__ void wa_group(const WaveArgs& a, float2 q, const int4
             acc[1] += (double)((hx - lx) + (hy - ly));
         }
     }
+    if (OUT && sub == 0 && net >= 0)
+        a.out[net] = live ? make_float2((xpx - xnx) + (xpy - xny), (hx - lx) + (hy - ly)) : make_float2(0.f, 0.f);
 }
 
 // Each wavefront of the launch runs WA_PLAN_PER_WAVE consecutive wavefronts of the plan, their lane data and cell
 // positions loaded before the first is reduced (their memory waits overlap).
+template <bool OUT = false>
 __device__ __forceinline__ void wa_wave_body(const WaveArgs& a, int wg, int nwg) {
     __shared__ double red[16 * 2];
     const int lane = threadIdx.x & 63;
@@ -102,12 +107,13 @@ __device__ __forceinline__ void wa_wave_body(const WaveArgs& a, int wg, int nwg)
     const float2* pos = a.flag ? (*a.flag == 0 ? a.pos_other : a.pos) : a.pos;
     double acc[2] = {0.0, 0.0};
     int4 ln[WA_PLAN_PER_WAVE];
-    int lg[WA_PLAN_PER_WAVE];
+    int lg[WA_PLAN_PER_WAVE], nt[WA_PLAN_PER_WAVE];
 #pragma unroll
     for (int k = 0; k < WA_PLAN_PER_WAVE; k++) {
         const bool in = w0 + k < a.num_waves;       // uniform over the wavefront
         ln[k] = in ? a.lanes[(long long)(w0 + k) * 64 + lane] : make_int4(-1, -1, 0, 0);
         lg[k] = in ? a.lg[w0 + k] : 0;
+        nt[k] = OUT && in ? a.lane_net[(long long)(w0 + k) * 64 + lane] : -1;
     }
     float2 q[WA_PLAN_PER_WAVE];
 #pragma unroll
@@ -117,14 +123,15 @@ __device__ __forceinline__ void wa_wave_body(const WaveArgs& a, int wg, int nwg)
     for (int k = 0; k < WA_PLAN_PER_WAVE; k++) {
         switch (lg[k]) {   // uniform over the wavefront
             case 0: break;
-            case 1: wa_group<1>(a, q[k], ln[k], lane, acc); break;
-            case 2: wa_group<2>(a, q[k], ln[k], lane, acc); break;
-            case 3: wa_group<3>(a, q[k], ln[k], lane, acc); break;
-            case 4: wa_group<4>(a, q[k], ln[k], lane, acc); break;
-            case 5: wa_group<5>(a, q[k], ln[k], lane, acc); break;
-            default: wa_group<6>(a, q[k], ln[k], lane, acc); break;
+            case 1: wa_group<1, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
+            case 2: wa_group<2, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
+            case 3: wa_group<3, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
+            case 4: wa_group<4, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
+            case 5: wa_group<5, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
+            default: wa_group<6, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
         }
     }
+    if (OUT) return;                                 // per-net outputs only: no workgroup partials
     block_sum<2>(acc, red);
     if (threadIdx.x == 0) {
         a.partial[2 * wg] = acc[0];
@@ -133,6 +140,10 @@ __device__ __forceinline__ void wa_wave_body(const WaveArgs& a, int wg, int nwg)
 }
 
 __global__ __launch_bounds__(256) void wa_wave_kernel(WaveArgs a) { wa_wave_body(a, blockIdx.x, gridDim.x); }
+// the drop-in's planned WA: also every net's (WA, HPWL)
+__global__ __launch_bounds__(256) void wa_wave_nets_kernel(WaveArgs a) {
+    wa_wave_body<true>(a, blockIdx.x, gridDim.x);
+}
 
 // many levels per launch: workgroup b does map[b] = (level, workgroup, workgroups of the level, 0); a level whose
 // control state says stopped (S[STOP] != 0) is skipped
@@ -152,7 +163,8 @@ int launch_wa_grouped(const WaveArgs* d_tab, const int4* d_map, int count, const
 }
 
 WaveArgs WaPlan::wave_args(const WaArgs& a, double* partial) const {
-    return WaveArgs{a.pos, a.pos_other, a.flag, d_lg, d_lanes, num_waves, a.num_mov, a.gamma, a.pin_grad, partial};
+    return WaveArgs{a.pos,     a.pos_other, a.flag, d_lg,  d_lanes, num_waves, a.num_mov, a.gamma, a.pin_grad,
+                    partial,   a.out,       d_net};
 }
 
 // nets of more than 64 pins: one workgroup per net, the drop-in kernel's workgroup-wide path
@@ -162,6 +174,7 @@ __global__ __launch_bounds__(WA_BLOCK) void wa_big_nets_kernel(WaArgs a, const i
     double acc[2] = {0.0, 0.0};
     if (a.flag && *a.flag == 0) a.pos = a.pos_other;
     wa_big_net_path(a, big[blockIdx.x], acc[0], acc[1], red);
+    if (!partial) return;                            // per-net outputs only
     block_sum<2>(acc, dred);
     if (threadIdx.x == 0) {
         partial[2 * blockIdx.x] = acc[0];
@@ -249,7 +262,7 @@ __global__ __launch_bounds__(256) void wa_plan_lanes(const int* net_start, const
                                                      int num_nets, ClassOffsets o, ClassOffsets count,
                                                      ClassOffsets partial_wave, const int* by_rank,
                                                      const int* wave_of_completing, int* lane_pin,
-                                                     unsigned char* lg) {
+                                                     unsigned char* lg, int* lane_net) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= num_nets) return;
     const int k = cls[i];
@@ -260,6 +273,7 @@ __global__ __launch_bounds__(256) void wa_plan_lanes(const int* net_start, const
     const int base = wave * 64 + (r % per_wave) * (1 << k);
     for (int q = net_start[i], q1 = net_start[i + 1]; q < q1; q++) lane_pin[base + (q - net_start[i])] = q;
     if (r % per_wave == 0) lg[wave] = (unsigned char)k;
+    if (lane_net) lane_net[base] = i;                        // the group's first lane, also for a net of no pins
 }
 
 __global__ __launch_bounds__(256) void wa_plan_clear(int* a, long long n) {
@@ -268,7 +282,7 @@ __global__ __launch_bounds__(256) void wa_plan_clear(int* a, long long n) {
 }
 
 int WaPlan::create(const int* d_net_start, int num_nets, const int* d_pin_cell, const float* d_pin_off,
-                   const int* d_pin_slot, int num_pins) {
+                   const int* d_pin_slot, int num_pins, bool with_nets) {
     num_waves = num_big = 0;
     if (num_nets <= 0) return AP_OK;
     int first = 0, end = 0;
@@ -353,9 +367,14 @@ int WaPlan::create(const int* d_net_start, int num_nets, const int* d_pin_cell, 
             release();
             AP_HIP(e);
         }
+        if (with_nets && (e = hipMalloc(&d_net, n * sizeof(int))) != hipSuccess) {
+            release();
+            AP_HIP(e);
+        }
         hipLaunchKernelGGL(wa_plan_clear, dim3(grid_of(n, 256)), b, 0, nullptr, lane_pin, n);
+        if (d_net) hipLaunchKernelGGL(wa_plan_clear, dim3(grid_of(n, 256)), b, 0, nullptr, d_net, n);
         hipLaunchKernelGGL(wa_plan_lanes, g, b, 0, nullptr, d_net_start, cls, rank, num_nets, off, count, partial,
-                           by_rank, order, lane_pin, d_lg);
+                           by_rank, order, lane_pin, d_lg, d_net);
         hipLaunchKernelGGL(wa_lanes_fill, dim3(grid_of(n, 256)), b, 0, nullptr, lane_pin, n, d_pin_cell,
                            (const float2*)d_pin_off, d_pin_slot, d_lanes);
     }
@@ -369,23 +388,86 @@ void WaPlan::destroy() {
     if (d_lg) (void)hipFree(d_lg);
     if (d_lanes) (void)hipFree(d_lanes);
     if (d_big) (void)hipFree(d_big);
+    if (d_net) (void)hipFree(d_net);
     d_lg = nullptr;
     d_lanes = nullptr;
     d_big = nullptr;
+    d_net = nullptr;
 }
 
 int WaPlan::launch(const WaArgs& a, double* partial, hipStream_t s) const {
     if (num_waves) {
         const WaveArgs w = wave_args(a, partial);
-        hipLaunchKernelGGL(wa_wave_kernel, dim3(wave_blocks()), dim3(256), 0, s, w);
+        if (a.out) {
+            AP_REQUIRE(d_net, "wa plan: per-net outputs need a plan made with its nets");
+            hipLaunchKernelGGL(wa_wave_nets_kernel, dim3(wave_blocks()), dim3(256), 0, s, w);
+        } else {
+            hipLaunchKernelGGL(wa_wave_kernel, dim3(wave_blocks()), dim3(256), 0, s, w);
+        }
         AP_LAUNCHED();
     }
     if (num_big) {
         hipLaunchKernelGGL(wa_big_nets_kernel, dim3(num_big), dim3(WA_BLOCK), 0, s, a, d_big, num_big,
-                           partial + 2LL * wave_blocks());
+                           partial ? partial + 2LL * wave_blocks() : nullptr);
         AP_LAUNCHED();
     }
     return AP_OK;
 }
 
 }  // namespace ap
+
+// ------------------------------------------------------------------------------ C ABI: the drop-in WA with a plan
+using namespace ap;
+
+struct ap_wa_plan {
+    WaPlan plan;
+    int num_nets = 0, num_pins = 0;
+};
+
+extern "C" {
+
+int ap_wa_plan_create(const int* pin_cell, const float* pin_off, const int* net_start, int num_nets, int num_pins,
+                      ap_wa_plan** plan) {
+    AP_REQUIRE(plan && num_nets >= 0 && num_pins >= 0, "wa_plan_create: bad arguments");
+    AP_REQUIRE(num_nets == 0 || (pin_cell && pin_off && net_start), "wa_plan_create: null pointer");
+    ap_wa_plan* h = new ap_wa_plan();
+    h->num_nets = num_nets;
+    h->num_pins = num_pins;
+    const int st = h->plan.create(net_start, num_nets, pin_cell, pin_off, nullptr, num_pins, true);
+    if (st) {
+        h->plan.destroy();
+        delete h;
+        return st;
+    }
+    *plan = h;
+    return AP_OK;
+}
+
+int ap_wa_plan_destroy(ap_wa_plan* plan) {
+    if (plan) {
+        (void)hipDeviceSynchronize();
+        plan->plan.destroy();
+        delete plan;
+    }
+    return AP_OK;
+}
+
+int ap_wa_wirelength_planned(const ap_wa_plan* plan, const float* pos, const int* pin_cell, const float* pin_off,
+                             const int* net_start, const int* cell_start, const int* cell_pin, int num_nets,
+                             int num_mov, const float* gamma, float* pin_grad, float* out, float* grad, int num_pins,
+                             void* stream) {
+    AP_REQUIRE(plan, "wa_planned: null plan");
+    AP_REQUIRE(num_nets == plan->num_nets && num_pins == plan->num_pins,
+               "wa_planned: the plan is of %d nets and %d pins, not %d and %d", plan->num_nets, plan->num_pins,
+               num_nets, num_pins);
+    AP_REQUIRE(num_mov >= 0, "wa_planned: negative size");
+    AP_REQUIRE(num_nets == 0 || (net_start && gamma && pin_grad && out), "wa_planned: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const WaArgs a{(const float2*)pos, pin_cell, (const float2*)pin_off, net_start, num_nets, num_mov, gamma,
+                   (float2*)pin_grad, (float2*)out, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int st = plan->plan.launch(a, nullptr, s);
+    if (st) return st;
+    return launch_wa_cell_sum((const float2*)pin_grad, cell_start, cell_pin, num_mov, (float2*)grad, s);
+}
+
+}  // extern "C"

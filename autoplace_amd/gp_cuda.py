@@ -7,7 +7,9 @@ dtype (RuntimeError otherwise), allocates its outputs on the inputs' device, and
 
 `poisson_field` replaces the torch.fft call of src/core/electronic_density_layer.py:133.
 """
+import collections
 import ctypes
+import weakref
 
 import torch
 
@@ -29,16 +31,77 @@ D = lambda t, w="": _ptr(t, torch.float64, w)      # noqa: E731
 I = lambda t, w="": _ptr(t, torch.int32, w)        # noqa: E731
 
 
-def wa(pos, pin_cell, pin_off, net_start, cell_start, cell_pin, num_mov, gamma):
-    """-> (gradient [num_mov, 2], per-net (WA, HPWL) [num_nets, 2]) (gp_cuda.cpp:31-38)."""
+class _WaPlan(object):
+    """A netlist's WA plan (ap_wa_plan), destroyed with this object."""
+
+    def __init__(self, pin_cell, pin_off, net_start, num_nets):
+        torch.cuda.current_stream().synchronize()      # the plan reads the netlist now, outside the stream
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().ap_wa_plan_create(I(pin_cell, "pin_cell"), F(pin_off, "pin_off"),
+                                                I(net_start, "net_start"), num_nets, pin_cell.numel(),
+                                                ctypes.byref(h)), "wa_plan_create")
+        self.handle = h
+        self.versions = (pin_cell._version, pin_off._version, net_start._version)
+
+    def close(self):
+        if self.handle is not None:
+            _lib.lib().ap_wa_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:                              # interpreter shutdown
+            pass
+
+
+# the plans of the netlists wa() has seen, by the netlist tensors' storage; an entry goes when one of its tensors is
+# freed, when one of them was changed in place (torch's version counter), or as the oldest of more than _WA_PLANS
+_WA_PLANS = 8
+_wa_plans = collections.OrderedDict()
+
+
+def _wa_plan(pin_cell, pin_off, net_start, num_nets):
+    key = (pin_cell.data_ptr(), pin_off.data_ptr(), net_start.data_ptr(), pin_cell.numel(), num_nets)
+    plan = _wa_plans.get(key)
+    if plan is not None and plan.versions == (pin_cell._version, pin_off._version, net_start._version):
+        _wa_plans.move_to_end(key)
+        return plan
+    if plan is not None:
+        _wa_plans.pop(key).close()
+    plan = _WaPlan(pin_cell, pin_off, net_start, num_nets)
+    _wa_plans[key] = plan
+    for t in (pin_cell, pin_off, net_start):
+        weakref.finalize(t, _drop_wa_plan, key, plan)
+    while len(_wa_plans) > _WA_PLANS:
+        _wa_plans.popitem(last=False)[1].close()
+    return plan
+
+
+def _drop_wa_plan(key, plan):
+    if _wa_plans.get(key) is plan:
+        del _wa_plans[key]
+    plan.close()
+
+
+def wa(pos, pin_cell, pin_off, net_start, cell_start, cell_pin, num_mov, gamma, planned=True):
+    """-> (gradient [num_mov, 2], per-net (WA, HPWL) [num_nets, 2]) (gp_cuda.cpp:31-38).
+
+    planned (not in the reference's signature; default on): the netlist's WA plan, made on the first call with
+    these netlist tensors and kept (ap_wa_wirelength_planned); False: the plan-free kernel (ap_wa_wirelength). The
+    two agree to float rounding."""
     num_nets = net_start.size(0) - 1
     pin_grad = torch.empty_like(pin_off)
     out = pos.new_empty((num_nets, 2))
     grad = pos.new_empty((num_mov, 2))
-    _lib.check(_lib.lib().ap_wa_wirelength(F(pos, "pos"), I(pin_cell, "pin_cell"), F(pin_off, "pin_off"),
-                                           I(net_start, "net_start"), I(cell_start, "cell_start"),
-                                           I(cell_pin, "cell_pin"), num_nets, num_mov, F(gamma, "gamma"),
-                                           F(pin_grad), F(out), F(grad), pin_cell.numel(), _stream()), "wa")
+    args = (F(pos, "pos"), I(pin_cell, "pin_cell"), F(pin_off, "pin_off"), I(net_start, "net_start"),
+            I(cell_start, "cell_start"), I(cell_pin, "cell_pin"), num_nets, num_mov, F(gamma, "gamma"), F(pin_grad),
+            F(out), F(grad), pin_cell.numel(), _stream())
+    if planned and num_nets > 0:
+        plan = _wa_plan(pin_cell, pin_off, net_start, num_nets)
+        _lib.check(_lib.lib().ap_wa_wirelength_planned(plan.handle, *args), "wa")
+    else:
+        _lib.check(_lib.lib().ap_wa_wirelength(*args), "wa")
     return grad, out
 
 

@@ -104,7 +104,9 @@ class Wirelength(object):
         s = self.opt
         if not torch.is_tensor(gamma):
             gamma = self._gamma.fill_(gamma)
+        # a level evaluates this host-side twin a few times only (the engine has its own plan): the plan-free
+        # kernel, which needs no set-up
         grad, out = gp_cuda.wa(pos, s["cell32"], s["off"], s["net_start"], s["cell_start"], s["cell_pin"],
-                               self.num_mov, gamma)
+                               self.num_mov, gamma, planned=False)
         total = out.double().sum(0)
         return total[0], grad, total[1]

@@ -44,6 +44,23 @@ AP_API int ap_wa_wirelength(const float* pos, const int* pin_cell, const float* 
                      const int* cell_start, const int* cell_pin, int num_nets, int num_mov, const float* gamma,
                      float* pin_grad, float* out, float* grad, int num_pins, void* stream);
 
+/* WA wirelength with a plan of its netlist: the same arguments and outputs as ap_wa_wirelength (gp_cuda.cpp:4-5),
+ * plus the plan. The plan (no counterpart in the reference, whose kernel gives each net one thread) lays the netlist
+ * out once, on the device, as the GP engine's WA: every net of d <= 64 pins in an aligned group of 2^ceil(log2 d)
+ * lanes, one degree class per wavefront, each lane one pin, the net's max / min and exponential sums as butterfly
+ * reductions over its group (nets of more pins: one workgroup each). It copies what it needs of pin_cell, pin_off and
+ * net_start, so a plan describes the netlist as it was at ap_wa_plan_create: a changed netlist needs a new plan.
+ * create is synchronous (the netlist must be written before the call); destroy synchronises the device. The per-net
+ * values and gradients agree with ap_wa_wirelength's to float rounding (the sums are taken in another order). */
+typedef struct ap_wa_plan ap_wa_plan;
+AP_API int ap_wa_plan_create(const int* pin_cell, const float* pin_off, const int* net_start, int num_nets,
+                             int num_pins, ap_wa_plan** plan);
+AP_API int ap_wa_plan_destroy(ap_wa_plan* plan);
+AP_API int ap_wa_wirelength_planned(const ap_wa_plan* plan, const float* pos, const int* pin_cell,
+                                    const float* pin_off, const int* net_start, const int* cell_start,
+                                    const int* cell_pin, int num_nets, int num_mov, const float* gamma,
+                                    float* pin_grad, float* out, float* grad, int num_pins, void* stream);
+
 /* Density map (gp_cuda.cpp:6-7 `density_map`; kernels gp_cuda_kernel.cu:84-95,130-146), computed by the GP
  * engine's tile binning (per-tile LDS sums of the reference's 2^32 fixed-point contributions: bit-exact).
  * map: the caller's u64[2][nbx][nby] (the reference's binding allocates it zeroed), here only a workspace: when it

@@ -36,8 +36,10 @@ def small(small_design):
     return data, p, pos, smooth, ratio, bx, by
 
 
+@pytest.mark.parametrize("planned", [True, False])
 @pytest.mark.parametrize("gamma", [0.5, 4.0, 60.0])
-def test_wa_parity(small, gamma):
+def test_wa_parity(small, gamma, planned):
+    """gp_cuda.wa with the netlist's plan (lane groups per net) and without it (one lane per net)."""
     from autoplace_amd import gp_cuda
     data, p, pos, *_ = small
     csr = owa.build_csr(data.pin_cell, data.pin_off, data.pin_net, data.num_nets, data.num_mov, 100)
@@ -45,21 +47,24 @@ def test_wa_parity(small, gamma):
     wa64, grad64, hpwl64, out64, _ = owa.wa_forward(pos, csr, gamma, np.float64)
     g_dev, out_dev = gp_cuda.wa(_t(pos), _t(csr["cell"], torch.int32), _t(csr["off"]), _t(csr["net_start"], torch.int32),
                                 _t(csr["cell_start"], torch.int32), _t(csr["cell_pin"], torch.int32), data.num_mov,
-                                _t([gamma]))
+                                _t([gamma]), planned=planned)
     g_dev, out_dev = g_dev.cpu().numpy(), out_dev.cpu().numpy()
     e_out, e_grad = rel_err(out_dev, out), norm_err(g_dev, grad)
-    print("WA gamma %g: out err vs fp32 %.2e (vs fp64 %.2e); grad err vs fp32 %.2e (vs fp64 %.2e)"
-          % (gamma, e_out, rel_err(out_dev, out64), e_grad, norm_err(g_dev, grad64)))
+    print("WA gamma %g (%s): out err vs fp32 %.2e (vs fp64 %.2e); grad err vs fp32 %.2e (vs fp64 %.2e)"
+          % (gamma, "planned" if planned else "plan-free", e_out, rel_err(out_dev, out64), e_grad,
+             norm_err(g_dev, grad64)))
     assert e_out < 1e-5 and e_grad < 1e-5
     assert abs(out_dev[:, 1].astype(np.float64).sum() - hpwl) / hpwl < 1e-6
 
 
-def test_wa_big_net_and_fixed_pins():
-    """A net larger than the LDS stage (the block-wide path), fixed pins, degree-2 nets around it."""
+@pytest.mark.parametrize("planned", [True, False])
+def test_wa_big_net_and_fixed_pins(planned):
+    """A net larger than the LDS stage and the 64-lane groups (the block-wide path), fixed pins, nets of one pin,
+    degree-2 nets around them."""
     from autoplace_amd import gp_cuda
     rng = np.random.default_rng(9)
     n = 3000
-    deg = [2, 3, 700, 2, 1500, 5, 2]
+    deg = [2, 3, 700, 2, 1, 1500, 5, 64, 65, 2, 1]
     cells = np.concatenate([rng.choice(n + 1, d, replace=False) for d in deg])     # n = fixed pin
     net = np.repeat(np.arange(len(deg)), deg)
     off = rng.uniform(-2, 2, (len(cells), 2)).astype(np.float32)
@@ -68,9 +73,45 @@ def test_wa_big_net_and_fixed_pins():
     csr = owa.build_csr(cells, off, net, len(deg), n, 10 ** 6)
     wa, grad, hpwl, out, _ = owa.wa_forward(pos, csr, 7.0, np.float32)
     g_dev, out_dev = gp_cuda.wa(_t(pos), _t(csr["cell"], torch.int32), _t(csr["off"]), _t(csr["net_start"], torch.int32),
-                                _t(csr["cell_start"], torch.int32), _t(csr["cell_pin"], torch.int32), n, _t([7.0]))
+                                _t(csr["cell_start"], torch.int32), _t(csr["cell_pin"], torch.int32), n, _t([7.0]),
+                                planned=planned)
     assert rel_err(out_dev.cpu().numpy(), out) < 1e-5
     assert norm_err(g_dev.cpu().numpy(), grad) < 1e-5
+
+
+def test_wa_plan_follows_the_netlist():
+    """The drop-in's WA plan: nets of no pin give (0, 0) as the plan-free kernel does, every output agrees with the
+    plan-free kernel's to float rounding, and a netlist changed in place (its tensors' version counters) is planned
+    again: the next call sees the new pins."""
+    from autoplace_amd import gp_cuda
+    rng = np.random.default_rng(4)
+    n = 500
+    deg = rng.integers(2, 9, 300)
+    deg[[3, 77, 299]] = 0                             # empty nets, one of them the last
+    deg[[10, 11]] = [64, 1]
+    net_start = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    cells = rng.integers(0, n + 1, int(deg.sum())).astype(np.int32)    # n = fixed pin
+    off = rng.uniform(-2, 2, (len(cells), 2)).astype(np.float32)
+    pos = _t(rng.uniform(0, 300, (n, 2)).astype(np.float32))
+    order = np.argsort(cells, kind="stable")
+    cell_start = np.searchsorted(cells[order], np.arange(n + 1)).astype(np.int32)
+    t_cells, t_off, t_ns = _t(cells, torch.int32), _t(off), _t(net_start, torch.int32)
+    t_cs, t_cp = _t(cell_start, torch.int32), _t(order.astype(np.int32), torch.int32)
+    gamma = _t([5.0])
+
+    def both():
+        a = gp_cuda.wa(pos, t_cells, t_off, t_ns, t_cs, t_cp, n, gamma, planned=True)
+        b = gp_cuda.wa(pos, t_cells, t_off, t_ns, t_cs, t_cp, n, gamma, planned=False)
+        return [x.cpu().numpy() for x in a], [x.cpu().numpy() for x in b]
+
+    (g1, o1), (g0, o0) = both()
+    assert np.array_equal(o1[[3, 77, 299]], np.zeros((3, 2), np.float32))
+    assert np.array_equal(o0[[3, 77, 299]], np.zeros((3, 2), np.float32))
+    assert rel_err(o1, o0) < 1e-5 and norm_err(g1, g0) < 1e-5
+    t_off.mul_(3.0)                                   # the same tensors, new offsets: a new plan
+    (g2, o2), (h0, p0) = both()
+    assert rel_err(o2, p0) < 1e-5 and norm_err(g2, h0) < 1e-5
+    assert rel_err(o2, o1) > 1e-3
 
 
 def test_density_map_bit_exact(small):
