@@ -64,6 +64,8 @@ _SIGNATURES = {
     "ap_level_autotune": ([P, I, I, P], I),
     "ap_level_set_tile_layout": ([P, I, I, I, I], I),
     "ap_level_set_bucket_room": ([P, I, I], I),
+    "ap_level_launch_shape": ([P, P], I),
+    "ap_level_set_launch_shape": ([P, I, I, I], I),
     "ap_level_time_every": ([P, I], I),
     "ap_level_bucket_spills": ([P, P], I),
     "ap_batch_create": ([P, I, ctypes.POINTER(P)], I),

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "gp_common.h"
@@ -776,31 +777,30 @@ __device__ __forceinline__ float2 cell_pin_sum(const GatherArgs& a, int i) {
     return make_float2(wx, wy);
 }
 
-// The cells' WA gradients and part_wl[b] (their squared norms summed over workgroup b, as the gather would sum
-// them): the part of the gather that needs only WA, run beside the density stage. Same workgroups as the gather.
+// The cells' WA gradients and part_wl[w] (their squared norms summed over the wavefront of nodes w = i / 64, as the
+// gather would sum them): the part of the gather that needs only WA, run beside the density stage, over the nodes.
 __global__ __launch_bounds__(EB) void eng_cell_wl(GatherArgs a, float2* cell_wl, double* part_wl) {
-    __shared__ double red[16];
     const int i = xcd_remap(blockIdx.x, gridDim.x) * EB + threadIdx.x;
-    double t2[1] = {0.0};
+    double t2 = 0.0;
     if (i < a.num_mov) {
         const float2 w = cell_pin_sum(a, i);
         cell_wl[i] = w;
-        t2[0] = (double)w.x * w.x + (double)w.y * w.y;
+        t2 = (double)w.x * w.x + (double)w.y * w.y;
     }
-    block_sum<1>(t2, red);
-    if (threadIdx.x == 0) part_wl[blockIdx.x] = t2[0];
+    t2 = wave_sum(t2);
+    if ((threadIdx.x & 63) == 0 && i < a.num_nodes) part_wl[i >> 6] = t2;
 }
 
-// part[b]: |g' - g|^2 of workgroup b; part_wl[b], part_dn[b]: squared norms of the wirelength part and of the
-// weighted density part of the (unpreconditioned) gradient, the learned controller's gradient-ratio feature
-// (part_wl is eng_cell_wl's when a.cell_wl is set).
+// part[w]: |g' - g|^2 of the wavefront of nodes w = i / 64; part_wl[w], part_dn[w]: squared norms of the wirelength
+// part and of the weighted density part of the (unpreconditioned) gradient, the learned controller's gradient-ratio
+// feature (part_wl is eng_cell_wl's when a.cell_wl is set). Partials per wavefront of nodes: the same sums for any
+// workgroup size (the autotuner's choice).
 // DROPIN: the drop-in density_grad -- cell_wl is the caller's wirelength gradient of the first num_mov nodes (no
 // extra row), the weights come from weight_p / precond_p, and no partials are written.
 template <bool DROPIN = false>
 __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, double* part_wl, double* part_dn, int wg,
                                             int nwg) {
-    __shared__ double red[16 * 3];
-    const int i = xcd_remap(wg, nwg) * EB + threadIdx.x;
+    const int i = xcd_remap(wg, nwg) * (int)blockDim.x + threadIdx.x;
     double t2[3] = {0.0, 0.0, 0.0};
     if (i < a.num_nodes) {
         int cur = a.flags ? a.flags[0] : 0;
@@ -889,14 +889,15 @@ __device__ __forceinline__ void gather_body(const GatherArgs& a, double* part, d
         }
     }
     if (DROPIN) return;
-    block_sum<3>(t2, red);
-    if (threadIdx.x == 0) {
-        part[wg] = t2[0];
-        if (!a.cell_wl) part_wl[wg] = t2[1];
-        part_dn[wg] = t2[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) t2[k] = wave_sum(t2[k]);
+    if ((threadIdx.x & 63) == 0 && i < a.num_nodes) {
+        part[i >> 6] = t2[0];
+        if (!a.cell_wl) part_wl[i >> 6] = t2[1];
+        part_dn[i >> 6] = t2[2];
     }
 }
-__global__ __launch_bounds__(EB) void eng_gather(GatherArgs a, double* part, double* part_wl, double* part_dn) {
+__global__ __launch_bounds__(512) void eng_gather(GatherArgs a, double* part, double* part_wl, double* part_dn) {
     gather_body(a, part, part_wl, part_dn, blockIdx.x, gridDim.x);
 }
 
@@ -919,6 +920,7 @@ struct Segs {
 // (moved^2, turned^2, WA, HPWL, overflow, wl^2, density^2) -> out[b][RED_VALS]. The slices and the order inside
 // them do not depend on the launch, so the sums are deterministic.
 constexpr int RED_BLOCKS = 64, RED_VALS = 7;
+constexpr int RED_LANES = EB;   // lanes per workgroup of the first stage
 
 __device__ __forceinline__ void slice(int n, int wg, int& lo, int& hi) {
     const int per = (n + RED_BLOCKS - 1) / RED_BLOCKS;
@@ -926,8 +928,9 @@ __device__ __forceinline__ void slice(int n, int wg, int& lo, int& hi) {
     hi = min(n, lo + per);
 }
 
-// (a slice is mostly at most EB long: each lane's first element of every segment is loaded before any is added --
-// one memory wait, not one per segment -- and any further ones follow in the loops; the per-lane order is the same)
+// (each lane's first RED_AHEAD elements of every segment are loaded before any is added -- one memory wait, not one
+// per element and segment -- and any further ones follow in the loops; each lane adds its elements in slice order)
+constexpr int RED_AHEAD = 4;
 __device__ __forceinline__ void reduce_body(const Segs& s, double* out, int wg) {
     __shared__ double red[16 * RED_VALS];
     const int t = threadIdx.x;
@@ -937,25 +940,37 @@ __device__ __forceinline__ void reduce_body(const Segs& s, double* out, int wg) 
     slice(s.n_wa, wg, lo[2], hi[2]);
     slice(s.n_over, wg, lo[3], hi[3]);
     slice(s.n_norm, wg, lo[4], hi[4]);
-    bool in[5];
+    double x[RED_AHEAD][RED_VALS];
 #pragma unroll
-    for (int q = 0; q < 5; q++) in[q] = lo[q] + t < hi[q];
+    for (int j = 0; j < RED_AHEAD; j++) {
+        bool in[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) in[q] = lo[q] + t + j * RED_LANES < hi[q];
+        const int o = t + j * RED_LANES;
+        x[j][0] = in[0] ? s.moved[lo[0] + o] : 0.0;
+        x[j][1] = in[1] ? s.turned[lo[1] + o] : 0.0;
+        x[j][2] = in[2] ? s.wa[2 * (lo[2] + o)] : 0.0;
+        x[j][3] = in[2] ? s.wa[2 * (lo[2] + o) + 1] : 0.0;
+        x[j][4] = in[3] ? s.over[lo[3] + o] : 0.0;
+        x[j][5] = in[4] ? s.wl2[lo[4] + o] : 0.0;
+        x[j][6] = in[4] ? s.dn2[lo[4] + o] : 0.0;
+    }
     double v[RED_VALS];
-    v[0] = in[0] ? s.moved[lo[0] + t] : 0.0;
-    v[1] = in[1] ? s.turned[lo[1] + t] : 0.0;
-    v[2] = in[2] ? s.wa[2 * (lo[2] + t)] : 0.0;
-    v[3] = in[2] ? s.wa[2 * (lo[2] + t) + 1] : 0.0;
-    v[4] = in[3] ? s.over[lo[3] + t] : 0.0;
-    v[5] = in[4] ? s.wl2[lo[4] + t] : 0.0;
-    v[6] = in[4] ? s.dn2[lo[4] + t] : 0.0;
-    for (int k = lo[0] + t + EB; k < hi[0]; k += EB) v[0] += s.moved[k];
-    for (int k = lo[1] + t + EB; k < hi[1]; k += EB) v[1] += s.turned[k];
-    for (int k = lo[2] + t + EB; k < hi[2]; k += EB) {
+#pragma unroll
+    for (int k = 0; k < RED_VALS; k++) {
+        v[k] = x[0][k];
+#pragma unroll
+        for (int j = 1; j < RED_AHEAD; j++) v[k] += x[j][k];   // elements past the slice add 0.0: no change
+    }
+    const int more = t + RED_AHEAD * RED_LANES;
+    for (int k = lo[0] + more; k < hi[0]; k += RED_LANES) v[0] += s.moved[k];
+    for (int k = lo[1] + more; k < hi[1]; k += RED_LANES) v[1] += s.turned[k];
+    for (int k = lo[2] + more; k < hi[2]; k += RED_LANES) {
         v[2] += s.wa[2 * k];
         v[3] += s.wa[2 * k + 1];
     }
-    for (int k = lo[3] + t + EB; k < hi[3]; k += EB) v[4] += s.over[k];
-    for (int k = lo[4] + t + EB; k < hi[4]; k += EB) {
+    for (int k = lo[3] + more; k < hi[3]; k += RED_LANES) v[4] += s.over[k];
+    for (int k = lo[4] + more; k < hi[4]; k += RED_LANES) {
         v[5] += s.wl2[k];
         v[6] += s.dn2[k];
     }
@@ -963,7 +978,7 @@ __device__ __forceinline__ void reduce_body(const Segs& s, double* out, int wg) 
     if (t == 0)
         for (int k = 0; k < RED_VALS; k++) out[wg * RED_VALS + k] = v[k];
 }
-__global__ __launch_bounds__(EB) void eng_reduce(Segs s, double* out) { reduce_body(s, out, blockIdx.x); }
+__global__ __launch_bounds__(RED_LANES) void eng_reduce(Segs s, double* out) { reduce_body(s, out, blockIdx.x); }
 
 // Second stage and the control. With a learned controller (ctl.params) the 64 lanes evaluate the MLP on the pass's
 // features first; its clipped outputs (plus the exploration noise, when asked) scale the rule's density-weight
@@ -1235,6 +1250,8 @@ struct Level {
     Ctl ctl{};                 // the learned controller (params null: the plain rule)
     int* flags = nullptr;
     int nblk_n = 0, nblk_wa = 0;
+    int wav_n = 0;             // wavefronts of 64 nodes: the gather's and the cells' WA sums' partials
+    int gather_lanes = EB;     // lanes per workgroup of the gather (the autotuner's choice; results do not depend on it)
     double *p_moved = nullptr, *p_wa = nullptr, *p_over = nullptr, *p_turned = nullptr, *p_wl2 = nullptr,
            *p_dn2 = nullptr;
     hipStream_t stream = nullptr;
@@ -1324,7 +1341,7 @@ struct Level {
     size_t state_bytes() const;
     int state_copy(char* keep, bool restore, hipStream_t s);
     int time_passes(int reps, hipStream_t s, float& ms);
-    int autotune(int reps, hipStream_t s, int* sides, bool online = false);
+    int autotune(int reps, hipStream_t s, int* sides, bool online = false, bool shapes = true);
     int create(const ap_level_desc& desc);
     void destroy();
     int objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev, const float2* g_prev_b,
@@ -1549,7 +1566,7 @@ int Level::time_passes(int reps, hipStream_t s, float& ms) {
 // starts from the current state (saved once, restored before every timing and at the end). Online, the timings are
 // the level's own next passes: they advance the level as any pass does (the control counts them), so the tuning
 // costs no extra passes -- the passes end bit for bit where untuned ones end, as the layout changes no result.
-int Level::autotune(int reps, hipStream_t s, int* sides, bool online) {
+int Level::autotune(int reps, hipStream_t s, int* sides, bool online, bool shapes) {
     AP_HIP(hipDeviceSynchronize());
     int st;
     char* keep = nullptr;
@@ -1597,6 +1614,40 @@ int Level::autotune(int reps, hipStream_t s, int* sides, bool online) {
         set_side(*J, best, best_g);
         update_lds();
     }
+    // then the launch shapes of the per-net and per-node kernels (shapes: single-level passes only; the batch's
+    // grouped launches keep theirs): WA's plan wavefronts per launched wavefront and lanes per workgroup, then the
+    // gather's lanes per workgroup. Their partials are per wavefront, so the results do not depend on them either.
+    auto sweep = [&](const std::vector<std::pair<int, int>>& cand, std::function<void(int, int)> apply,
+                     std::pair<int, int> cur) -> int {
+        std::pair<int, int> best = cur;
+        float best_ms = 1e30f;
+        for (const auto& c : cand) {
+            apply(c.first, c.second);
+            float warm = 0, ms = 0, again = 0;
+            int e;
+            if ((e = timed(1, warm))) return e;
+            if (warm > 1.5f * best_ms) continue;
+            if ((e = timed(reps, ms))) return e;
+            if (ms < 1.1f * best_ms) {
+                if ((e = timed(reps, again))) return e;
+                ms = std::min(ms, again);
+            }
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        apply(best.first, best.second);
+        return AP_OK;
+    };
+    if (!st && shapes && wa_plan.num_waves) {
+        std::vector<std::pair<int, int>> cand;
+        for (int p : {1, 2, 4})
+            for (int l : {128, 256, 512}) cand.push_back({p, l});
+        st = sweep(cand, [&](int p, int l) { wa_plan.per = p; wa_plan.lanes = l; }, {wa_plan.per, wa_plan.lanes});
+    }
+    if (!st && shapes)
+        st = sweep({{128, 0}, {256, 0}, {512, 0}}, [&](int l, int) { gather_lanes = l; }, {gather_lanes, 0});
     // offline: the state the timing passes changed, back; then the keep buffer goes
     if (!st && keep) st = state_copy(keep, true, s);
     (void)hipStreamSynchronize(s);
@@ -1685,6 +1736,7 @@ int Level::create(const ap_level_desc& desc) {
         AP_HIP(hipMalloc(&fy, nb * sizeof(float)));
     }
     nblk_n = grid_of(N, EB);
+    wav_n = grid_of(N, 64);
     trace("buffers, pin slots");
     if ((st = wa_plan.create(d.net_start, d.num_nets, d.pin_cell, d.pin_off, pin_slot, d.num_pins))) return st;
     nblk_wa = wa_plan.partial_count();
@@ -1701,15 +1753,15 @@ int Level::create(const ap_level_desc& desc) {
     // one overflow partial per tile: room for the most tiles any side the autotuner may pick gives (a grid has at
     // most BIN_MAX_BUCKETS tiles)
     const long long n_over = 2LL * BIN_MAX_BUCKETS;
-    const long long np = nblk_n + 2LL * nblk_wa + n_over + 3LL * nblk_n;
+    const long long np = nblk_n + 2LL * nblk_wa + n_over + 3LL * wav_n;
     AP_HIP(hipMalloc(&part, np * sizeof(double)));
     AP_HIP(hipMemset(part, 0, np * sizeof(double)));
     p_moved = part;
     p_wa = p_moved + nblk_n;
     p_over = p_wa + 2LL * nblk_wa;
     p_turned = p_over + n_over;
-    p_wl2 = p_turned + nblk_n;
-    p_dn2 = p_wl2 + nblk_n;
+    p_wl2 = p_turned + wav_n;
+    p_dn2 = p_wl2 + wav_n;
     AP_HIP(hipMalloc(&stats, 8 * sizeof(double)));
     AP_HIP(hipMalloc(&red, RED_BLOCKS * RED_VALS * sizeof(double)));
     AP_HIP(hipMalloc(&ctl_x, NUM_X * sizeof(double)));
@@ -1795,7 +1847,7 @@ GatherArgs Level::gather_args(const Pick& pos, const float* kp_ptr, const float2
 
 Segs Level::segs(bool with_state, bool has_prev, int use_density) const {
     return Segs{p_moved, with_state ? nblk_n : 0, p_wa, nblk_wa, p_over, use_density ? fj.tiles + jj.tiles : 0,
-                p_turned, has_prev ? nblk_n : 0, p_wl2, p_dn2, nblk_n};
+                p_turned, has_prev ? wav_n : 0, p_wl2, p_dn2, wav_n};
 }
 
 double Level::over_scale() const { return d.has_judge ? d.judge_bin_area / d.mov_area : d.bin_area / d.mov_area; }
@@ -1876,12 +1928,13 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     }
     mark(AP_PROF_GATHER, s);
     tbegin(AP_PROF_GATHER, s);
-    hipLaunchKernelGGL(eng_gather, dim3(nblk_n), dim3(EB), 0, s, g, p_turned, p_wl2, p_dn2);
+    hipLaunchKernelGGL(eng_gather, dim3(grid_of(d.num_nodes, gather_lanes)), dim3(gather_lanes), 0, s, g, p_turned,
+                       p_wl2, p_dn2);
     AP_LAUNCHED();
     tend(AP_PROF_GATHER, s);
     mark(AP_PROF_CONTROL, s);
     tbegin(AP_PROF_CONTROL, s);
-    hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(EB), 0, s, segs(with_state, g_prev != nullptr, use_density),
+    hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(RED_LANES), 0, s, segs(with_state, g_prev != nullptr, use_density),
                        red);
     AP_LAUNCHED();
     Ctl ctl_pass = ctl;
@@ -2042,7 +2095,7 @@ __global__ __launch_bounds__(EB) void eng_gather_grouped(const PassTab* tab, con
     const GatherArgs g = T.gather;
     gather_body(g, T.p_turned, T.p_wl2, T.p_dn2, m.y, m.z);
 }
-__global__ __launch_bounds__(EB) void eng_reduce_grouped(const PassTab* tab, const int4* map) {
+__global__ __launch_bounds__(RED_LANES) void eng_reduce_grouped(const PassTab* tab, const int4* map) {
     const int4 m = map[blockIdx.x];
     if (stopped(tab, m.x)) return;
     const PassTab& T = tab[m.x];
@@ -2174,7 +2227,7 @@ int Batch::build() {
         add(ST_CONTROL, 1);
         add(ST_FINAL, L.nblk_n);
         add(ST_FLAGS, 1);
-        add(ST_WA, L.wa_plan.wave_blocks());
+        add(ST_WA, L.wa_plan.wave_blocks(WA_PLAN_PER_WAVE, WA_LANES));   // the grouped launch's shape
     }
     AP_HIP(hipMalloc(&d_tab, nb * sizeof(PassTab)));
     AP_HIP(hipMemcpy(d_tab, tab.data(), nb * sizeof(PassTab), hipMemcpyHostToDevice));
@@ -2227,7 +2280,7 @@ int Batch::pass(hipStream_t s) {
     AP_GROUPED(eng_bin_tiles_grouped, ST_TILES, TB, acc_lds);
     if ((st = fft.launch(d_fft, s))) return st;
     AP_GROUPED(eng_gather_grouped, ST_GATHER, EB, 0);
-    AP_GROUPED(eng_reduce_grouped, ST_REDUCE, EB, 0);
+    AP_GROUPED(eng_reduce_grouped, ST_REDUCE, RED_LANES, 0);
     AP_GROUPED(eng_control_grouped, ST_CONTROL, 64, 0);
     return AP_OK;
 }
@@ -2788,6 +2841,33 @@ int ap_level_set_tile_layout(ap_level* level, int force_side, int force_group, i
     return AP_OK;
 }
 
+int ap_level_launch_shape(ap_level* level, int* shape) {
+    AP_REQUIRE(level && shape, "level_launch_shape: null argument");
+    const Level& L = level->L;
+    shape[0] = L.wa_plan.per;
+    shape[1] = L.wa_plan.lanes;
+    shape[2] = L.gather_lanes;
+    return AP_OK;
+}
+
+int ap_level_set_launch_shape(ap_level* level, int wa_per, int wa_lanes, int gather_lanes) {
+    AP_REQUIRE(level, "level_set_launch_shape: null argument");
+    AP_REQUIRE(WaPlan::shape_ok(wa_per, wa_lanes) && gather_lanes >= 64 && gather_lanes <= 512 &&
+                   gather_lanes % 64 == 0,
+               "level_set_launch_shape: WA %d x %d lanes, gather %d lanes outside the kernels' shapes", wa_per,
+               wa_lanes, gather_lanes);
+    Level& L = level->L;
+    AP_HIP(hipDeviceSynchronize());
+    L.wa_plan.per = wa_per;
+    L.wa_plan.lanes = wa_lanes;
+    L.gather_lanes = gather_lanes;
+    for (GraphEntry& G : L.graphs) {                 // the launches change: capture again
+        if (L.last_ev == &G.ev) L.last_ev = nullptr;
+        G.clear();
+    }
+    return AP_OK;
+}
+
 int ap_level_set_bucket_room(ap_level* level, int percent, int extra) {
     AP_REQUIRE(level && percent >= 0 && percent <= 150 && extra >= 0 && extra <= 64,
                "level_set_bucket_room: percent %d, extra %d outside 0..150, 0..64", percent, extra);
@@ -2819,7 +2899,7 @@ int ap_batch_autotune(ap_batch* batch, int reps, int* layout) {
     Batch& B = batch->B;
     for (size_t b = 0; b < B.levels.size(); b++) {
         int four[4];
-        const int st = B.levels[b]->autotune(reps, B.stream, four);
+        const int st = B.levels[b]->autotune(reps, B.stream, four, false, false);
         if (st) return st;
         if (layout) {
             layout[2 * b] = four[0];

@@ -6,7 +6,8 @@
 
 namespace ap {
 
-constexpr int WA_PLAN_PER_WAVE = 2;   // wavefronts of the plan run by one launched wavefront
+constexpr int WA_PLAN_PER_WAVE = 2;   // wavefronts of the plan run by one launched wavefront (default)
+constexpr int WA_LANES = 256;         // lanes per workgroup of the WA launch (default)
 
 struct WaveArgs {
     const float2* pos;
@@ -28,6 +29,9 @@ int launch_wa_grouped(const WaveArgs* d_tab, const int4* d_map, int count, const
 
 struct WaPlan {
     int num_waves = 0, num_big = 0;
+    // the launch shape of launch(): plan wavefronts per launched wavefront (1, 2, 4) and lanes per workgroup (64 ..
+    // 512). Each plan wavefront writes its own partial, so the results do not depend on the shape.
+    int per = WA_PLAN_PER_WAVE, lanes = WA_LANES;
     unsigned char* d_lg = nullptr;   // per wavefront: log2 of its lanes per net
     int4* d_lanes = nullptr;         // per lane, in wavefront order: pin cell (-1: no pin), gradient slot (-1: none),
                                      // pin offset x, y (float bits)
@@ -37,8 +41,11 @@ struct WaPlan {
     int create(const int* d_net_start, int num_nets, const int* d_pin_cell, const float* d_pin_off,
                const int* d_pin_slot, int num_pins, bool with_nets = false);
     void destroy();
-    int wave_blocks() const { return (num_waves + 4 * WA_PLAN_PER_WAVE - 1) / (4 * WA_PLAN_PER_WAVE); }
-    int partial_count() const { return wave_blocks() + num_big; }   // (WA, HPWL) pairs written per launch
+    int wave_blocks(int p, int l) const { return (num_waves + (l / 64) * p - 1) / ((l / 64) * p); }
+    int wave_blocks() const { return wave_blocks(per, lanes); }
+    // (WA, HPWL) pairs written per launch: one per plan wavefront, then one per big net
+    int partial_count() const { return num_waves + num_big; }
+    static bool shape_ok(int p, int l) { return (p == 1 || p == 2 || p == 4) && l >= 64 && l <= 512 && l % 64 == 0; }
     // a.out set: also every net's (WA, HPWL) (a plan made with_nets)
     int launch(const WaArgs& a, double* partial, hipStream_t s) const;
     WaveArgs wave_args(const WaArgs& a, double* partial) const;

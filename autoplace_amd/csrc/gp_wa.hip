@@ -97,52 +97,60 @@ __device__ __forceinline__ void wa_group(const WaveArgs& a, float2 q, const int4
         a.out[net] = live ? make_float2((xpx - xnx) + (xpy - xny), (hx - lx) + (hy - ly)) : make_float2(0.f, 0.f);
 }
 
-// Each wavefront of the launch runs WA_PLAN_PER_WAVE consecutive wavefronts of the plan, their lane data and cell
-// positions loaded before the first is reduced (their memory waits overlap).
-template <bool OUT = false>
+// Each wavefront of the launch runs PER consecutive wavefronts of the plan, their lane data and cell positions loaded
+// before the first is reduced (their memory waits overlap). Each plan wavefront's (WA, HPWL) sums go to its own
+// partial (a wavefront sum in a fixed order), so the partials do not depend on PER or the workgroup size.
+template <bool OUT = false, int PER = WA_PLAN_PER_WAVE>
 __device__ __forceinline__ void wa_wave_body(const WaveArgs& a, int wg, int nwg) {
-    __shared__ double red[16 * 2];
     const int lane = threadIdx.x & 63;
-    const int w0 = (xcd_remap(wg, nwg) * 4 + (threadIdx.x >> 6)) * WA_PLAN_PER_WAVE;
+    const int w0 = (xcd_remap(wg, nwg) * (int)(blockDim.x >> 6) + (threadIdx.x >> 6)) * PER;
     const float2* pos = a.flag ? (*a.flag == 0 ? a.pos_other : a.pos) : a.pos;
-    double acc[2] = {0.0, 0.0};
-    int4 ln[WA_PLAN_PER_WAVE];
-    int lg[WA_PLAN_PER_WAVE], nt[WA_PLAN_PER_WAVE];
+    double acc[PER][2];
+    int4 ln[PER];
+    int lg[PER], nt[PER];
 #pragma unroll
-    for (int k = 0; k < WA_PLAN_PER_WAVE; k++) {
+    for (int k = 0; k < PER; k++) {
+        acc[k][0] = acc[k][1] = 0.0;
         const bool in = w0 + k < a.num_waves;       // uniform over the wavefront
         ln[k] = in ? a.lanes[(long long)(w0 + k) * 64 + lane] : make_int4(-1, -1, 0, 0);
         lg[k] = in ? a.lg[w0 + k] : 0;
         nt[k] = OUT && in ? a.lane_net[(long long)(w0 + k) * 64 + lane] : -1;
     }
-    float2 q[WA_PLAN_PER_WAVE];
+    float2 q[PER];
 #pragma unroll
-    for (int k = 0; k < WA_PLAN_PER_WAVE; k++)
+    for (int k = 0; k < PER; k++)
         q[k] = ln[k].x >= 0 && ln[k].x < a.num_mov ? pos[ln[k].x] : make_float2(0.f, 0.f);
 #pragma unroll
-    for (int k = 0; k < WA_PLAN_PER_WAVE; k++) {
+    for (int k = 0; k < PER; k++) {
         switch (lg[k]) {   // uniform over the wavefront
             case 0: break;
-            case 1: wa_group<1, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
-            case 2: wa_group<2, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
-            case 3: wa_group<3, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
-            case 4: wa_group<4, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
-            case 5: wa_group<5, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
-            default: wa_group<6, OUT>(a, q[k], ln[k], lane, acc, nt[k]); break;
+            case 1: wa_group<1, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
+            case 2: wa_group<2, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
+            case 3: wa_group<3, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
+            case 4: wa_group<4, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
+            case 5: wa_group<5, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
+            default: wa_group<6, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
         }
     }
-    if (OUT) return;                                 // per-net outputs only: no workgroup partials
-    block_sum<2>(acc, red);
-    if (threadIdx.x == 0) {
-        a.partial[2 * wg] = acc[0];
-        a.partial[2 * wg + 1] = acc[1];
+    if (OUT) return;                                 // per-net outputs only: no partials
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const double wa = wave_sum(acc[k][0]), hp = wave_sum(acc[k][1]);
+        if (lane == 0 && w0 + k < a.num_waves) {
+            a.partial[2 * (w0 + k)] = wa;
+            a.partial[2 * (w0 + k) + 1] = hp;
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void wa_wave_kernel(WaveArgs a) { wa_wave_body(a, blockIdx.x, gridDim.x); }
+template <int PER>
+__global__ __launch_bounds__(512) void wa_wave_kernel(WaveArgs a) {
+    wa_wave_body<false, PER>(a, blockIdx.x, gridDim.x);
+}
 // the drop-in's planned WA: also every net's (WA, HPWL)
-__global__ __launch_bounds__(256) void wa_wave_nets_kernel(WaveArgs a) {
-    wa_wave_body<true>(a, blockIdx.x, gridDim.x);
+template <int PER>
+__global__ __launch_bounds__(512) void wa_wave_nets_kernel(WaveArgs a) {
+    wa_wave_body<true, PER>(a, blockIdx.x, gridDim.x);
 }
 
 // many levels per launch: workgroup b does map[b] = (level, workgroup, workgroups of the level, 0); a level whose
@@ -152,12 +160,12 @@ __global__ __launch_bounds__(256) void wa_wave_grouped(const WaveArgs* tab, cons
     const int4 m = map[blockIdx.x];
     if (S[m.x][STOP] != 0) return;
     const WaveArgs a = tab[m.x];
-    wa_wave_body(a, m.y, m.z);
+    wa_wave_body<false, WA_PLAN_PER_WAVE>(a, m.y, m.z);   // 256-lane workgroups
 }
 
 int launch_wa_grouped(const WaveArgs* d_tab, const int4* d_map, int count, const double* const* d_S, hipStream_t s) {
     if (!count) return AP_OK;
-    hipLaunchKernelGGL(wa_wave_grouped, dim3(count), dim3(256), 0, s, d_tab, d_map, d_S);
+    hipLaunchKernelGGL(wa_wave_grouped, dim3(count), dim3(WA_LANES), 0, s, d_tab, d_map, d_S);
     AP_LAUNCHED();
     return AP_OK;
 }
@@ -398,17 +406,23 @@ void WaPlan::destroy() {
 int WaPlan::launch(const WaArgs& a, double* partial, hipStream_t s) const {
     if (num_waves) {
         const WaveArgs w = wave_args(a, partial);
+        AP_REQUIRE(shape_ok(per, lanes), "wa plan: launch shape %d x %d", per, lanes);
+        AP_REQUIRE(!a.out || d_net, "wa plan: per-net outputs need a plan made with its nets");
+        const dim3 g(wave_blocks()), b(lanes);
         if (a.out) {
-            AP_REQUIRE(d_net, "wa plan: per-net outputs need a plan made with its nets");
-            hipLaunchKernelGGL(wa_wave_nets_kernel, dim3(wave_blocks()), dim3(256), 0, s, w);
+            if (per == 1) hipLaunchKernelGGL(wa_wave_nets_kernel<1>, g, b, 0, s, w);
+            else if (per == 2) hipLaunchKernelGGL(wa_wave_nets_kernel<2>, g, b, 0, s, w);
+            else hipLaunchKernelGGL(wa_wave_nets_kernel<4>, g, b, 0, s, w);
         } else {
-            hipLaunchKernelGGL(wa_wave_kernel, dim3(wave_blocks()), dim3(256), 0, s, w);
+            if (per == 1) hipLaunchKernelGGL(wa_wave_kernel<1>, g, b, 0, s, w);
+            else if (per == 2) hipLaunchKernelGGL(wa_wave_kernel<2>, g, b, 0, s, w);
+            else hipLaunchKernelGGL(wa_wave_kernel<4>, g, b, 0, s, w);
         }
         AP_LAUNCHED();
     }
     if (num_big) {
         hipLaunchKernelGGL(wa_big_nets_kernel, dim3(num_big), dim3(WA_BLOCK), 0, s, a, d_big, num_big,
-                           partial ? partial + 2LL * wave_blocks() : nullptr);
+                           partial ? partial + 2LL * num_waves : nullptr);
         AP_LAUNCHED();
     }
     return AP_OK;

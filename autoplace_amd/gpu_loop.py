@@ -135,6 +135,17 @@ class Engine(object):
         _lib.check(_lib.lib().ap_level_set_tile_layout(self._h, int(force[0]), int(force[1]), int(overflow[0]),
                                                        int(overflow[1])), "level_set_tile_layout")
 
+    def launch_shape(self):
+        """(WA plan wavefronts per launched wavefront, WA lanes per workgroup, gather lanes per workgroup)."""
+        out = (ctypes.c_int * 3)()
+        _lib.check(_lib.lib().ap_level_launch_shape(self._h, ctypes.cast(out, ctypes.c_void_p)), "level_launch_shape")
+        return tuple(int(x) for x in out)
+
+    def set_launch_shape(self, wa_per, wa_lanes, gather_lanes):
+        """Set the WA and gather launch shapes (the autotuner's choice, replayed; results do not depend on them)."""
+        _lib.check(_lib.lib().ap_level_set_launch_shape(self._h, int(wa_per), int(wa_lanes), int(gather_lanes)),
+                   "level_set_launch_shape")
+
     def set_bucket_room(self, percent=150, extra=64):
         """Each density bucket's room per pass: percent % of its last count + extra (smaller: more entries spill)."""
         _lib.check(_lib.lib().ap_level_set_bucket_room(self._h, int(percent), int(extra)), "level_set_bucket_room")

@@ -1,7 +1,8 @@
 """Benchmark of the GP run on the 2M-cell synthetic design, one GPU.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config C] [--no-cpu] [--no-batch]
-                    [--save-snapshot F] [--snapshot F] [--copy-calibration] [--tile-layout F,G,J,H] [--dropin]
+                    [--save-snapshot F] [--snapshot F] [--copy-calibration] [--tile-layout F,G,J,H]
+                    [--launch-shape P,L,G] [--dropin]
 
 Workload: the 2,000,000-cell design (2,100,000 nets, ~8.2M pins, 16 fixed macros on 15 % of the die, generator
 seed 0, placer seed 0): force grid 2048 x 2048, overflow grid 512 x 512, default options.
@@ -32,7 +33,8 @@ calls of autoplace_amd.gp_cuda (its own gpu_loop.py on this library), and prints
 --snapshot F starts from a state written by --save-snapshot (no full run: value is then the pass rate): the form
 used under rocprofv3, together with --copy-calibration (three 256 MiB device copies first, the scale of the
 FETCH_SIZE / WRITE_SIZE counters) and --tile-layout F,G,J,H (the autotuner's choice of tile side and tiles per
-workgroup for the force and the overflow grid, replayed, so that its trial launches stay out of the profile).
+workgroup for the force and the overflow grid, replayed, so that its trial launches stay out of the profile), with
+--launch-shape P,L,G (its choice of the WA and gather launch shapes).
 """
 import argparse
 import json
@@ -168,7 +170,7 @@ def copy_calibration():
     torch.cuda.synchronize()
 
 
-def gpu_passes(snap, steps, warmup, layout=None):
+def gpu_passes(snap, steps, warmup, layout=None, shape=None):
     import torch
 
     from autoplace_amd import gpu_loop
@@ -181,10 +183,14 @@ def gpu_passes(snap, steps, warmup, layout=None):
     if layout:
         eng.set_tile_layout(layout[:2], layout[2:])  # a choice of the autotuner, replayed (profiling runs)
         lay = (tuple(layout[:2]), tuple(layout[2:]))
+        if shape:
+            eng.set_launch_shape(*shape)
     else:
         lay = eng.autotune(5)                        # as the flow does for levels of 100k nodes and more
-    print("density tile layout (%s): force grid side %d x %d per workgroup, overflow grid side %d x %d"
-          % ((("given" if layout else "autotuned"),) + lay[0] + lay[1]), file=sys.stderr, flush=True)
+    print("density tile layout (%s): force grid side %d x %d per workgroup, overflow grid side %d x %d; launch "
+          "shapes: WA %d plan wavefronts per wavefront, %d lanes; gather %d lanes"
+          % ((("given" if layout else "autotuned"),) + lay[0] + lay[1] + eng.launch_shape()), file=sys.stderr,
+          flush=True)
     # the warm-up: a quarter of the passes profiled (every class in order on one stream: the per-class breakdown),
     # the rest as graph runs with events around one single-kernel class each (the passes as the timed region runs
     # them, the two streams side by side); the roofline kernel is the one with the longest time in those runs
@@ -456,6 +462,8 @@ def main():
     p.add_argument("--copy-calibration", action="store_true", help="three 256 MiB device copies first")
     p.add_argument("--tile-layout", default="", help="F,G,J,H: force side, group, overflow side, group "
                                                        "(skip the autotuner)")
+    p.add_argument("--launch-shape", default="", help="P,L,G: WA plan wavefronts per wavefront and lanes, gather "
+                                                        "lanes (with --tile-layout)")
     p.add_argument("--dropin", action="store_true",
                    help="only the drop-in loop against the engine's passes on --config (default grids of B: "
                         "512 / 1024), one JSON line")
@@ -483,8 +491,9 @@ def main():
         import numpy as np
         np.savez(a.save_snapshot, **{k: snap[k].cpu().numpy() for k in ("u", "v", "g", "S", "kp", "c")})
     layout = tuple(int(x) for x in a.tile_layout.split(",")) if a.tile_layout else None
+    shape = tuple(int(x) for x in a.launch_shape.split(",")) if a.launch_shape else None
     elapsed, prof, dom, dom_ms, dims, method, stop, in_graph, in_graph_ms = gpu_passes(snap, a.steps, a.warmup,
-                                                                                       layout)
+                                                                                       layout, shape)
     log("per-pass kernel ms (warm-up, direct launches): " + ", ".join("%s %.3f" % kv for kv in prof.items()))
     log("roofline kernel %s (%s): %.4f ms per launch over the timed passes" % (dom, SINGLE_KERNEL[dom], dom_ms))
     by = algorithmic_bytes(dom, dims)

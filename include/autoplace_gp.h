@@ -242,14 +242,17 @@ AP_API int ap_level_objective(ap_level* level, const float* pos, float* grad, do
  * ap_level_objective): the top-left quadrant of the reference's density_map result. */
 AP_API int ap_level_density(ap_level* level, float* rho, void* stream);
 
-/* Autotuner of the density tile layout: the admissible sides from half to twice the set-up side (8 .. 64 bins) with
- * 1 or 2 tiles per workgroup of the tile kernel, for the force grid, then the overflow grid, then the force grid
- * again, timed on whole passes (reps each); the fastest is kept. flags 0: timed from the level's current state,
- * which is saved first and restored after every timing (no pass of the level runs). flags AP_AUTOTUNE_ONLINE: timed
- * on the level's own next passes, which advance it as ap_level_run's passes would (the control counts them): no extra
- * passes. layout i32[4] (optional): force side, force group, overflow side, overflow group (0, 0: no overflow grid).
- * The density map depends on neither, so the results are the same whatever the choice. Drops the level's captured
- * graphs. */
+/* Autotuner of the density tile layout and of the per-net and per-node kernels' launch shapes: the admissible tile
+ * sides from half to twice the set-up side (8 .. 64 bins) with 1 or 2 tiles per workgroup of the tile kernel, for
+ * the force grid, then the overflow grid, then the force grid again; then the WA kernel's plan wavefronts per
+ * launched wavefront (1, 2, 4) and lanes per workgroup (128, 256, 512), then the gather's lanes per workgroup (128,
+ * 256, 512); each timed on whole passes (reps each), the fastest kept. flags 0: timed from the level's current
+ * state, which is saved first and restored after every timing (no pass of the level runs). flags
+ * AP_AUTOTUNE_ONLINE: timed on the level's own next passes, which advance it as ap_level_run's passes would (the
+ * control counts them): no extra passes. layout i32[4] (optional): force side, force group, overflow side, overflow
+ * group (0, 0: no overflow grid); ap_level_launch_shape gives the shapes. The density map is an integer sum and the
+ * WA and gather partials are per wavefront, so the results are the same whatever the choice. Drops the level's
+ * captured graphs. */
 #define AP_AUTOTUNE_ONLINE 1
 AP_API int ap_level_autotune(ap_level* level, int reps, int flags, int* layout);
 /* Set the density tile layout directly (an admissible side and group of each grid, group 0: the default for the
@@ -257,6 +260,13 @@ AP_API int ap_level_autotune(ap_level* level, int reps, int flags, int* layout);
  * captured graphs. */
 AP_API int ap_level_set_tile_layout(ap_level* level, int force_side, int force_group, int overflow_side,
                                     int overflow_group);
+/* No counterpart in the reference. The launch shapes of the level's WA and gather kernels, shape i32[3]: WA plan
+ * wavefronts per launched wavefront, WA lanes per workgroup, gather lanes per workgroup (the autotuner's choice, or
+ * the defaults 2, 256, 256). set: those shapes directly (WA: 1, 2 or 4 and a multiple of 64 up to 512; gather: a
+ * multiple of 64 up to 512), the autotuner's choice replayed; the results do not depend on them. Drops the level's
+ * captured graphs. */
+AP_API int ap_level_launch_shape(ap_level* level, int* shape);
+AP_API int ap_level_set_launch_shape(ap_level* level, int wa_per, int wa_lanes, int gather_lanes);
 /* No counterpart in the reference. The room the density binning gives each tile's bucket in a pass: percent % of the
  * entries the bucket had in the pass before, plus extra (defaults 150 and 64; percent 0..150, extra 0..64). Entries
  * past a bucket's room take the slower spill path (global integer atomics), so a smaller room changes the time, never

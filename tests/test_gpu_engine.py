@@ -167,7 +167,7 @@ def test_engine_density_bit_exact(small_design, case):
 
 
 def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cls=None, profile=False, grids=None,
-         room=None, spills=None, every=1):
+         room=None, spills=None, every=1, shape=None):
     args, prob, wl, den, pos, S, kp, c = _setup(data, grids=grids)
     g0 = np.zeros_like(pos)
     eng, bufs = _engine(wl, den, pos, S, kp, c, g0, reorder=False, poisson=poisson)
@@ -180,6 +180,9 @@ def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cl
     eng, bufs = _engine(wl, den, pos, S, kp, c, g0, reorder=reorder, poisson=poisson)
     if room:
         eng.set_bucket_room(*room)
+    if shape:
+        eng.set_launch_shape(*shape)
+        assert eng.launch_shape() == tuple(shape)
     if time_cls:
         eng.time_every(every)
         eng.time_class(time_cls)
@@ -229,6 +232,19 @@ def test_engine_two_streams_equal_one_stream(small_design, grids):
         two, _ = _run(data, 16, use_graph, grids=grids)
         for x, y in zip(one, two):
             assert np.array_equal(x, y)
+
+
+def test_launch_shapes_change_nothing(small_design):
+    """The WA and gather launch shapes the autotuner chooses from (WA: 1, 2 or 4 plan wavefronts per launched
+    wavefront, 64 .. 512 lanes; gather: 64 .. 512 lanes) leave 16 passes bit for bit as they are: their partials are
+    per wavefront, summed in a fixed order."""
+    _, _, data = small_design
+    ref, _ = _run(data, 16, True, grids=(32, 64))
+    for shape in ((1, 128, 128), (4, 512, 512), (1, 64, 64), (4, 256, 128), (2, 512, 256)):
+        for use_graph in (True, False):
+            out, _ = _run(data, 16, use_graph, grids=(32, 64), shape=shape)
+            for x, y in zip(out, ref):
+                assert np.array_equal(x, y), shape
 
 
 @pytest.mark.parametrize("room", [(0, 0), (50, 0)])

@@ -69,8 +69,8 @@ def test_run_is_bit_for_bit_repeatable(design_a, runs):
 
 
 def test_autotuned_run_ends_where_the_plain_run_ends(design_a, runs):
-    """The tile-layout autotuner (forced on for both levels of this 10k-cell design) times whole passes from the
-    level's state and restores it: the run ends bit for bit where the run without it ends."""
+    """The autotuner (forced on for both levels of this 10k-cell design: tile layouts, WA and gather launch shapes)
+    times whole passes on the level's own passes: the run ends bit for bit where the run without it ends."""
     from autoplace_amd.problem import default_args
     from autoplace_amd.run_placement import place
     pos, m = place(design_a, default_args(seed=0), "cuda:0", opts={"autotune": True})
