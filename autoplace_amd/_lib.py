@@ -23,6 +23,8 @@ PROF_CLASSES = ("nesterov", "wa", "wa_cells", "density_bin", "density_tiles", "o
 _SIGNATURES = {
     "ap_last_error": ([], ctypes.c_char_p),
     "ap_version": ([], I),
+    "ap_cached_device_bytes": ([], ctypes.c_longlong),
+    "ap_release_cached_memory": ([], I),
     "ap_wa_wirelength": ([P, P, P, P, P, P, I, I, P, P, P, P, I, P], I),
     "ap_wa_plan_create": ([P, P, P, I, I, ctypes.POINTER(P)], I),
     "ap_wa_plan_destroy": ([P], I),

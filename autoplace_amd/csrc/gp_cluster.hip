@@ -12,7 +12,8 @@
 //           merged, self-loops dropped).
 // Weights and areas are integers and the scores one IEEE division each, so the clusters are those of the
 // reference's torch operators on any device, bit for bit. Each step is a thread per cluster or per edge; the
-// sorts and scans are rocPRIM's.
+// sorts and scans are rocPRIM's. The counts of a round (clusters and edges after it) stay on the device, read by the
+// kernels that follow; the host reads them once per round, at its end, to decide on the next.
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -77,23 +78,36 @@ __global__ __launch_bounds__(CLU_BLOCK) void clu_emit_pairs(const int* net_start
     }
 }
 
-// edges from the merged keys: row (cluster) and column of every edge
-__global__ __launch_bounds__(CLU_BLOCK) void clu_split_keys(const unsigned long long* key, const unsigned* num_edges,
-                                                            unsigned long long num, int* ei, int* ej) {
+// count[0]: edges, count[1]: clusters, on the device. The merged keys end with the self-loops' key num^2 when a
+// round made self-loops: that entry is not an edge.
+__global__ void clu_drop_self_loops(const unsigned long long* key, unsigned* count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned c = count[0];
+        const unsigned long long num = count[1];
+        if (c > 0 && key[c - 1] == num * num) count[0] = c - 1;
+    }
+}
+
+// edges from the merged keys: row (cluster), column and weight of every edge
+__global__ __launch_bounds__(CLU_BLOCK) void clu_split_keys(const unsigned long long* key, const int* weight,
+                                                            const unsigned* count, int* ei, int* ej, int* ew) {
     const unsigned e = blockIdx.x * CLU_BLOCK + threadIdx.x;
-    if (e >= *num_edges) return;
+    if (e >= count[0]) return;
+    const unsigned long long num = count[1];
     const unsigned long long k = key[e];
     ei[e] = (int)(k / num);
     ej[e] = (int)(k % num);
+    ew[e] = weight[e];
 }
 
 // row_start[r] = first edge of row r: a binary search of the sorted keys for r * num
-__global__ __launch_bounds__(CLU_BLOCK) void clu_row_starts(const unsigned long long* key, const unsigned* num_edges,
-                                                            int num, int* row_start) {
+__global__ __launch_bounds__(CLU_BLOCK) void clu_row_starts(const unsigned long long* key, const unsigned* count,
+                                                            int* row_start) {
     const int r = blockIdx.x * CLU_BLOCK + threadIdx.x;
+    const int num = (int)count[1];
     if (r > num) return;
     const unsigned long long target = (unsigned long long)r * (unsigned long long)num;
-    unsigned lo = 0, hi = *num_edges;
+    unsigned lo = 0, hi = count[0];
     while (lo < hi) {
         const unsigned mid = (lo + hi) >> 1;
         if (key[mid] < target) lo = mid + 1; else hi = mid;
@@ -134,6 +148,13 @@ __global__ __launch_bounds__(CLU_BLOCK) void clu_identity(int* parent, int num) 
     if (c < num) parent[c] = c;
 }
 
+// the cells' first clusters (one per cell), and count[1] = the clusters
+__global__ __launch_bounds__(CLU_BLOCK) void clu_start(int* cid, int n, unsigned* count) {
+    const int c = blockIdx.x * CLU_BLOCK + threadIdx.x;
+    if (c < n) cid[c] = c;
+    if (c == 0) count[1] = (unsigned)n;
+}
+
 // every head takes the PER_HEAD best of the clusters that picked it: score descending, then the lowest tail index
 // (the row's columns ascend, and an entry only passes the ones strictly below it)
 template <int PER_HEAD>
@@ -168,9 +189,18 @@ __global__ __launch_bounds__(CLU_BLOCK) void clu_take(const int* row_start, cons
         if (ts[k] >= 0) parent[ts[k]] = h;
 }
 
-__global__ __launch_bounds__(CLU_BLOCK) void clu_kept(const int* parent, int num, int* kept) {
+// the kept clusters (their own parents), and the merged areas' sums cleared (no more clusters after the round)
+__global__ __launch_bounds__(CLU_BLOCK) void clu_kept(const int* parent, int num, int* kept, long long* area_out) {
     const int c = blockIdx.x * CLU_BLOCK + threadIdx.x;
-    if (c < num) kept[c] = parent[c] == c;
+    if (c < num) {
+        kept[c] = parent[c] == c;
+        area_out[c] = 0;
+    }
+}
+
+// count[1] = the clusters after the round: the rank of the last cluster plus its own flag
+__global__ void clu_count_kept(const int* rank, const int* kept, int num, unsigned* count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) count[1] = (unsigned)(rank[num - 1] + kept[num - 1]);
 }
 
 // new index of every old cluster: the rank of its parent among the kept ones
@@ -184,11 +214,6 @@ __global__ __launch_bounds__(CLU_BLOCK) void clu_relabel_cells(int* cid, int n, 
     if (x < n) cid[x] = new_id[cid[x]];
 }
 
-__global__ __launch_bounds__(CLU_BLOCK) void clu_zero_i64(long long* a, int num) {
-    const int c = blockIdx.x * CLU_BLOCK + threadIdx.x;
-    if (c < num) a[c] = 0;
-}
-
 // integer sums: the same whatever the order of the atomics
 __global__ __launch_bounds__(CLU_BLOCK) void clu_merge_area(const long long* area, int num, const int* new_id,
                                                             long long* area_out) {
@@ -197,13 +222,14 @@ __global__ __launch_bounds__(CLU_BLOCK) void clu_merge_area(const long long* are
         atomicAdd((unsigned long long*)&area_out[new_id[c]], (unsigned long long)area[c]);
 }
 
-// the edges between new clusters; a self-loop gets the key one past every valid one, so it sorts last
+// the edges between new clusters (count[1] of them); a self-loop gets the key one past every valid one, so it sorts
+// last
 __global__ __launch_bounds__(CLU_BLOCK) void clu_relabel_edges(const int* ei, const int* ej, const int* ew,
-                                                               const unsigned* num_edges, const int* new_id,
-                                                               unsigned long long num_new, unsigned long long* key,
-                                                               int* weight) {
+                                                               const unsigned* count, const int* new_id,
+                                                               unsigned long long* key, int* weight) {
     const unsigned e = blockIdx.x * CLU_BLOCK + threadIdx.x;
-    if (e >= *num_edges) return;
+    if (e >= count[0]) return;
+    const unsigned long long num_new = count[1];
     const unsigned long long a = (unsigned long long)new_id[ei[e]], b = (unsigned long long)new_id[ej[e]];
     key[e] = a == b ? num_new * num_new : a * num_new + b;
     weight[e] = ew[e];
@@ -308,7 +334,8 @@ struct Cluster {
     int *cid = nullptr, *parent = nullptr, *kept = nullptr, *rank = nullptr, *new_id = nullptr, *best_to = nullptr;
     double* best_score = nullptr;
     long long *area = nullptr, *area_alt = nullptr;
-    unsigned *dev_count = nullptr;      // [0]: edges after a merge, [1]: kept clusters (rank of the last + its flag)
+    unsigned *dev_count = nullptr;      // [0]: edges, [1]: clusters (both on the device, ahead of the host's copies)
+    unsigned* host_count = nullptr;     // pinned: dev_count read back at the end of a round
     void* temp = nullptr;
     size_t temp_bytes = 0;
     std::vector<void*> mem;
@@ -316,19 +343,39 @@ struct Cluster {
 
     template <typename T>
     int alloc(T** p, unsigned long long count) {
-        AP_HIP(hipMalloc((void**)p, (size_t)(count > 0 ? count : 1) * sizeof(T)));
+        AP_HIP(dev_malloc((void**)p, (size_t)(count > 0 ? count : 1) * sizeof(T)));
         mem.push_back((void*)*p);
         return AP_OK;
     }
     void destroy() {
-        for (void* p : mem) (void)hipFree(p);
+        (void)hipDeviceSynchronize();      // then every buffer below is idle
+        for (void* p : mem) dev_free_synced(p);
         mem.clear();
-        if (temp) (void)hipFree(temp);
+        dev_free_synced(temp);
         temp = nullptr;
         if (heads.pinned) (void)hipHostFree(heads.pinned);
         heads.pinned = nullptr;
+        if (host_count) (void)hipHostFree(host_count);
+        host_count = nullptr;
     }
-    int sort_and_merge(unsigned long long entries, unsigned long long num_keys, hipStream_t s);
+    int temp_for(size_t need) {
+        if (need > temp_bytes) {
+            dev_free(temp);
+            temp = nullptr;
+            AP_HIP(dev_malloc(&temp, need));
+            temp_bytes = need;
+        }
+        return AP_OK;
+    }
+    // the counts on the device, read back: the host's num_edges and num
+    int read_counts(hipStream_t s) {
+        AP_HIP(hipMemcpyAsync(host_count, dev_count, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        AP_HIP(hipStreamSynchronize(s));
+        num_edges = host_count[0];
+        num = (int)host_count[1];
+        return AP_OK;
+    }
+    int sort_and_merge(unsigned long long entries, unsigned long long max_keys, hipStream_t s);
     int create(int cells, const int* pin_cell, const int* net_start, int num_nets, const long long* area_in,
                int max_degree, hipStream_t s);
     // while_busy: host work run once the round's first kernels are queued, before the host waits for them
@@ -336,51 +383,34 @@ struct Cluster {
               HeadStream* while_busy = nullptr);
 };
 
-// sort `entries` (key, weight) pairs by key, sum equal keys into the edge list, drop the self-loop key num_keys^2
-// when present, and rebuild the rows (num_keys clusters)
-int Cluster::sort_and_merge(unsigned long long entries, unsigned long long num_keys, hipStream_t s) {
-    const unsigned end_bit = bits_for(num_keys * num_keys);
-    size_t need = 0;
-    AP_HIP(rocprim::radix_sort_pairs(nullptr, need, key, key_alt, weight, weight_alt, (size_t)entries, 0, end_bit, s));
-    size_t need2 = 0;
-    AP_HIP(rocprim::reduce_by_key(nullptr, need2, key_alt, weight_alt, (size_t)entries, key, weight, dev_count,
-                                  rocprim::plus<int>(), rocprim::equal_to<unsigned long long>(), s));
-    need = need > need2 ? need : need2;
-    if (need > temp_bytes) {
-        if (temp) AP_HIP(hipFree(temp));
-        temp = nullptr;
-        AP_HIP(hipMalloc(&temp, need));
-        temp_bytes = need;
-    }
+// sort `entries` (key, weight) pairs by key, sum equal keys into the edge list, drop the self-loop key num^2 when
+// present (num = count[1], the clusters, at most max_keys) and rebuild the rows; no wait for the device
+int Cluster::sort_and_merge(unsigned long long entries, unsigned long long max_keys, hipStream_t s) {
+    const unsigned end_bit = bits_for(max_keys * max_keys);
     if (entries > 0) {
+        size_t need = 0, need2 = 0;
+        AP_HIP(rocprim::radix_sort_pairs(nullptr, need, key, key_alt, weight, weight_alt, (size_t)entries, 0, end_bit,
+                                         s));
+        AP_HIP(rocprim::reduce_by_key(nullptr, need2, key_alt, weight_alt, (size_t)entries, key, weight, dev_count,
+                                      rocprim::plus<int>(), rocprim::equal_to<unsigned long long>(), s));
+        int st = temp_for(need > need2 ? need : need2);
+        if (st) return st;
         size_t b = temp_bytes;
         AP_HIP(rocprim::radix_sort_pairs(temp, b, key, key_alt, weight, weight_alt, (size_t)entries, 0, end_bit, s));
         b = temp_bytes;
         AP_HIP(rocprim::reduce_by_key(temp, b, key_alt, weight_alt, (size_t)entries, key, weight, dev_count,
                                       rocprim::plus<int>(), rocprim::equal_to<unsigned long long>(), s));
-        unsigned count = 0;
-        AP_HIP(hipMemcpyAsync(&count, dev_count, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        AP_HIP(hipStreamSynchronize(s));
-        if (count > 0) {
-            unsigned long long last = 0;
-            AP_HIP(hipMemcpyAsync(&last, key + count - 1, sizeof(last), hipMemcpyDeviceToHost, s));
-            AP_HIP(hipStreamSynchronize(s));
-            if (last == num_keys * num_keys) count--;            // the self-loops' key
-        }
-        num_edges = count;
+        hipLaunchKernelGGL(clu_drop_self_loops, dim3(1), dim3(64), 0, s, key, dev_count);
+        AP_LAUNCHED();
+        hipLaunchKernelGGL(clu_split_keys, dim3(grid_of((long long)entries, CLU_BLOCK)), dim3(CLU_BLOCK), 0, s, key,
+                           weight, dev_count, ei, ej, ew);
+        AP_LAUNCHED();
     } else {
-        num_edges = 0;
+        AP_HIP(hipMemsetAsync(dev_count, 0, sizeof(unsigned), s));
     }
-    AP_HIP(hipMemcpyAsync(dev_count, &num_edges, sizeof(unsigned), hipMemcpyHostToDevice, s));
-    if (num_edges)
-        hipLaunchKernelGGL(clu_split_keys, dim3(grid_of(num_edges, CLU_BLOCK)), dim3(CLU_BLOCK), 0, s, key, dev_count,
-                           num_keys, ei, ej);
+    hipLaunchKernelGGL(clu_row_starts, dim3(grid_of((long long)max_keys + 1, CLU_BLOCK)), dim3(CLU_BLOCK), 0, s, key,
+                       dev_count, row_start);
     AP_LAUNCHED();
-    AP_HIP(hipMemcpyAsync(ew, weight, (size_t)num_edges * sizeof(int), hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(clu_row_starts, dim3(grid_of((long long)num_keys + 1, CLU_BLOCK)), dim3(CLU_BLOCK), 0, s, key,
-                       dev_count, (int)num_keys, row_start);
-    AP_LAUNCHED();
-    AP_HIP(hipStreamSynchronize(s));                              // the host copy of num_edges above is freed now
     return AP_OK;
 }
 
@@ -390,6 +420,7 @@ int Cluster::create(int cells, const int* pin_cell, const int* net_start, int nu
     int st;
     unsigned* count = nullptr;
     unsigned* offset = nullptr;
+    AP_HIP(hipHostMalloc((void**)&host_count, 2 * sizeof(unsigned), hipHostMallocDefault));
     if ((st = alloc(&count, (unsigned long long)num_nets + 1)) || (st = alloc(&offset, (unsigned long long)num_nets + 1)))
         return st;
     if (num_nets > 0) {
@@ -401,18 +432,12 @@ int Cluster::create(int cells, const int* pin_cell, const int* net_start, int nu
     size_t need = 0;
     AP_HIP(rocprim::exclusive_scan(nullptr, need, count, offset, 0u, (size_t)num_nets + 1, rocprim::plus<unsigned>(),
                                    s));
-    void* scan_temp = nullptr;
-    AP_HIP(hipMalloc(&scan_temp, need > 0 ? need : 1));
-    const hipError_t e = rocprim::exclusive_scan(scan_temp, need, count, offset, 0u, (size_t)num_nets + 1,
-                                                 rocprim::plus<unsigned>(), s);
-    unsigned total = 0;
-    const hipError_t e2 = e == hipSuccess ? hipMemcpyAsync(&total, offset + num_nets, sizeof(unsigned),
-                                                           hipMemcpyDeviceToHost, s) : e;
-    const hipError_t e3 = hipStreamSynchronize(s);
-    (void)hipFree(scan_temp);
-    AP_HIP(e);
-    AP_HIP(e2);
-    AP_HIP(e3);
+    if ((st = temp_for(need > 0 ? need : 1))) return st;
+    size_t b = temp_bytes;
+    AP_HIP(rocprim::exclusive_scan(temp, b, count, offset, 0u, (size_t)num_nets + 1, rocprim::plus<unsigned>(), s));
+    AP_HIP(hipMemcpyAsync(host_count, offset + num_nets, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    AP_HIP(hipStreamSynchronize(s));
+    const unsigned total = host_count[0];
     cap_edges = total;
     if ((st = alloc(&key, cap_edges)) || (st = alloc(&key_alt, cap_edges)) || (st = alloc(&weight, cap_edges)) ||
         (st = alloc(&weight_alt, cap_edges)) || (st = alloc(&ei, cap_edges)) || (st = alloc(&ej, cap_edges)) ||
@@ -426,12 +451,16 @@ int Cluster::create(int cells, const int* pin_cell, const int* net_start, int nu
                            num_nets, pin_cell, n, max_degree, offset, key, weight);
         AP_LAUNCHED();
     }
-    hipLaunchKernelGGL(clu_identity, dim3(grid_of(n, CLU_BLOCK)), dim3(CLU_BLOCK), 0, s, cid, n);
+    hipLaunchKernelGGL(clu_start, dim3(grid_of(n, CLU_BLOCK)), dim3(CLU_BLOCK), 0, s, cid, n, dev_count);
     AP_LAUNCHED();
     AP_HIP(hipMemcpyAsync(area, area_in, (size_t)n * sizeof(long long), hipMemcpyDeviceToDevice, s));
-    return sort_and_merge(total, (unsigned long long)n, s);
+    if ((st = sort_and_merge(total, (unsigned long long)n, s))) return st;
+    return read_counts(s);
 }
 
+// The round's steps run over the clusters before it (num, known to the host) and read the clusters after it from
+// count[1] on the device; when nothing merges, every cluster keeps its index and the edges their keys, so the state
+// is rebuilt unchanged. One wait, at the end, for the counts.
 int Cluster::round(const unsigned char* head, float cap, long long big, int per_head, hipStream_t s,
                    HeadStream* while_busy) {
     const dim3 g(grid_of(num, CLU_BLOCK)), b(CLU_BLOCK);
@@ -446,30 +475,19 @@ int Cluster::round(const unsigned char* head, float cap, long long big, int per_
         default: hipLaunchKernelGGL(clu_take<4>, g, b, 0, s, row_start, ej, num, head, best_to, best_score, parent);
     }
     AP_LAUNCHED();
-    hipLaunchKernelGGL(clu_kept, g, b, 0, s, parent, num, kept);
+    hipLaunchKernelGGL(clu_kept, g, b, 0, s, parent, num, kept, area_alt);
     AP_LAUNCHED();
     size_t need = 0;
     AP_HIP(rocprim::exclusive_scan(nullptr, need, kept, rank, 0, (size_t)num, rocprim::plus<int>(), s));
-    if (need > temp_bytes) {
-        if (temp) AP_HIP(hipFree(temp));
-        temp = nullptr;
-        AP_HIP(hipMalloc(&temp, need));
-        temp_bytes = need;
-    }
+    int st = temp_for(need);
+    if (st) return st;
     size_t tb = temp_bytes;
     AP_HIP(rocprim::exclusive_scan(temp, tb, kept, rank, 0, (size_t)num, rocprim::plus<int>(), s));
-    int last_rank = 0, last_kept = 0;
-    AP_HIP(hipMemcpyAsync(&last_rank, rank + num - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    AP_HIP(hipMemcpyAsync(&last_kept, kept + num - 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    if (while_busy) while_busy->prepare((size_t)num);         // the next round's heads, beside this round
-    AP_HIP(hipStreamSynchronize(s));
-    const int num_new = last_rank + last_kept;
-    if (num_new == num) return AP_OK;                             // nothing merged: the state stays
+    hipLaunchKernelGGL(clu_count_kept, dim3(1), dim3(64), 0, s, rank, kept, num, dev_count);
+    AP_LAUNCHED();
     hipLaunchKernelGGL(clu_new_index, g, b, 0, s, parent, rank, num, new_id);
     AP_LAUNCHED();
     hipLaunchKernelGGL(clu_relabel_cells, dim3(grid_of(n, CLU_BLOCK)), b, 0, s, cid, n, new_id);
-    AP_LAUNCHED();
-    hipLaunchKernelGGL(clu_zero_i64, dim3(grid_of(num_new, CLU_BLOCK)), b, 0, s, area_alt, num_new);
     AP_LAUNCHED();
     hipLaunchKernelGGL(clu_merge_area, g, b, 0, s, area, num, new_id, area_alt);
     AP_LAUNCHED();
@@ -477,10 +495,11 @@ int Cluster::round(const unsigned char* head, float cap, long long big, int per_
     const unsigned entries = num_edges;
     if (entries)
         hipLaunchKernelGGL(clu_relabel_edges, dim3(grid_of(entries, CLU_BLOCK)), b, 0, s, ei, ej, ew, dev_count,
-                           new_id, (unsigned long long)num_new, key, weight);
+                           new_id, key, weight);
     AP_LAUNCHED();
-    num = num_new;
-    return sort_and_merge(entries, (unsigned long long)num_new, s);
+    if ((st = sort_and_merge(entries, (unsigned long long)num, s))) return st;
+    if (while_busy) while_busy->prepare((size_t)num);         // the next round's heads, beside this round
+    return read_counts(s);
 }
 
 }  // namespace ap

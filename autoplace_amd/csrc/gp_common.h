@@ -30,6 +30,17 @@ enum { C_BASE_GAMMA, C_MOTION_UNIT, C_TOTAL_PINS, C_TOTAL_AREA, C_PITCH, C_MIN_I
 
 void set_error(const char* fmt, ...);
 
+// Device buffers of the library (gp_devmem.hip): dev_malloc as hipMalloc, from a cache of released blocks of the same
+// size class on the current device; dev_free synchronises the device (as hipFree does) and keeps the block for the
+// next request of its class; dev_free_synced is for callers that have just synchronised the device. Null is ignored.
+hipError_t dev_malloc(void** p, size_t bytes);
+void dev_free(void* p);
+void dev_free_synced(void* p);
+template <typename T>
+hipError_t dev_malloc(T** p, size_t bytes) {
+    return dev_malloc((void**)p, bytes);
+}
+
 #define AP_HIP(call)                                                                                   \
     do {                                                                                               \
         hipError_t e_ = (call);                                                                        \

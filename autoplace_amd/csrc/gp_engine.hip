@@ -1324,7 +1324,7 @@ struct Level {
 
     template <typename T>
     int alloc(T** p, long long n) {
-        AP_HIP(hipMalloc((void**)p, (size_t)(n > 0 ? n : 1) * sizeof(T)));
+        AP_HIP(dev_malloc((void**)p, (size_t)(n > 0 ? n : 1) * sizeof(T)));
         bin_mem.push_back((void*)*p);
         return AP_OK;
     }
@@ -1376,7 +1376,7 @@ int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nod
     unsigned long long over[4] = {0, 0, 0, 0};
     {
         unsigned long long* d_over = nullptr;
-        AP_HIP(hipMalloc(&d_over, sizeof(over)));
+        AP_HIP(dev_malloc(&d_over, sizeof(over)));
         hipError_t e = hipMemset(d_over, 0, sizeof(over));
         if (e == hipSuccess && num_nodes > 0) {
             hipLaunchKernelGGL(eng_reach_counts, dim3(grid_of(num_nodes, 256)), dim3(256), 0, nullptr, J.size,
@@ -1384,7 +1384,7 @@ int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nod
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipMemcpy(over, d_over, sizeof(over), hipMemcpyDeviceToHost);
-        (void)hipFree(d_over);
+        (void)dev_free(d_over);
         AP_HIP(e);
     }
     J.T = J.T0 = tile_side(num_nodes, over, nbx, nby, planes);
@@ -1571,9 +1571,9 @@ int Level::autotune(int reps, hipStream_t s, int* sides, bool online, bool shape
     int st;
     char* keep = nullptr;
     if (!online) {
-        AP_HIP(hipMalloc(&keep, state_bytes()));
+        AP_HIP(dev_malloc(&keep, state_bytes()));
         if ((st = state_copy(keep, false, s))) {
-            (void)hipFree(keep);
+            (void)dev_free(keep);
             return st;
         }
     }
@@ -1651,7 +1651,7 @@ int Level::autotune(int reps, hipStream_t s, int* sides, bool online, bool shape
     // offline: the state the timing passes changed, back; then the keep buffer goes
     if (!st && keep) st = state_copy(keep, true, s);
     (void)hipStreamSynchronize(s);
-    if (keep) (void)hipFree(keep);
+    if (keep) (void)dev_free(keep);
     if (st) return st;
     // the timing runs leave the fill form's counts behind: back to zero, the state the counting-sort form (the
     // batch's levels) starts from; the fill form counts again at the new layout anyway (rooms_stale)
@@ -1714,12 +1714,12 @@ int Level::create(const ap_level_desc& desc) {
     }
     trace("poisson plan");
     const long long N = d.num_nodes, nb = (long long)d.nbx * d.nby;
-    AP_HIP(hipMalloc(&u1, N * sizeof(float2)));
-    AP_HIP(hipMalloc(&v1, N * sizeof(float2)));
-    AP_HIP(hipMalloc(&g1, N * sizeof(float2)));
-    AP_HIP(hipMalloc(&pin_grad, (d.num_pins + 1) * sizeof(float2)));
-    AP_HIP(hipMalloc(&cell_wl, ((size_t)d.num_mov + 1) * sizeof(float2)));
-    AP_HIP(hipMalloc(&pin_slot, (d.num_pins + 1) * sizeof(int)));
+    AP_HIP(dev_malloc(&u1, N * sizeof(float2)));
+    AP_HIP(dev_malloc(&v1, N * sizeof(float2)));
+    AP_HIP(dev_malloc(&g1, N * sizeof(float2)));
+    AP_HIP(dev_malloc(&pin_grad, (d.num_pins + 1) * sizeof(float2)));
+    AP_HIP(dev_malloc(&cell_wl, ((size_t)d.num_mov + 1) * sizeof(float2)));
+    AP_HIP(dev_malloc(&pin_slot, (d.num_pins + 1) * sizeof(int)));
     if (d.num_pins > 0) {
         hipLaunchKernelGGL(eng_pin_slot_clear, dim3(grid_of(d.num_pins, EB)), dim3(EB), 0, nullptr, pin_slot,
                            d.num_pins);
@@ -1728,12 +1728,12 @@ int Level::create(const ap_level_desc& desc) {
                            d.cell_start, d.cell_pin, d.num_mov, d.num_pins);
         AP_LAUNCHED();
     }
-    AP_HIP(hipMalloc(&rho, nb * sizeof(float)));
+    AP_HIP(dev_malloc(&rho, nb * sizeof(float)));
     if (poisson.method == AP_POISSON_FFT) {        // the field as (x, y) pairs: one load per bin in the gather
-        AP_HIP(hipMalloc(&f2, nb * sizeof(float2)));
+        AP_HIP(dev_malloc(&f2, nb * sizeof(float2)));
     } else {
-        AP_HIP(hipMalloc(&fx, nb * sizeof(float)));
-        AP_HIP(hipMalloc(&fy, nb * sizeof(float)));
+        AP_HIP(dev_malloc(&fx, nb * sizeof(float)));
+        AP_HIP(dev_malloc(&fy, nb * sizeof(float)));
     }
     nblk_n = grid_of(N, EB);
     wav_n = grid_of(N, 64);
@@ -1754,7 +1754,7 @@ int Level::create(const ap_level_desc& desc) {
     // most BIN_MAX_BUCKETS tiles)
     const long long n_over = 2LL * BIN_MAX_BUCKETS;
     const long long np = nblk_n + 2LL * nblk_wa + n_over + 3LL * wav_n;
-    AP_HIP(hipMalloc(&part, np * sizeof(double)));
+    AP_HIP(dev_malloc(&part, np * sizeof(double)));
     AP_HIP(hipMemset(part, 0, np * sizeof(double)));
     p_moved = part;
     p_wa = p_moved + nblk_n;
@@ -1762,12 +1762,12 @@ int Level::create(const ap_level_desc& desc) {
     p_turned = p_over + n_over;
     p_wl2 = p_turned + wav_n;
     p_dn2 = p_wl2 + wav_n;
-    AP_HIP(hipMalloc(&stats, 8 * sizeof(double)));
-    AP_HIP(hipMalloc(&red, RED_BLOCKS * RED_VALS * sizeof(double)));
-    AP_HIP(hipMalloc(&ctl_x, NUM_X * sizeof(double)));
+    AP_HIP(dev_malloc(&stats, 8 * sizeof(double)));
+    AP_HIP(dev_malloc(&red, RED_BLOCKS * RED_VALS * sizeof(double)));
+    AP_HIP(dev_malloc(&ctl_x, NUM_X * sizeof(double)));
     AP_HIP(hipMemset(ctl_x, 0, NUM_X * sizeof(double)));
     ctl.X = ctl_x;
-    AP_HIP(hipMalloc(&flags, 4 * sizeof(int)));
+    AP_HIP(dev_malloc(&flags, 4 * sizeof(int)));
     AP_HIP(hipMemset(flags, 0, 4 * sizeof(int)));
     AP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     AP_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
@@ -1785,6 +1785,7 @@ int Level::create(const ap_level_desc& desc) {
 }
 
 void Level::destroy() {
+    (void)hipDeviceSynchronize();          // then every buffer below is idle
     for (GraphEntry& G : graphs) G.clear();
     for (hipEvent_t e : direct_ev) (void)hipEventDestroy(e);
     direct_ev.clear();
@@ -1793,9 +1794,8 @@ void Level::destroy() {
         if (ev) (void)hipEventDestroy(ev);
     void* ptrs[] = {u1, v1, g1, pin_grad, cell_wl, rho, fx, fy, f2, part, stats, flags, red, ctl_x, block_net,
                     pin_slot, start_st, start_hist};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    for (void* p : bin_mem) (void)hipFree(p);
+    for (void* p : ptrs) dev_free_synced(p);
+    for (void* p : bin_mem) dev_free_synced(p);
     bin_mem.clear();
     if (!poisson_borrowed) poisson.destroy();
     poisson = Poisson{};
@@ -2157,13 +2157,13 @@ struct Batch {
 void Batch::release_tables() {
     void* ptrs[] = {d_tab, d_wave, d_fft, d_S};
     for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+        if (p) (void)dev_free(p);
     d_tab = nullptr;
     d_wave = nullptr;
     d_fft = nullptr;
     d_S = nullptr;
     for (int k = 0; k < NUM_ST; k++) {
-        if (map[k]) (void)hipFree(map[k]);
+        if (map[k]) (void)dev_free(map[k]);
         map[k] = nullptr;
         count[k] = 0;
     }
@@ -2229,18 +2229,18 @@ int Batch::build() {
         add(ST_FLAGS, 1);
         add(ST_WA, L.wa_plan.wave_blocks(WA_PLAN_PER_WAVE, WA_LANES));   // the grouped launch's shape
     }
-    AP_HIP(hipMalloc(&d_tab, nb * sizeof(PassTab)));
+    AP_HIP(dev_malloc(&d_tab, nb * sizeof(PassTab)));
     AP_HIP(hipMemcpy(d_tab, tab.data(), nb * sizeof(PassTab), hipMemcpyHostToDevice));
-    AP_HIP(hipMalloc(&d_wave, nb * sizeof(WaveArgs)));
+    AP_HIP(dev_malloc(&d_wave, nb * sizeof(WaveArgs)));
     AP_HIP(hipMemcpy(d_wave, wave.data(), nb * sizeof(WaveArgs), hipMemcpyHostToDevice));
-    AP_HIP(hipMalloc(&d_fft, nb * sizeof(FftTab)));
+    AP_HIP(dev_malloc(&d_fft, nb * sizeof(FftTab)));
     AP_HIP(hipMemcpy(d_fft, ffts.data(), nb * sizeof(FftTab), hipMemcpyHostToDevice));
-    AP_HIP(hipMalloc(&d_S, nb * sizeof(double*)));
+    AP_HIP(dev_malloc(&d_S, nb * sizeof(double*)));
     AP_HIP(hipMemcpy(d_S, S.data(), nb * sizeof(double*), hipMemcpyHostToDevice));
     for (int k = 0; k < NUM_ST; k++) {
         count[k] = (int)m[k].size();
         if (!count[k]) continue;
-        AP_HIP(hipMalloc(&map[k], m[k].size() * sizeof(int4)));
+        AP_HIP(dev_malloc(&map[k], m[k].size() * sizeof(int4)));
         AP_HIP(hipMemcpy(map[k], m[k].data(), m[k].size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     int st = fft.create(ffts);
@@ -2520,12 +2520,12 @@ int ap_level_start_begin(ap_level* level, double tol, int window, int max_passes
                "level_start_begin: tol %g, window %d, max_passes %d", tol, window, max_passes);
     Level& L = level->L;
     AP_HIP(hipStreamSynchronize(L.stream));
-    if (!L.start_st) AP_HIP(hipMalloc((void**)&L.start_st, START_STATE * sizeof(double)));
+    if (!L.start_st) AP_HIP(dev_malloc((void**)&L.start_st, START_STATE * sizeof(double)));
     if (L.start_cap < max_passes) {
-        if (L.start_hist) AP_HIP(hipFree(L.start_hist));
+        dev_free(L.start_hist);
         L.start_hist = nullptr;
         L.start_cap = 0;
-        AP_HIP(hipMalloc((void**)&L.start_hist, 2 * (size_t)max_passes * sizeof(double)));
+        AP_HIP(dev_malloc((void**)&L.start_hist, 2 * (size_t)max_passes * sizeof(double)));
         L.start_cap = max_passes;
     }
     const double st[START_STATE] = {0.0, 0.0, tol, (double)window, (double)max_passes, 0.0, 0.0, 0.0};

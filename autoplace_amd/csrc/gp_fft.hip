@@ -500,7 +500,7 @@ int FftGroup::create(const std::vector<FftTab>& tabs) {
     for (int k = 0; k < 5; k++) {
         count[k] = (int)m[k].size();
         if (!count[k]) continue;
-        AP_HIP(hipMalloc(&map[k], m[k].size() * sizeof(int4)));
+        AP_HIP(dev_malloc(&map[k], m[k].size() * sizeof(int4)));
         AP_HIP(hipMemcpy(map[k], m[k].data(), m[k].size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     return AP_OK;
@@ -508,7 +508,7 @@ int FftGroup::create(const std::vector<FftTab>& tabs) {
 
 void FftGroup::destroy() {
     for (int k = 0; k < 5; k++) {
-        if (map[k]) (void)hipFree(map[k]);
+        if (map[k]) (void)dev_free(map[k]);
         map[k] = nullptr;
         count[k] = 0;
     }
@@ -589,17 +589,17 @@ int Poisson::fft_create() {
     std::vector<float> om(N), nu(M);
     for (int k = 0; k < N; k++) om[k] = (float)(M_PI * k / (N * bx));
     for (int l = 0; l < M; l++) nu[l] = (float)(M_PI * l / (M * by));
-    AP_HIP(hipMalloc(&d_tw, h.size() * sizeof(float2)));
+    AP_HIP(dev_malloc(&d_tw, h.size() * sizeof(float2)));
     AP_HIP(hipMemcpy(d_tw, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
-    AP_HIP(hipMalloc(&d_omega, (N + M) * sizeof(float)));
+    AP_HIP(dev_malloc(&d_omega, (N + M) * sizeof(float)));
     AP_HIP(hipMemcpy(d_omega, om.data(), N * sizeof(float), hipMemcpyHostToDevice));
     AP_HIP(hipMemcpy(d_omega + N, nu.data(), M * sizeof(float), hipMemcpyHostToDevice));
     return AP_OK;
 }
 
 void Poisson::fft_destroy() {
-    if (d_tw) (void)hipFree(d_tw);
-    if (d_omega) (void)hipFree(d_omega);
+    if (d_tw) (void)dev_free(d_tw);
+    if (d_omega) (void)dev_free(d_omega);
     d_tw = nullptr;
     d_omega = nullptr;
 }

@@ -179,7 +179,7 @@ static void trig(int n, std::vector<float>& c, std::vector<float>& st, std::vect
 }
 
 static int upload(const std::vector<float>& h, float** d) {
-    AP_HIP(hipMalloc(d, h.size() * sizeof(float)));
+    AP_HIP(dev_malloc(d, h.size() * sizeof(float)));
     AP_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     return AP_OK;
 }
@@ -202,9 +202,9 @@ int Poisson::create(int nbx, int nby, double bx_, double by_, int method_) {
     int st;
     if ((st = upload(wx, &d_wx)) || (st = upload(wy, &d_wy))) return st;
     const size_t nm = (size_t)N * M;
-    AP_HIP(hipMalloc(&d_work, 4 * nm * sizeof(float)));
-    AP_HIP(hipMalloc(&d_part, MEAN_BLOCKS * sizeof(double)));
-    AP_HIP(hipMalloc(&d_mean, sizeof(float)));
+    AP_HIP(dev_malloc(&d_work, 4 * nm * sizeof(float)));
+    AP_HIP(dev_malloc(&d_part, MEAN_BLOCKS * sizeof(double)));
+    AP_HIP(dev_malloc(&d_mean, sizeof(float)));
     if (method == AP_POISSON_GEMM) {
         std::vector<float> cx, sx, cxt, sxt, cy, sy, cyt, syt;
         trig(N, cx, sx, cxt, sxt);
@@ -221,7 +221,7 @@ int Poisson::create(int nbx, int nby, double bx_, double by_, int method_) {
 void Poisson::destroy() {
     void* ptrs[] = {d_wx, d_wy, d_work, d_cx, d_cxt, d_sxt, d_cy, d_sy, d_cyt, d_part, d_mean};
     for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+        if (p) (void)dev_free(p);
     fft_destroy();
 }
 

@@ -304,13 +304,13 @@ int WaPlan::create(const int* d_net_start, int num_nets, const int* d_pin_cell, 
     void* temp = nullptr;
     std::vector<void*> scratch;
     auto take = [&](void** p, size_t bytes) {
-        const hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 1);
+        const hipError_t e = dev_malloc(p, bytes > 0 ? bytes : 1);
         if (e == hipSuccess) scratch.push_back(*p);
         return e;
     };
     auto release = [&]() {
         (void)hipDeviceSynchronize();
-        for (void* p : scratch) (void)hipFree(p);
+        for (void* p : scratch) (void)dev_free(p);
         scratch.clear();
     };
     const dim3 g(grid_of(num_nets, 256)), b(256);
@@ -348,7 +348,7 @@ int WaPlan::create(const int* d_net_start, int num_nets, const int* d_pin_cell, 
         acc += k ? count.off[k] : 0;
     }
     num_big = count.off[0];
-    if (num_big && (e = hipMalloc(&d_big, num_big * sizeof(int))) != hipSuccess) {
+    if (num_big && (e = dev_malloc(&d_big, num_big * sizeof(int))) != hipSuccess) {
         release();
         AP_HIP(e);
     }
@@ -370,12 +370,12 @@ int WaPlan::create(const int* d_net_start, int num_nets, const int* d_pin_cell, 
     }
     if (num_waves) {
         const long long n = (long long)num_waves * 64;
-        if ((e = hipMalloc(&d_lg, num_waves)) != hipSuccess || (e = hipMalloc(&d_lanes, n * sizeof(int4))) !=
+        if ((e = dev_malloc(&d_lg, num_waves)) != hipSuccess || (e = dev_malloc(&d_lanes, n * sizeof(int4))) !=
             hipSuccess || (e = take((void**)&lane_pin, n * sizeof(int))) != hipSuccess) {
             release();
             AP_HIP(e);
         }
-        if (with_nets && (e = hipMalloc(&d_net, n * sizeof(int))) != hipSuccess) {
+        if (with_nets && (e = dev_malloc(&d_net, n * sizeof(int))) != hipSuccess) {
             release();
             AP_HIP(e);
         }
@@ -393,10 +393,10 @@ int WaPlan::create(const int* d_net_start, int num_nets, const int* d_pin_cell, 
 }
 
 void WaPlan::destroy() {
-    if (d_lg) (void)hipFree(d_lg);
-    if (d_lanes) (void)hipFree(d_lanes);
-    if (d_big) (void)hipFree(d_big);
-    if (d_net) (void)hipFree(d_net);
+    if (d_lg) (void)dev_free(d_lg);
+    if (d_lanes) (void)dev_free(d_lanes);
+    if (d_big) (void)dev_free(d_big);
+    if (d_net) (void)dev_free(d_net);
     d_lg = nullptr;
     d_lanes = nullptr;
     d_big = nullptr;

@@ -32,6 +32,12 @@ extern "C" {
 AP_API const char* ap_last_error(void);
 /* Library ABI version (major * 100 + minor). */
 AP_API int ap_version(void);
+/* The library keeps the device memory of the engines, plans and clusterings it destroys, per device and size class,
+ * for the next ones (no reference counterpart: the reference's buffers are torch tensors, which torch's caching
+ * allocator keeps the same way). ap_cached_device_bytes: the bytes kept and not in use; ap_release_cached_memory hands
+ * them back to HIP (the library also does so by itself when an allocation runs out of memory). */
+AP_API long long ap_cached_device_bytes(void);
+AP_API int ap_release_cached_memory(void);
 
 /* ---------------------------------------------------------------- part 1: drop-in kernels */
 

@@ -152,6 +152,7 @@ hipError_t hipGetDevice(int* d) {
     *d = 0;
     return hipSuccess;
 }
+hipError_t hipSetDevice(int d) { return d == 0 ? hipSuccess : hipErrorInvalidDevice; }
 int hipGetStreamDeviceId(hipStream_t) { return 0; }
 hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
     switch (a) {

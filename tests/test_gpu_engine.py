@@ -218,6 +218,24 @@ def test_engine_graph_equals_direct_and_is_deterministic(small_design):
     assert a[4][K["loop_it"]] == 40 + 32 and a[4][K["evals"]] == 32
 
 
+def test_cached_device_memory_changes_nothing(small_design):
+    """An engine built on the blocks a destroyed one left in the library's device-memory cache (their old contents
+    still in them) runs its passes bit for bit as one built on blocks fresh from HIP."""
+    from autoplace_amd import _lib
+    lib = _lib.lib()
+    _, _, data = small_design
+    lib.ap_release_cached_memory()
+    a, _ = _run(data, 16, True, calls=2)
+    kept = lib.ap_cached_device_bytes()
+    assert kept > 0
+    b, _ = _run(data, 16, True, calls=2)
+    assert lib.ap_cached_device_bytes() == kept          # the second engine took the same blocks back
+    assert lib.ap_release_cached_memory() == 0 and lib.ap_cached_device_bytes() == 0
+    c, _ = _run(data, 16, True, calls=2)
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, y) and np.array_equal(y, z)
+
+
 @pytest.mark.parametrize("grids", [None, (32, 64)])
 def test_engine_two_streams_equal_one_stream(small_design, grids):
     """Graph and direct passes, which run WA and the overflow grid on a second stream beside the force grid, end bit
