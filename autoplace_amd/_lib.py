@@ -68,6 +68,7 @@ _SIGNATURES = {
     "ap_level_set_bucket_room": ([P, I, I], I),
     "ap_level_launch_shape": ([P, P], I),
     "ap_level_set_launch_shape": ([P, I, I, I], I),
+    "ap_level_renumber": ([P, I, I, I, I, ctypes.c_double, ctypes.c_double] + [P] * 3 + [I, I] + [P] * 7, I),
     "ap_level_time_every": ([P, I], I),
     "ap_level_bucket_spills": ([P, P], I),
     "ap_batch_create": ([P, I, ctypes.POINTER(P)], I),

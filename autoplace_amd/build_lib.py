@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libautoplace_gp.so")
 SOURCES = ["gp_kernels.hip", "gp_wa.hip", "gp_poisson.hip", "gp_fft.hip", "gp_engine.hip", "gp_mlp.hip",
-           "gp_cluster.hip", "gp_prep.hip", "gp_devmem.hip"]
+           "gp_cluster.hip", "gp_prep.hip", "gp_devmem.hip",
+           "gp_renumber.hip"]
 ARCH = os.environ.get("AUTOPLACE_ARCH", "gfx950")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-fvisibility=hidden", "-Wall", "-Wno-unused-function"]

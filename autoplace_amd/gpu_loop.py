@@ -13,7 +13,7 @@ import time
 import torch
 
 from . import _lib
-from .locality import hilbert_order, renumber_netlist
+from .locality import renumber_level
 
 FIELDS = ("lam gam alpha a iter hpwl_prev motion ww mu settled best_hpwl best_ovf is_best improved stop accept "
           "loop_it evals res_hpwl res_ovf has_best").split()
@@ -59,8 +59,8 @@ def setup_level(obj, wirelength, density, u, v, g, best, S, kp, c, poisson="auto
     obj.user = (u, v, g, best)
     n, N = d.num_mov, d.num_nodes
     if reorder:
-        perm = hilbert_order(v, d.nbx, d.nby, float(d.bin_np[0]), float(d.bin_np[1]), n)
-        net = renumber_netlist(w["cell32"], w["off"], w["net_start"], n, perm)
+        perm, net = renumber_level(v, d.nbx, d.nby, float(d.bin_np[0]), float(d.bin_np[1]), n, w["cell32"], w["off"],
+                                   w["net_start"])
     else:
         perm = None
         net = w

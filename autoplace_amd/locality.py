@@ -1,93 +1,46 @@
-"""Spatial renumbering of a level's nodes and nets for the device engine.
+"""Spatial renumbering of a level's nodes and nets for the device engine (ap_level_renumber in the library).
 
 The engine's scatter, WA and gather kernels read node data by index. In the problem's own order, the nodes of a
 workgroup (and the cells of a net) are spread over the whole die, so every access is a separate cache line and every
-density contribution a separate global atomic. Renumbering cells (and, separately, fillers) along a Morton curve of
+density contribution a separate global atomic. Renumbering cells (and, separately, fillers) along the Hilbert curve of
 their bins at the start of a level makes index neighbours spatial neighbours; nets are renumbered by their first
 cell in the new order so that the pins of neighbouring nets, and the pins a cell gathers, are close too.
 
 The renumbering is a permutation: the integer density sums do not change, per-net and per-cell sums keep their pin
 order within the net / cell, so results change only by the order of FP64 norm sums. It is deterministic (stable
-sorts of integer keys).
+radix sorts of integer keys, on the device).
 """
+import ctypes
+
 import torch
 
-
-def _spread_bits(v):
-    v = v.long() & 0xFFFF
-    v = (v | (v << 8)) & 0x00FF00FF
-    v = (v | (v << 4)) & 0x0F0F0F0F
-    v = (v | (v << 2)) & 0x33333333
-    v = (v | (v << 1)) & 0x55555555
-    return v
+from . import _lib
 
 
-def morton_order(pos, nbx, nby, bx, by, num_mov):
-    """Permutation new -> old of the nodes: cells, then fillers, each sorted by the Morton code of their bin."""
-    ix = (pos[:, 0] / bx).floor().clamp_(0, nbx - 1)
-    iy = (pos[:, 1] / by).floor().clamp_(0, nby - 1)
-    key = _spread_bits(ix) | (_spread_bits(iy) << 1)
-    cells = torch.argsort(key[:num_mov], stable=True)
-    fill = torch.argsort(key[num_mov:], stable=True) + num_mov
-    return torch.cat([cells, fill])
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
 
 
-def hilbert_key(ix, iy, order):
-    """Index of the cells (ix, iy) along the Hilbert curve of a 2^order x 2^order grid (the classic xy -> d walk,
-    vectorised). Consecutive indices are edge neighbours, so any run of indices covers a compact region."""
-    x, y = ix.long().clone(), iy.long().clone()
-    d = torch.zeros_like(x)
-    s = 1 << (order - 1)
-    while s > 0:
-        rx = ((x & s) > 0).long()
-        ry = ((y & s) > 0).long()
-        d += s * s * ((3 * rx) ^ ry)
-        # rotate the quadrant
-        flip = ry == 0
-        swap_x = torch.where(flip & (rx == 1), s - 1 - x, x)
-        swap_y = torch.where(flip & (rx == 1), s - 1 - y, y)
-        x = torch.where(flip, swap_y, swap_x)
-        y = torch.where(flip, swap_x, swap_y)
-        s >>= 1
-    return d
-
-
-def hilbert_order(pos, nbx, nby, bx, by, num_mov):
-    """Permutation new -> old of the nodes: cells, then fillers, each sorted along the Hilbert curve of their bin."""
-    ix = (pos[:, 0] / bx).floor().clamp_(0, nbx - 1)
-    iy = (pos[:, 1] / by).floor().clamp_(0, nby - 1)
-    order = max(1, int(max(nbx, nby) - 1).bit_length())
-    key = hilbert_key(ix, iy, order)
-    cells = torch.argsort(key[:num_mov], stable=True)
-    fill = torch.argsort(key[num_mov:], stable=True) + num_mov
-    return torch.cat([cells, fill])
-
-
-def renumber_netlist(cell32, off, net_start, num_mov, perm):
-    """Pins grouped by net (optimized-net CSR) -> the same netlist with cells renumbered by `perm` (new -> old) and
-    nets ordered by their smallest new cell index. -> dict(cell32, off, net_start, cell_start, cell_pin)."""
-    dev = cell32.device
-    inv = torch.empty(num_mov + 1, dtype=torch.long, device=dev)
-    inv[perm[:num_mov]] = torch.arange(num_mov, device=dev)
-    inv[num_mov] = num_mov                                     # a fixed pin stays a fixed pin
-    new_cell = inv[cell32.long()]
-    ns = net_start.long()
-    deg = ns[1:] - ns[:-1]
-    nets = len(deg)
-    pins = len(cell32)
-    net_of = torch.repeat_interleave(torch.arange(nets, device=dev), deg)
-    key = torch.full((nets,), num_mov + 1, dtype=torch.long, device=dev)
-    key.scatter_reduce_(0, net_of, new_cell, "amin", include_self=True)
-    order = torch.argsort(key, stable=True)
-    deg_new = deg[order]
-    start_new = torch.zeros(nets + 1, dtype=torch.long, device=dev)
-    start_new[1:] = torch.cumsum(deg_new, 0)
-    src = torch.repeat_interleave(ns[:-1][order], deg_new) + \
-        (torch.arange(pins, device=dev) - torch.repeat_interleave(start_new[:-1], deg_new))
-    c = new_cell[src]
-    o = off[src].contiguous()
-    by_cell = torch.argsort(c, stable=True)
-    cell_start = torch.searchsorted(c[by_cell], torch.arange(num_mov + 1, device=dev))
-    return {"cell32": c.int().contiguous(), "off": o, "net_start": start_new.int().contiguous(),
-            "cell_start": cell_start.int().contiguous(),
-            "cell_pin": by_cell[:int(cell_start[-1])].int().contiguous(), "num_nets": nets}
+def renumber_level(pos, nbx, nby, bx, by, num_mov, cell32, off, net_start):
+    """pos f32 [nodes, 2]; the optimized-net CSR cell32 i32 [P] (num_mov = fixed pin), off f32 [P, 2], net_start
+    i32 [nets + 1], all on the device. -> (perm int64 [nodes], new -> old: cells then fillers, each along the Hilbert
+    curve of their bin on the nbx x nby grid of bx x by bins; dict(cell32, off, net_start, cell_start, cell_pin,
+    num_nets) of the renumbered netlist, cell_pin holding every pin's entry, the fixed pins' after
+    cell_start[num_mov])."""
+    dev = pos.device
+    for t, dt in ((pos, torch.float32), (cell32, torch.int32), (off, torch.float32), (net_start, torch.int32)):
+        if t.dtype != dt or not t.is_contiguous() or t.device != dev:
+            raise ValueError("renumber_level: %s contiguous on %s expected" % (dt, dev))
+    n_nodes, n_pins, n_nets = pos.shape[0], cell32.numel(), net_start.numel() - 1
+    perm = torch.empty(n_nodes, dtype=torch.int32, device=dev)
+    cell = torch.empty(n_pins, dtype=torch.int32, device=dev)
+    o = torch.empty((n_pins, 2), dtype=torch.float32, device=dev)
+    start = torch.empty(n_nets + 1, dtype=torch.int32, device=dev)
+    cell_start = torch.empty(num_mov + 1, dtype=torch.int32, device=dev)
+    cell_pin = torch.empty(max(n_pins, 1), dtype=torch.int32, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(_lib.lib().ap_level_renumber(_p(pos), n_nodes, num_mov, nbx, nby, float(bx), float(by), _p(cell32),
+                                             _p(off), _p(net_start), n_nets, n_pins, _p(perm), _p(cell), _p(o),
+                                             _p(start), _p(cell_start), _p(cell_pin), stream), "level_renumber")
+    return perm.long(), {"cell32": cell, "off": o, "net_start": start, "cell_start": cell_start,
+                         "cell_pin": cell_pin, "num_nets": n_nets}

@@ -273,6 +273,18 @@ AP_API int ap_level_set_tile_layout(ap_level* level, int force_side, int force_g
  * captured graphs. */
 AP_API int ap_level_launch_shape(ap_level* level, int* shape);
 AP_API int ap_level_set_launch_shape(ap_level* level, int wa_per, int wa_lanes, int gather_lanes);
+/* No counterpart in the reference (its kernels read nodes in the problem's order). The spatial renumbering a level's
+ * engine runs on: perm i32 [num_nodes] (new -> old) lists the cells, then the fillers, each in a stable order of the
+ * Hilbert index of their bin (pos f32 [num_nodes][2] over an nbx x nby grid of bx x by bins, nbx, nby <= 32768; the
+ * bin of x is floor(x * (1 / bx)) in single precision, clamped to the grid). The netlist (pin_cell i32 [num_pins]
+ * grouped by net, cell num_mov = fixed pin; pin_off f32 [num_pins][2]; net_start i32 [num_nets + 1]) comes back with
+ * the cells renumbered and the nets in a stable order of their smallest new cell (pin_cell_out, pin_off_out,
+ * net_start_out), with the pins of every cell: cell_start i32 [num_mov + 1] and cell_pin i32 [num_pins] (new pin
+ * indices in order, cell by cell; the fixed pins' entries after cell_start[num_mov]). Synchronises the stream. */
+AP_API int ap_level_renumber(const float* pos, int num_nodes, int num_mov, int nbx, int nby, double bx, double by,
+                             const int* pin_cell, const float* pin_off, const int* net_start, int num_nets,
+                             int num_pins, int* perm, int* pin_cell_out, float* pin_off_out, int* net_start_out,
+                             int* cell_start, int* cell_pin, void* stream);
 /* No counterpart in the reference. The room the density binning gives each tile's bucket in a pass: percent % of the
  * entries the bucket had in the pass before, plus extra (defaults 150 and 64; percent 0..150, extra 0..64). Entries
  * past a bucket's room take the slower spill path (global integer atomics), so a smaller room changes the time, never

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "..", "autoplace_amd", "csrc")
 INCLUDE = os.path.join(HERE, "..", "include")
 STUB = os.path.join(HERE, "hoststub")
 SOURCES = ["gp_kernels", "gp_wa", "gp_poisson", "gp_fft", "gp_engine", "gp_mlp", "gp_cluster", "gp_prep",
-           "gp_devmem"]
+           "gp_devmem", "gp_renumber"]
 
 
 def _hipcc():

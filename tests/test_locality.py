@@ -1,10 +1,12 @@
-"""The engine's spatial renumbering (CPU tensors): a permutation that keeps every net and every cell's pins."""
+"""The statement of the engine's spatial renumbering in torch (tests/renumber_ref.py, CPU tensors): the Hilbert curve
+it follows, a permutation that keeps every net and every cell's pins."""
 import numpy as np
+import pytest
 import torch
 
 
 def test_morton_order_is_a_permutation_keeping_cells_first():
-    from autoplace_amd.locality import morton_order
+    from renumber_ref import morton_order
     rng = np.random.default_rng(0)
     pos = torch.tensor(rng.uniform(0, 100, (1000, 2)), dtype=torch.float32)
     perm = morton_order(pos, 64, 64, 100 / 64, 100 / 64, 700)
@@ -15,10 +17,29 @@ def test_morton_order_is_a_permutation_keeping_cells_first():
     assert (p[1:] - p[:-1]).norm(dim=1).mean() < 10
 
 
-def test_renumbered_netlist_has_the_same_nets():
+def test_hilbert_key_walks_the_grid_cell_by_cell():
+    """The curve the renumbering follows: on a 2^k x 2^k grid the keys are a permutation of 0 .. 4^k - 1 and cells of
+    consecutive keys share an edge; it starts at (0, 0) and ends at (2^k - 1, 0); the 2 x 2 grid is visited
+    (0,0) (0,1) (1,1) (1,0). A wrong rotation or quadrant order breaks one of these."""
+    from renumber_ref import hilbert_key
+    for k in (1, 2, 3, 5):
+        m = 1 << k
+        ix, iy = torch.meshgrid(torch.arange(m), torch.arange(m), indexing="ij")
+        key = hilbert_key(ix.reshape(-1), iy.reshape(-1), k)
+        assert sorted(key.tolist()) == list(range(m * m))
+        at = torch.empty(m * m, 2, dtype=torch.long)
+        at[key] = torch.stack([ix.reshape(-1), iy.reshape(-1)], 1)
+        assert torch.all((at[1:] - at[:-1]).abs().sum(1) == 1)
+        assert at[0].tolist() == [0, 0] and at[-1].tolist() == [m - 1, 0]
+        if k == 1:
+            assert at.tolist() == [[0, 0], [0, 1], [1, 1], [1, 0]]
+
+
+@pytest.mark.parametrize("order", ["morton", "hilbert"])
+def test_renumbered_netlist_has_the_same_nets(order):
     from autoplace_amd import synth
     from autoplace_amd.database import PlaceData
-    from autoplace_amd.locality import morton_order, renumber_netlist
+    from renumber_ref import hilbert_order, morton_order, renumber_netlist
     from autoplace_amd.wirelength import net_csr
     lef, d = synth.generate(800, seed=4, io_pins=5)
     data = PlaceData(lef, d)
@@ -26,7 +47,7 @@ def test_renumbered_netlist_has_the_same_nets():
     csr = net_csr(data.pin_cell, data.pin_off, data.pin_net, data.num_nets, n, 100)
     t = lambda a: torch.as_tensor(a)   # noqa: E731
     pos = torch.tensor(np.random.default_rng(1).uniform(0, data.width, (n + 50, 2)), dtype=torch.float32)
-    perm = morton_order(pos, 32, 32, data.width / 32, data.height / 32, n)
+    perm = (morton_order if order == "morton" else hilbert_order)(pos, 32, 32, data.width / 32, data.height / 32, n)
     new = renumber_netlist(t(csr["cell"]), t(csr["off"]), t(csr["net_start"]), n, perm)
     old_of = torch.cat([perm[:n], torch.tensor([n])])
 
