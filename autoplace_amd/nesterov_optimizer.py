@@ -16,16 +16,22 @@ def _dist(a, b):
 
 
 class NesterovOptimizer(object):
-    def __init__(self, pos, grad_fn, lr, bounds, stats=None):
+    def __init__(self, pos, grad_fn, lr, bounds, stats=None, start=None):
         """pos: updated in place (holds v). grad_fn(x) -> gradient. bounds = (lo, hi). stats: callable -> 0-d tensors
-        of the last evaluated point, fetched with the step estimate in one transfer (left in `self.fetched`)."""
+        of the last evaluated point, fetched with the step estimate in one transfer (left in `self.fetched`).
+        start: the (point, gradient) estimate_initial_learning_rate(..., keep=True) left, used instead of a second
+        evaluation of grad_fn when that point is pos itself (the reference evaluates it again, nesterov_optimizer.py:
+        22-23; the objective is the same, so is the gradient)."""
         self.grad_fn = grad_fn
         self.stats, self.fetched = stats or (lambda: ()), []
         self.lo, self.hi = bounds
         self.kernel = True
         self._scalars = pos.new_empty(2)
         self.v = pos.clamp_(min=self.lo, max=self.hi)
-        self.g = grad_fn(self.v)
+        if start is not None and start[0] is pos:
+            self.g = start[1]
+        else:
+            self.g = grad_fn(self.v)
         self.u = self.v.clone()
         probe = torch.add(self.v, self.g, alpha=-lr).clamp_(min=self.lo, max=self.hi)
         self.alpha = (_dist(self.v, probe) / _dist(self.g, grad_fn(probe))).item()
@@ -52,9 +58,11 @@ class NesterovOptimizer(object):
         self.a = a_next
 
 
-def estimate_initial_learning_rate(pos, grad_fn, lr, bounds):
-    """Step length from one probe step of length lr (nesterov_optimizer.py:56-61)."""
+def estimate_initial_learning_rate(pos, grad_fn, lr, bounds, keep=False):
+    """Step length from one probe step of length lr (nesterov_optimizer.py:56-61). keep: -> (step length,
+    (pos, its gradient)), the start NesterovOptimizer can take over."""
     pos.clamp_(min=bounds[0], max=bounds[1])
     g = grad_fn(pos)
     probe = (pos - lr * g).clamp_(min=bounds[0], max=bounds[1])
-    return (_dist(pos, probe) / _dist(g, grad_fn(probe))).item()
+    step = (_dist(pos, probe) / _dist(g, grad_fn(probe))).item()
+    return (step, (pos, g)) if keep else step

@@ -37,9 +37,9 @@ def _place_level(pos, objective, ps, args, log, first_iter=0, opts=None, phase=0
     opts = opts or {}
     log = gpu_loop.as_log(log)
     bounds = (objective.density.pos_lo, objective.density.pos_hi)
-    lr = estimate_initial_learning_rate(pos, objective, args.lr, bounds)
+    lr, start = estimate_initial_learning_rate(pos, objective, args.lr, bounds, keep=True)
     optimizer = NesterovOptimizer(pos, objective, lr, bounds, stats=lambda: (objective.hpwl, objective.overflow,
-                                                                               objective.wa))
+                                                                               objective.wa), start=start)
     reason, iteration, start_iters = "max_iter", first_iter, 0
     hpwl = overflow = float("nan")
     engine = None
