@@ -92,7 +92,7 @@ def profile_traffic(kernel):
         return None
     for line in open(TRAFFIC_PROFILE):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == "ap::" + kernel:
+        if len(cells) == 5 and (cells[0] == "ap::" + kernel or cells[0].startswith("ap::" + kernel + "<")):
             return float(cells[4]) * 1e6
     return None
 
