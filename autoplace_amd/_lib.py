@@ -71,6 +71,8 @@ _SIGNATURES = {
     "ap_level_renumber": ([P, I, I, I, I, ctypes.c_double, ctypes.c_double] + [P] * 3 + [I, I] + [P] * 7, I),
     "ap_level_time_every": ([P, I], I),
     "ap_level_bucket_spills": ([P, P], I),
+    "ap_level_set_tile_split": ([P, I, I], I),
+    "ap_level_tile_splits": ([P, P], I),
     "ap_batch_create": ([P, I, ctypes.POINTER(P)], I),
     "ap_batch_destroy": ([P], I),
     "ap_batch_prepare": ([P, I], I),

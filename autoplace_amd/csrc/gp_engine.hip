@@ -126,6 +126,11 @@ __device__ __forceinline__ void nesterov_body(const NestArgs& a, int wg) {
 constexpr int BIN_PER_LANE = 8;
 constexpr int BIN_NODES = EB * BIN_PER_LANE;
 constexpr int BIN_MAX_BUCKETS = 16384;   // planes x tiles of a grid: the count / fill kernels' LDS histogram, 64 KB
+constexpr int SPLIT_SLICE = 4096;        // entries per workgroup above which a tile's bucket is split
+constexpr int SPLIT_MAX = 16;            // workgroups that may sum one tile
+constexpr int SPLIT_HELPERS = 256;       // helper workgroups of a grid's tile launch
+constexpr int SPLIT_FACTOR = 8;          // a split tile holds at least this many times the mean entries per tile
+constexpr int SPLIT_LIST = 256;          // marked tiles a pass can split
 constexpr int SCAN_T = 1024;
 
 struct BinJob {
@@ -162,7 +167,21 @@ struct BinJob {
     unsigned long long* spill_n;  // [1] fill form: entries spilled since the level was created
     float* rho_mirror;         // drop-in map (null: none): the total density mirrored in both axes, [2 nbx][2 nby]
     float* over_map;           // drop-in map (null: none): max(cells + blocked - target, 0) per bin, [nbx][nby]
+    // fill form, split tiles: a tile whose bucket reaches split_need entries in a pass (marked by the fill kernel as
+    // it counts) is summed by its own workgroup and helper workgroups (leading the tile launch), a slice of split_slice
+    // entries or more each, at most SPLIT_MAX workgroups a tile
+    int split_slice;           // 0: no splitting
+    int split_factor;          // split_need: the larger of 2 x split_slice and split_factor x the mean per tile
+    int split_need;            // entries of one bucket that mark its tile (0: none)
+    int helpers;               // helper workgroups leading the tile launch (0: none)
+    int* heavy_n;              // [1] tiles marked in the pass (cleared by the rooms step)
+    int* heavy_list;           // [SPLIT_LIST] the marked tiles, in the order they were marked
+    int* heavy_pos;            // [tiles] a marked tile's place in heavy_list (SPLIT_LIST: not listed); -1 unmarked
+    unsigned* arrive;          // [tiles] participants of a split tile done (zero between passes)
+    unsigned long long* split_n;  // [1] split tiles summed since the level was created
 };
+
+void split_threshold(BinJob& J);
 
 __device__ __forceinline__ unsigned long long contribution(float wx, const Span& sy, int by) {
     return fixed_point32(wx * span_overlap(sy, by));
@@ -356,6 +375,7 @@ __device__ __forceinline__ void rooms_body(const BinJob& J) {
     if (threadIdx.x == 0) {
         J.start[n] = (int)(carry < cap ? carry : cap);
         *J.big_count = 0;
+        if (J.helpers) *J.heavy_n = 0;
     }
 }
 // the rooms of both grids before an evaluation that does not follow a Nesterov step (workgroup 0: force grid,
@@ -505,6 +525,14 @@ __device__ __forceinline__ void bin_fill_body(const Pick& pos_p, const BinJob& f
         if (!c) continue;
         const int at = atomicAdd(&J.count[t], c), lo = J.start[t], hi = J.start[t + 1];
         hist[t] = hi - lo - at >= c ? lo + at : ~(lo + at);
+        if (J.helpers && J.split_need && at < J.split_need && at + c >= J.split_need) {   // this bucket's tile is heavy
+            const int tt = t >= J.tiles ? t - J.tiles : t;
+            if (atomicCAS(J.heavy_pos + tt, -1, -2) == -1) {
+                const int idx = atomicAdd(J.heavy_n, 1);
+                if (idx < SPLIT_LIST) J.heavy_list[idx] = tt;
+                J.heavy_pos[tt] = min(idx, SPLIT_LIST);
+            }
+        }
     }
     __syncthreads();
     unsigned spilled = 0;   // bit 4 q + k: entry k of node q goes to the spill map
@@ -563,17 +591,36 @@ constexpr int TILE_PF = 4;  // fixed-map values per lane loaded ahead (the whole
 // One workgroup of NT lanes sums J.G tiles (G = 1, 2, 4, 8), each with its own NT / G lanes (whole waves) and its
 // own LDS accumulator, so that small tiles do not each hold a workgroup (and the launch's LDS reservation) to
 // themselves. The sub-groups pass the same barriers; a sub-group past the last tile only takes part in them.
+// Split tiles (fill form, J.helpers): the tile's workgroup sums the first slice of its bucket's entries, and helper
+// workgroups (the launch's first J.helpers, all lanes on one tile, dispatched before the tiles' own so that they run
+// beside them) sum the others, as the split plan of the pass's
+// rooms step assigned them. Each adds its sums to the grid's spill map with integer atomics and counts itself in
+// arrive[t]; the one that arrives last takes the tile's sums back (clearing them) and writes the tile as an unsplit
+// tile's workgroup does. Integer sums, so the map does not depend on the split; no workgroup waits for another.
 // (wg: the workgroup within the grid's launch; part: the grid's overflow partials)
 template <int NT>
 __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& job, float target, double* part,
                                                int wg) {
     extern __shared__ unsigned long long acc_all[];   // G x planes x T x T
     __shared__ double red[NT / 64];
+    __shared__ int last_of[8];         // per sub-group: this workgroup writes its split tile
     const BinJob J = job;              // by value: a reference to a kernel argument would go to scratch
-    const int S = NT / J.G;            // lanes per tile
+    const bool split_on = J.helpers && J.fill;
+    const bool helper = split_on && wg < J.helpers;
+    // the marked tiles share the helpers evenly, R each (up to SPLIT_MAX - 1); helper h takes slice h % R + 1 of
+    // marked tile h / R
+    int R = 0;
+    if (split_on) {
+        const int nh = min(*J.heavy_n, SPLIT_LIST);
+        R = nh ? min(SPLIT_MAX - 1, J.helpers / nh) : 0;
+        if (helper && (R == 0 || wg / R >= nh)) return;   // an idle helper leaves before any barrier
+    }
+    const int S = helper ? NT : NT / J.G;   // lanes per tile
     const int sub = threadIdx.x / S, l = threadIdx.x - sub * S;
-    const int t = wg * J.G + sub;
+    const int t = helper ? J.heavy_list[wg / R] : (wg - (split_on ? J.helpers : 0)) * J.G + sub;
     const bool live = t < J.tiles;
+    const int hp = live && split_on ? J.heavy_pos[t] : -1;   // its place among the marked tiles
+    const int slice = helper ? wg % R + 1 : 0;
     const int T = J.T, TT = T * T;
     const int X0 = (t / J.tiles_y) * T, Y0 = (t % J.tiles_y) * T;
     const int X1 = min(X0 + T, J.nbx) - 1, Y1 = min(Y0 + T, J.nby) - 1;
@@ -583,7 +630,7 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
     // its stride), so that a tile pays two dependent memory latencies, not four
     int lo0 = 0, n0 = 0, lo1 = 0, n1 = 0, c0 = 0, c1 = 0, spilled = 0, nbig = 0;
     if (live) {
-        nbig = J.fill ? *J.big_count : *J.big_n;
+        if (slice == 0) nbig = J.fill ? *J.big_count : *J.big_n;   // big nodes: the tile's own workgroup
         lo0 = J.start[t];
         n0 = J.start[t + 1];
         if (J.planes == 2) {
@@ -593,9 +640,14 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
         if (J.fill) {
             c0 = J.count[t];
             if (J.planes == 2) c1 = J.count[J.tiles + t];
-            spilled = J.spill_tile[t];
         }
     }
+    // workgroups that sum tile t: a listed tile is split into slices of split_slice entries or more, as many as its
+    // share of helpers allows
+    const int k = hp >= 0 && hp < SPLIT_LIST && J.split_slice > 0
+                      ? 1 + min(R, max(0, (c0 + c1) / J.split_slice - 1)) : 1;
+    if (helper && slice >= k) return;              // more helpers than slices: uniform over the workgroup
+    if (live && J.fill && k == 1) spilled = J.spill_tile[t];   // a split tile's spilled sums stay until its end
     float fix[TILE_PF];
 #pragma unroll
     for (int i = 0; i < TILE_PF; i++) {
@@ -613,11 +665,13 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
     __syncthreads();
     // the entries of both planes as one range (plane 0 first), two per lane per round, both loaded before either is
     // used
-    const int n = live ? n0 + n1 : 0;
+    const int n_all = live ? n0 + n1 : 0;
+    // this workgroup's slice of the range (all of it for an unsplit tile)
+    const int q0 = (int)((long long)n_all * slice / k), n = (int)((long long)n_all * (slice + 1) / k);
     // A bucket lists its nodes about in curve order, so consecutive entries hit the same bins. The lanes of one wave
     // take entries S / 64 apart (lane i of wave w: entry i S / 64 + w), which spreads a wave's atomics over the tile.
     const int lp = (l & 63) * (S >> 6) + (l >> 6);
-    for (int q = lp; q < n; q += 2 * S) {
+    for (int q = q0 + lp; q < n; q += 2 * S) {
         float4 g[2];
         float r[2];
         int pl[2];
@@ -671,6 +725,39 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
         if (l == 0) J.spill_tile[t] = 0;
     }
     __syncthreads();
+    // a split tile: this workgroup's sums into the spill map, then the count of the tile's workgroups done; the last
+    // one takes every sum back
+    bool fin = live && k == 1;
+    if (live && k > 1) {
+        for (int e = l; e < J.planes * TT; e += S) {
+            const int pl = e >= TT, bx = X0 + (e - pl * TT) / T, by = Y0 + (e - pl * TT) % T;
+            const unsigned long long v = acc[e];
+            if (v && bx <= X1 && by <= Y1) atomicAdd(J.spill + ((long long)pl * J.nbx + bx) * J.nby + by, v);
+        }
+        // the sums performed before the count: every access of the split protocol is a device-scope atomic (done
+        // at the point of coherence), so waiting for this lane's vector memory operations to complete orders them;
+        // an agent-scope fence would also write the L2 back (s_waitcnt: vmcnt 0, expcnt and lgkmcnt unchecked)
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+    }
+    __syncthreads();
+    if (live && k > 1 && l == 0) last_of[sub] = atomicAdd(J.arrive + t, 1u) == (unsigned)(k - 1);
+    __syncthreads();
+    if (live && k > 1 && last_of[sub]) {   // the others' sums are in the map: read and cleared by atomics
+        for (int e = l; e < J.planes * TT; e += S) {
+            const int pl = e >= TT, bx = X0 + (e - pl * TT) / T, by = Y0 + (e - pl * TT) % T;
+            acc[e] = bx <= X1 && by <= Y1 ? atomicExch(J.spill + ((long long)pl * J.nbx + bx) * J.nby + by, 0ull)
+                                          : 0ull;
+        }
+        if (l == 0) {
+            atomicExch(J.arrive + t, 0u);
+            J.spill_tile[t] = 0;
+            J.heavy_pos[t] = -1;
+            atomicAdd(J.split_n, 1ull);
+        }
+        fin = true;
+    }
+    if (live && k == 1 && hp != -1 && l == 0) J.heavy_pos[t] = -1;   // marked, not split: unmarked for the next pass
+    __syncthreads();
     double o = 0.0;
     auto post = [&](int e, float fixed) {
         const int bx = X0 + e / T, by = Y0 + e % T;
@@ -692,9 +779,9 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
 #pragma unroll
     for (int i = 0; i < TILE_PF; i++) {
         const int e = l + i * S;
-        if (live && e < TT) post(e, fix[i]);
+        if (fin && e < TT) post(e, fix[i]);
     }
-    for (int e = l + TILE_PF * S; live && e < TT; e += S) {
+    for (int e = l + TILE_PF * S; fin && e < TT; e += S) {
         const int bx = X0 + e / T, by = Y0 + e % T;
         post(e, bx <= X1 && by <= Y1 ? J.fixed[(long long)bx * J.nby + by] : 0.f);
     }
@@ -702,7 +789,7 @@ __device__ __forceinline__ void bin_tiles_body(const Pick& pos_p, const BinJob& 
     o = wave_sum(o);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = o;
     __syncthreads();
-    if (live && l == 0) {
+    if (fin && l == 0) {
         double s = 0;
         for (int w = 0; w < S / 64; w++) s += red[sub * (S / 64) + w];
         part[t] = s;
@@ -1412,8 +1499,18 @@ int Level::bin_job(BinJob& J, const float* size, const float* ratio, int num_nod
         (st = alloc(&J.off, num_nodes)) || (st = alloc(&J.geo, J.room)) || (st = alloc(&J.gr, J.room)) ||
         (st = alloc(&J.big_count, 1)) || (st = alloc(&J.big_n, 1)) || (st = alloc(&J.big, num_nodes)) ||
         (st = alloc(&J.spill, (long long)planes * nbx * nby)) || (st = alloc(&J.spill_tile, BIN_MAX_BUCKETS)) ||
-        (st = alloc(&J.spill_n, 1)))
+        (st = alloc(&J.spill_n, 1)) || (st = alloc(&J.heavy_n, 1)) || (st = alloc(&J.heavy_list, SPLIT_LIST)) ||
+        (st = alloc(&J.heavy_pos, BIN_MAX_BUCKETS)) || (st = alloc(&J.arrive, BIN_MAX_BUCKETS)) ||
+        (st = alloc(&J.split_n, 1)))
         return st;
+    J.split_slice = SPLIT_SLICE;
+    J.split_factor = SPLIT_FACTOR;
+    J.helpers = SPLIT_HELPERS;
+    split_threshold(J);
+    AP_HIP(hipMemset(J.heavy_n, 0, sizeof(int)));
+    AP_HIP(hipMemset(J.heavy_pos, 0xff, BIN_MAX_BUCKETS * sizeof(int)));    // -1: unmarked
+    AP_HIP(hipMemset(J.arrive, 0, BIN_MAX_BUCKETS * sizeof(unsigned)));
+    AP_HIP(hipMemset(J.split_n, 0, sizeof(unsigned long long)));
     AP_HIP(hipMemset(J.spill_n, 0, sizeof(unsigned long long)));
     AP_HIP(hipMemset(J.count, 0, BIN_MAX_BUCKETS * sizeof(int)));
     AP_HIP(hipMemset(J.start, 0, (BIN_MAX_BUCKETS + 1) * sizeof(int)));
@@ -1448,6 +1545,18 @@ void Level::set_side(BinJob& J, int T, int G) {
     J.tiles = (int)grid_of(J.nbx, T) * J.tiles_y;
     J.G = G ? G : default_group(J.planes, T);
     J.wgs = (J.tiles + J.G - 1) / J.G;
+    split_threshold(J);
+}
+
+// the bucket count that marks a tile for splitting: a tile needs 2 x split_slice entries and split_factor x the mean
+// per tile (nodes / tiles, about one entry per node); either of its planes' buckets reaching its share marks it
+void split_threshold(BinJob& J) {
+    if (!J.helpers || !J.split_slice || !J.tiles) {
+        J.split_need = 0;
+        return;
+    }
+    const long long need = std::max<long long>(2LL * J.split_slice, (long long)J.split_factor * J.num_nodes / J.tiles);
+    J.split_need = (int)std::min<long long>(need / J.planes, 1LL << 30);
 }
 
 void Level::update_lds() {
@@ -1509,10 +1618,11 @@ int Level::bin_grid(int which, const Pick& pos, hipStream_t s) {
 // the tile sums of one grid: density and overflow partials
 int Level::tiles_grid(int which, const Pick& pos, hipStream_t s) {
     if (!which)
-        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.wgs), dim3(TB_LEVEL), acc_f, s, pos, fj, d.target, p_over);
+        hipLaunchKernelGGL(eng_bin_tiles, dim3(fj.wgs + fj.helpers), dim3(TB_LEVEL), acc_f, s, pos, fj, d.target,
+                           p_over);
     else
-        hipLaunchKernelGGL(eng_over_tiles, dim3(jj.wgs), dim3(TB_LEVEL), acc_j, s, pos, jj, d.target, p_over,
-                           fj.tiles);
+        hipLaunchKernelGGL(eng_over_tiles, dim3(jj.wgs + jj.helpers), dim3(TB_LEVEL), acc_j, s, pos, jj, d.target,
+                           p_over, fj.tiles);
     AP_LAUNCHED();
     return AP_OK;
 }
@@ -2190,8 +2300,10 @@ int Batch::build() {
         T.pos = pos;
         T.fj = L.fj;
         T.jj = L.jj;
-        T.fj.fill = 0;                                 // the grouped launches: counting-sort form
+        T.fj.fill = 0;                                 // the grouped launches: counting-sort form, no split tiles
         T.jj.fill = 0;
+        T.fj.helpers = T.fj.split_need = 0;
+        T.jj.helpers = T.jj.split_need = 0;
         T.target = L.d.target;
         T.p_over = L.p_over;
         T.gather = L.gather_args(pos, L.d.kp, (const float2*)L.d.g, L.g1, (float2*)L.d.g, L.g1, 1);
@@ -2881,6 +2993,35 @@ int ap_level_set_bucket_room(ap_level* level, int percent, int extra) {
         if (L.last_ev == &G.ev) L.last_ev = nullptr;
         G.clear();
     }
+    return AP_OK;
+}
+
+int ap_level_set_tile_split(ap_level* level, int slice, int factor) {
+    AP_REQUIRE(level && (slice == 0 || (slice >= 64 && slice <= (1 << 24))) && factor >= 0 && factor <= 1024,
+               "level_set_tile_split: slice %d outside 0 (off) or 64 .. 2^24 entries, or factor %d outside 0 .. 1024",
+               slice, factor);
+    Level& L = level->L;
+    AP_HIP(hipDeviceSynchronize());
+    for (BinJob* J : {&L.fj, &L.jj})
+        if (J->helpers) {
+            J->split_slice = slice;
+            J->split_factor = factor;
+            split_threshold(*J);
+        }
+    for (GraphEntry& G : L.graphs) {                 // the kernels' arguments change: capture again
+        if (L.last_ev == &G.ev) L.last_ev = nullptr;
+        G.clear();
+    }
+    return AP_OK;
+}
+
+int ap_level_tile_splits(ap_level* level, unsigned long long* splits) {
+    AP_REQUIRE(level && splits, "level_tile_splits: null argument");
+    Level& L = level->L;
+    AP_HIP(hipDeviceSynchronize());
+    splits[0] = splits[1] = 0;
+    if (L.fj.split_n) AP_HIP(hipMemcpy(&splits[0], L.fj.split_n, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (L.jj.split_n) AP_HIP(hipMemcpy(&splits[1], L.jj.split_n, sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return AP_OK;
 }
 

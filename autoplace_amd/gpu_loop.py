@@ -156,6 +156,18 @@ class Engine(object):
         _lib.check(_lib.lib().ap_level_bucket_spills(self._h, ctypes.cast(out, ctypes.c_void_p)), "level_bucket_spills")
         return int(out[0]), int(out[1])
 
+    def set_tile_split(self, slice_entries=4096, factor=8):
+        """Split a density tile's bucket over up to 16 workgroups when it had 2 x slice_entries or more and factor x
+        the grid's mean per tile or more (slice_entries 0: never)."""
+        _lib.check(_lib.lib().ap_level_set_tile_split(self._h, int(slice_entries), int(factor)),
+                   "level_set_tile_split")
+
+    def tile_splits(self):
+        """(force, overflow): split density tiles summed since the engine was created."""
+        out = (ctypes.c_ulonglong * 2)()
+        _lib.check(_lib.lib().ap_level_tile_splits(self._h, ctypes.cast(out, ctypes.c_void_p)), "level_tile_splits")
+        return int(out[0]), int(out[1])
+
     def finish(self):
         """Write u / v / g / best back into the caller's tensors (caller's node order)."""
         if self.perm is None:

@@ -166,6 +166,63 @@ def test_engine_density_bit_exact(small_design, case):
     assert rel_err(ovf_dev, ovf) < 1e-12
 
 
+@pytest.mark.parametrize("spread", [1.0, 0.04])
+def test_split_tiles_change_nothing(spread):
+    """Tiles whose buckets are summed by several workgroups (a slice of the entries each, the sums met in the spill
+    map, the last workgroup writing the tile) give the oracle's map bit for bit and its overflow: every tile split
+    (slices of 64 and 512 entries without the mean condition), and the level's start case, all nodes in a corner of
+    4 % of the die's side, at the defaults."""
+    from autoplace_amd import synth
+    from autoplace_amd.database import PlaceData
+    from autoplace_amd.problem import build_problem, default_args
+    from autoplace_amd.run_placement import build_operators
+    from oracle import density as oden
+    lef, d = synth.generate(30000, num_macros=3, macro_frac=0.1, seed=12, io_pins=8, name="split")
+    data = PlaceData(lef, d)
+    args = default_args(force_num_bin=256)
+    prob = build_problem(data, args)
+    wl, den = build_operators(prob, args, DEV, "fft")
+    rng = np.random.default_rng(6)
+    lo, hi = den.pos_lo.cpu().numpy(), den.pos_hi.cpu().numpy()
+    pos = (lo + spread * rng.random(lo.shape) * (hi - lo)).astype(np.float32)
+    S = np.zeros(21)
+    kp = np.array([30.0, 1e-3, 1.0, 0.0, 0.0], np.float32)
+    c = np.zeros(13)
+    eng, _ = _engine(wl, den, pos, S, kp, c, np.zeros_like(pos), reorder=True, poisson="fft")
+    grad = torch.empty_like(eng.v)
+    stats = torch.zeros(8, dtype=torch.float64, device=DEV)
+
+    def evaluate():
+        stats.zero_()
+        eng.objective(eng.v, grad, stats)
+        return eng.density().cpu().numpy(), float(stats[2])
+
+    smooth, ratio = den.smooth["size"].cpu().numpy(), den.smooth["ratio"].cpu().numpy()
+    bx, by = float(den.bin_np[0]), float(den.bin_np[1])
+    m0, m1 = oden.density_maps(pos, smooth, ratio, data.num_mov, bx, by, den.nbx, den.nby)
+    rho, _ = oden.density_post(m0, m1, den.fixed_map.cpu().numpy(), den.target, den.nbx, den.nby)
+    # (slice, mean factor, whether tiles must split): spread out, the tiles hold a few hundred entries each and split
+    # only without the mean condition; in the corner, at the defaults
+    slices = ((64, 0, True), (512, 0, False), (64, 8, False)) if spread == 1.0 else ((4096, 8, True), (64, 0, True))
+    rho0, ovf0 = evaluate()
+    assert np.array_equal(rho0, rho)
+    for sl, factor, must in slices:
+        eng.set_tile_split(sl, factor)
+        for _ in range(2):                           # the split plan comes from the evaluation before
+            before = eng.tile_splits()
+            r, o = evaluate()
+            after = eng.tile_splits()
+            print("slice %d (spread %.2f): split tiles %s, map equal %s" % (sl, spread, (after[0] - before[0],
+                  after[1] - before[1]), np.array_equal(r, rho)), flush=True)
+            assert np.array_equal(r, rho), sl
+            assert rel_err(o, ovf0) < 1e-12, (sl, o, ovf0)
+        assert sum(after) > sum(before) or not must  # tiles were split
+    eng.set_tile_split(0, 0)
+    r, o = evaluate()
+    assert np.array_equal(r, rho) and rel_err(o, ovf0) < 1e-12
+    eng.close()
+
+
 def _run(data, passes, use_graph, calls=1, reorder=True, poisson="gemm", time_cls=None, profile=False, grids=None,
          room=None, spills=None, every=1, shape=None):
     args, prob, wl, den, pos, S, kp, c = _setup(data, grids=grids)
