@@ -129,7 +129,7 @@ constexpr int BIN_MAX_BUCKETS = 16384;   // planes x tiles of a grid: the count 
 constexpr int SPLIT_SLICE = 4096;        // entries per workgroup above which a tile's bucket is split
 constexpr int SPLIT_MAX = 16;            // workgroups that may sum one tile
 constexpr int SPLIT_HELPERS = 256;       // helper workgroups of a grid's tile launch
-constexpr int SPLIT_FACTOR = 8;          // a split tile holds at least this many times the mean entries per tile
+constexpr int SPLIT_FACTOR = 4;          // a split tile holds at least this many times the mean entries per tile
 constexpr int SPLIT_LIST = 256;          // marked tiles a pass can split
 constexpr int SCAN_T = 1024;
 

@@ -156,7 +156,7 @@ class Engine(object):
         _lib.check(_lib.lib().ap_level_bucket_spills(self._h, ctypes.cast(out, ctypes.c_void_p)), "level_bucket_spills")
         return int(out[0]), int(out[1])
 
-    def set_tile_split(self, slice_entries=4096, factor=8):
+    def set_tile_split(self, slice_entries=4096, factor=4):
         """Split a density tile's bucket over up to 16 workgroups when it had 2 x slice_entries or more and factor x
         the grid's mean per tile or more (slice_entries 0: never)."""
         _lib.check(_lib.lib().ap_level_set_tile_split(self._h, int(slice_entries), int(factor)),

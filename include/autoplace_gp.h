@@ -296,7 +296,7 @@ AP_API int ap_level_set_bucket_room(ap_level* level, int percent, int extra);
 AP_API int ap_level_bucket_spills(ap_level* level, unsigned long long* spills);
 /* No counterpart in the reference. A density tile whose bucket had, in the pass before, 2 x slice entries or more and
  * factor times the grid's mean per tile or more is summed by up to 16 workgroups, a slice of its entries each
- * (defaults: slice 4096, factor 8; slice 0: never split, else 64 .. 2^24; factor 0 .. 1024): the level's start can
+ * (defaults: slice 4096, factor 4; slice 0: never split, else 64 .. 2^24; factor 0 .. 1024): the level's start can
  * hold most nodes in a few tiles, which one workgroup each would sum alone while the rest of the GPU idles. Integer
  * sums, so the map does not change; a small slice with factor 0 splits every tile, which is what it is for (tests).
  * Drops the level's captured graphs. tile_splits: u64[2], the split tiles summed since the level was created (force

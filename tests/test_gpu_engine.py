@@ -203,7 +203,7 @@ def test_split_tiles_change_nothing(spread):
     rho, _ = oden.density_post(m0, m1, den.fixed_map.cpu().numpy(), den.target, den.nbx, den.nby)
     # (slice, mean factor, whether tiles must split): spread out, the tiles hold a few hundred entries each and split
     # only without the mean condition; in the corner, at the defaults
-    slices = ((64, 0, True), (512, 0, False), (64, 8, False)) if spread == 1.0 else ((4096, 8, True), (64, 0, True))
+    slices = ((64, 0, True), (512, 0, False), (64, 8, False)) if spread == 1.0 else ((4096, 4, True), (64, 0, True))
     rho0, ovf0 = evaluate()
     assert np.array_equal(rho0, rho)
     for sl, factor, must in slices:
