@@ -29,9 +29,11 @@ constexpr int DPP_QUAD_XOR2 = 0x4E;        // quad_perm [2, 3, 0, 1]
 constexpr int DPP_ROW_HALF_MIRROR = 0x141; // lane i of each 8 <- lane 7 - i
 constexpr int DPP_ROW_MIRROR = 0x140;      // lane i of each 16 <- lane 15 - i
 
+// Every pattern used reads a lane of the same row, so no lane takes the bound value; with bound_ctrl set the compiler
+// folds the move into the operation that consumes it (v_max_f32_dpp, v_add_f32_dpp) instead of a separate mov.
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
 struct MaxOp {
@@ -64,11 +66,12 @@ __device__ __forceinline__ float group_reduce(float v, Op op) {
 // 1/e), so every sum of weights and weighted positions keeps a relative error of about 2^-23.
 __device__ __forceinline__ float exp_weight(float t) { return __builtin_amdgcn_exp2f(t * 1.44269504f); }
 
-// q: the position of the pin's cell (zero for a fixed pin or an empty lane), loaded by the caller. OUT: the group's
-// first lane also writes the net's (WA, HPWL) to a.out[net] ((0, 0) for a net without pins)
+// q: the position of the pin's cell (zero for a fixed pin or an empty lane), loaded by the caller; ig = 1 / gamma,
+// divided once per thread by the caller. OUT: the group's first lane also writes the net's (WA, HPWL) to a.out[net]
+// ((0, 0) for a net without pins)
 template <int LG, bool OUT = false>
-__device__ __forceinline__ void wa_group(const WaveArgs& a, float2 q, const int4& ln, int lane, double (&acc)[2],
-                                         int net = -1) {
+__device__ __forceinline__ void wa_group(const WaveArgs& a, float2 q, const int4& ln, int lane, float ig,
+                                         double (&acc)[2], int net = -1) {
     constexpr int G = 1 << LG;
     const int sub = lane & (G - 1);
     const int c = ln.x, slot = ln.y;
@@ -76,7 +79,6 @@ __device__ __forceinline__ void wa_group(const WaveArgs& a, float2 q, const int4
     float2 x = make_float2(__int_as_float(ln.z), __int_as_float(ln.w));
     x.x += q.x;
     x.y += q.y;
-    const float ig = 1.0f / *a.gamma;
     const float NEG = -3.402823466e38f, POS = 3.402823466e38f;
     const float hx = group_reduce<LG>(live ? x.x : NEG, MaxOp()), hy = group_reduce<LG>(live ? x.y : NEG, MaxOp());
     const float lx = group_reduce<LG>(live ? x.x : POS, MinOp()), ly = group_reduce<LG>(live ? x.y : POS, MinOp());
@@ -126,16 +128,17 @@ __device__ __forceinline__ void wa_wave_body(const WaveArgs& a, int wg, int nwg)
 #pragma unroll
     for (int k = 0; k < PER; k++)
         q[k] = ln[k].x >= 0 && ln[k].x < a.num_mov ? pos[ln[k].x] : make_float2(0.f, 0.f);
+    const float ig = 1.0f / *a.gamma;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         switch (lg[k]) {   // uniform over the wavefront
             case 0: break;
-            case 1: wa_group<1, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
-            case 2: wa_group<2, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
-            case 3: wa_group<3, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
-            case 4: wa_group<4, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
-            case 5: wa_group<5, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
-            default: wa_group<6, OUT>(a, q[k], ln[k], lane, acc[k], nt[k]); break;
+            case 1: wa_group<1, OUT>(a, q[k], ln[k], lane, ig, acc[k], nt[k]); break;
+            case 2: wa_group<2, OUT>(a, q[k], ln[k], lane, ig, acc[k], nt[k]); break;
+            case 3: wa_group<3, OUT>(a, q[k], ln[k], lane, ig, acc[k], nt[k]); break;
+            case 4: wa_group<4, OUT>(a, q[k], ln[k], lane, ig, acc[k], nt[k]); break;
+            case 5: wa_group<5, OUT>(a, q[k], ln[k], lane, ig, acc[k], nt[k]); break;
+            default: wa_group<6, OUT>(a, q[k], ln[k], lane, ig, acc[k], nt[k]); break;
         }
     }
     if (OUT) return;                                 // per-net outputs only: no partials
