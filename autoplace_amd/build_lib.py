@@ -21,6 +21,11 @@ SOURCES = ["gp_kernels.hip", "gp_wa.hip", "gp_poisson.hip", "gp_fft.hip", "gp_en
 ARCH = os.environ.get("AUTOPLACE_ARCH", "gfx950")
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-fvisibility=hidden", "-Wall", "-Wno-unused-function"]
+# Per-source device flags. gp_wa.hip: its max / min reductions take finite positions (or the +-FLT_MAX of empty
+# lanes), so the device code is compiled without NaN semantics: fmaxf / fminf then need no canonicalising
+# instruction per operand and the DPP move folds into v_max_f32 / v_min_f32 (about 15 % fewer VALU instructions in
+# the WA kernels, the same bits for every finite input; a NaN position still makes its net's WA sum NaN).
+SOURCE_FLAGS = {"gp_wa.hip": ["-Xarch_device", "-fno-honor-nans"]}
 
 
 def _hipcc():
@@ -40,7 +45,7 @@ def build(verbose=False):
     with tempfile.TemporaryDirectory() as tmp:
         def compile_one(src):
             obj = os.path.join(tmp, src.replace(".hip", ".o"))
-            cmd = [hipcc] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+            cmd = [hipcc] + FLAGS + SOURCE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
             r = subprocess.run(cmd, capture_output=True, text=True)
             if r.returncode != 0:
                 raise RuntimeError("hipcc failed on %s:\n%s" % (src, r.stderr))
