@@ -1344,9 +1344,10 @@ struct Level {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;                             // WA and the overflow grid, beside the force grid
     hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr, ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_cells = nullptr;                          // the side stream's cell WA sums done (the gather's input)
     // the fork / join events of graph captures, kept apart from the ones direct launches record: an event is used
     // either inside captures or outside them, never both
-    hipEvent_t cap_fork = nullptr, cap_join = nullptr;
+    hipEvent_t cap_fork = nullptr, cap_join = nullptr, cap_cells = nullptr;
     bool capturing = false;
     GraphEntry graphs[GRAPH_CACHE];
     int graph_next = 0;
@@ -1885,8 +1886,10 @@ int Level::create(const ap_level_desc& desc) {
     AP_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
     AP_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     AP_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    AP_HIP(hipEventCreateWithFlags(&ev_cells, hipEventDisableTiming));
     AP_HIP(hipEventCreateWithFlags(&cap_fork, hipEventDisableTiming));
     AP_HIP(hipEventCreateWithFlags(&cap_join, hipEventDisableTiming));
+    AP_HIP(hipEventCreateWithFlags(&cap_cells, hipEventDisableTiming));
     AP_HIP(hipEventCreate(&t0));
     AP_HIP(hipEventCreate(&t1));
     AP_HIP(hipDeviceSynchronize());
@@ -1913,7 +1916,7 @@ void Level::destroy() {
     wa_plan.destroy();
     if (stream) (void)hipStreamDestroy(stream);
     if (side) (void)hipStreamDestroy(side);
-    hipEvent_t evs[] = {ev_in, ev_out, t0, t1, ev_fork, ev_join, cap_fork, cap_join};
+    hipEvent_t evs[] = {ev_in, ev_out, t0, t1, ev_fork, ev_join, ev_cells, cap_fork, cap_join, cap_cells};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
 }
@@ -1974,7 +1977,9 @@ Pick Level::cur_pos() const { return Pick{(const float2*)v1, (const float2*)d.v,
 // objective at pos with the parameters kp; g_prev (optional) gives |g' - g|. With S the control pass follows.
 // The objective's kernels. Passes that are not profiled run two streams: the caller's stream s takes the force grid
 // (binning, tiles, Poisson solve), and a side stream forked from it takes WA and the overflow grid, which read the
-// same positions and write buffers of their own; the gather, which needs both, waits for the side stream. Profiled
+// same positions and write buffers of their own. The gather waits only for the side stream's cell WA sums (done
+// long before the force grid's Poisson solve ends, so the wait costs no cross-queue latency), the reduction for the
+// whole side stream (its overflow grid), by when the gather has given that stream time to finish. Profiled
 // passes run every kernel class in order on s, so that the events between the classes time each of them alone.
 int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, const float2* g_prev,
                      const float2* g_prev_b, float2* g_out_a, float2* g_out_b, hipStream_t s, double* S, float* kp_out,
@@ -2015,6 +2020,7 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
     AP_LAUNCHED();
     tend(AP_PROF_WA_CELLS, w);
     g.cell_wl = cell_wl;
+    if (two) AP_HIP(hipEventRecord(capturing ? cap_cells : ev_cells, side));
     if (use_density) {
         mark(AP_PROF_MAP, s);
         if (over && (st = bin_grid(1, pos, w))) return st;
@@ -2032,16 +2038,17 @@ int Level::objective(const Pick& pos, const float* kp_ptr, bool with_state, cons
         if ((st = solve_field(s))) return st;
         tend(AP_PROF_POISSON, s);
     }
-    if (two) {
-        AP_HIP(hipEventRecord(capturing ? cap_join : ev_join, side));
-        AP_HIP(hipStreamWaitEvent(s, capturing ? cap_join : ev_join, 0));
-    }
+    if (two) AP_HIP(hipStreamWaitEvent(s, capturing ? cap_cells : ev_cells, 0));
     mark(AP_PROF_GATHER, s);
     tbegin(AP_PROF_GATHER, s);
     hipLaunchKernelGGL(eng_gather, dim3(grid_of(d.num_nodes, gather_lanes)), dim3(gather_lanes), 0, s, g, p_turned,
                        p_wl2, p_dn2);
     AP_LAUNCHED();
     tend(AP_PROF_GATHER, s);
+    if (two) {
+        AP_HIP(hipEventRecord(capturing ? cap_join : ev_join, side));
+        AP_HIP(hipStreamWaitEvent(s, capturing ? cap_join : ev_join, 0));
+    }
     mark(AP_PROF_CONTROL, s);
     tbegin(AP_PROF_CONTROL, s);
     hipLaunchKernelGGL(eng_reduce, dim3(RED_BLOCKS), dim3(RED_LANES), 0, s, segs(with_state, g_prev != nullptr, use_density),
